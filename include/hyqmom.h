@@ -1,0 +1,144 @@
+/*
+ * hyqmom.h -- C ABI of libhyqmom_hip.so, the MI355X (gfx950) implementation of the
+ * HyQMOM.jl 3D moment-solver time step.
+ *
+ * The reference (comp-physics/HyQMOM.jl) has no FFI of its own; its public boundary for
+ * this path is the Julia API
+ *     simulation_runner(params)                 src/simulation_runner.jl:63-759
+ *     run_simulation_with_snapshots(params; ..) src/simulation_runner.jl:994-1011
+ * whose step loop (src/simulation_runner.jl:346-560) these entry points replace.  A Julia
+ * caller keeps that API and binds the functions below with `ccall`.
+ *
+ * Conventions
+ *   - every function returns HYQ_OK (0) or a negative HYQ_E* code; nothing throws or
+ *     aborts across the ABI; hyq_last_error() gives the text of the last failure on the
+ *     calling thread;
+ *   - host state buffers use the reference's own layout: Julia column-major
+ *     (Nx, Ny, Nz, 35), x fastest, moment index slowest;
+ *   - "batch" entry points take DEVICE pointers to structure-of-arrays planes
+ *     ([35][n], plane m at ptr + m*n) and run on the default stream of the current
+ *     device; they synchronize before returning;
+ *   - one context per host thread; calls on one context must be serialized.
+ *
+ * The 35-moment order is the reference's (Flux_closure35_and_realizable_3D.jl:18-20):
+ *   M000 M100 M200 M300 M400 M010 M110 M210 M310 M020 M120 M220 M030 M130 M040
+ *   M001 M101 M201 M301 M002 M102 M202 M003 M103 M004 M011 M111 M211 M021 M121
+ *   M031 M012 M112 M013 M022
+ */
+#ifndef HYQMOM_H
+#define HYQMOM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes */
+#define HYQ_OK 0
+#define HYQ_EINVAL (-1)      /* invalid argument (sizes, null pointers, Nmom != 35) */
+#define HYQ_ENOMEM (-2)      /* device allocation failed */
+#define HYQ_EHIP (-3)        /* a HIP runtime call failed */
+#define HYQ_ENONFINITE (-4)  /* non-finite Jacobian in the z wave-speed solve: the
+                                reference throws there (eigenvalues6z_hyperbolic_3D.jl:108,121) */
+#define HYQ_ENOTCONV (-5)    /* an eigenvalue iteration did not converge */
+#define HYQ_ENODEV (-6)      /* no HIP device */
+
+/* per-cell branch flags (bitmask), accumulated over one pass-1 evaluation */
+#define HYQ_FLAG_UNIVARIATE (1u << 0)   /* enforce_univariate changed S3/S4 */
+#define HYQ_FLAG_EDGE_CORNER (1u << 1)  /* edge_corner_correction branch */
+#define HYQ_FLAG_FACES (1u << 2)        /* realizable_3D faces branch (S2 <= 1e-12) */
+#define HYQ_FLAG_FLAG220 (1u << 3)      /* the 6-pass flag220 loop ran */
+#define HYQ_FLAG_ROOTS2D (1u << 4)      /* realizable_2D L3 < 0 root bound evaluated */
+#define HYQ_FLAG_ROOTS4 (1u << 5)       /* a 4x4 minor of Delta2* was negative */
+#define HYQ_FLAG_EIGCORR_X (1u << 6)    /* complex x eigenvalues -> moment correction */
+#define HYQ_FLAG_EIGCORR_Y (1u << 7)
+#define HYQ_FLAG_EIGCORR_Z (1u << 8)
+#define HYQ_FLAG_ZNEGVAR (1u << 9)      /* negative-variance re-projection before z eigs */
+#define HYQ_FLAG_NONFINITE (1u << 10)   /* non-finite z Jacobian */
+#define HYQ_FLAG_RHOCLAMP (1u << 11)    /* density clamp in the z eigen path */
+#define HYQ_FLAG_EIG_FALLBACK (1u << 12) /* learned warm start fell back to the full solve */
+#define HYQ_FLAG_QR_NOCONV (1u << 13)   /* QR iteration limit reached */
+
+/* precision modes */
+#define HYQ_PREC_FP64 0           /* everything FP64 */
+#define HYQ_PREC_MIXED 1          /* FP32 fluxes, HLL update and BGK relaxation */
+#define HYQ_PREC_MIXED_LEARNED 2  /* MIXED + MLP warm-started wave speeds */
+
+/* Mirrors the params NamedTuple of simulation_runner (src/simulation_runner.jl:20-37,
+ * 74-113).  Fields the device step does not need (snapshot, symmetry, debug, MPI) stay
+ * on the host side. */
+typedef struct hyq_params {
+  int32_t Nx, Ny, Nz;
+  int32_t Nmom;       /* must be 35 */
+  int32_t flag2D;
+  int32_t positivity; /* POSITIVITY_ENABLED (src/HyQMOM.jl:20) */
+  int32_t precision;  /* HYQ_PREC_* */
+  int32_t reserved;
+  double tmax, Kn, Ma, CFL, dtmax;
+  double xmin, xmax, ymin, ymax, zmin, zmax;
+} hyq_params;
+
+typedef struct hyq_ctx hyq_ctx;
+
+/* Context: device SoA planes, stream and scratch for one run. */
+int hyq_create(const hyq_params* params, hyq_ctx** out);
+int hyq_destroy(hyq_ctx* ctx);
+
+/* Upload / download the (Nx,Ny,Nz,35) state (host, Julia column-major). */
+int hyq_set_state(hyq_ctx* ctx, const double* M);
+int hyq_get_state(hyq_ctx* ctx, double* M);
+
+/* One step of the reference loop body (simulation_runner.jl:346-560): pass 1 (closure,
+ * wave speeds), dt = min(CFL*min(dx,dy,dz)/vmax, min(dtmax,Kn), tmax - t), x/y/z HLL
+ * updates combined as Mx+My+Mz-2M, pass 2 realizability and BGK.  t and nn advance as
+ * in the reference. */
+int hyq_step(hyq_ctx* ctx, double* t_inout, double* dt_out, int64_t* nn_inout);
+
+/* while t < tmax && nn < nnmax: hyq_step.  The loop condition of
+ * simulation_runner.jl:346. */
+int hyq_run(hyq_ctx* ctx, double tmax, int64_t nnmax, double* t_inout, int64_t* nn_inout);
+
+/* Pass-1 outputs of the last step: wave speeds [6][N] (xmin,xmax,ymin,ymax,zmin,zmax; the
+ * reference's vp{x,y,z}{min,max}, simulation_runner.jl:374-379) and branch flags [N]. */
+int hyq_get_wave_speeds(hyq_ctx* ctx, double* vp);
+int hyq_get_flags(hyq_ctx* ctx, uint32_t* flags);
+
+/* Average device time per kernel over the steps since the last reset (HIP events on the
+ * context stream).  names: "pass1", "update_pass2".  ms_out[i] for i < n. */
+int hyq_kernel_times(hyq_ctx* ctx, double* ms_pass1, double* ms_update, int64_t* nsteps);
+int hyq_reset_kernel_times(hyq_ctx* ctx);
+
+/* Batched per-cell kernels on device SoA planes ([35][n] doubles). */
+
+/* Flux_closure35_and_realizable_3D (src/numerics/Flux_closure35_and_realizable_3D.jl:33-234)
+ * for n independent cells: realizable Mr and fluxes Fx, Fy, Fz (any output may be NULL). */
+int hyq_closure_batch(const double* M, int64_t n, double Ma, int32_t flag2D, double* Mr,
+                      double* Fx, double* Fy, double* Fz, uint32_t* flags);
+
+/* The pass-1 cell pipeline of simulation_runner.jl:352-380 for n independent cells:
+ * closure -> x/y/z wave speeds (eigenvalues6_hyperbolic_3D.jl, eigenvalues6z_hyperbolic_3D.jl)
+ * -> closure -> 1D closures (closure_and_eigenvalues.jl).  vp is [6][n]. */
+int hyq_wavespeed_batch(const double* M, int64_t n, double Ma, int32_t flag2D,
+                        int32_t positivity, int32_t precision, double* Mr, double* Fx,
+                        double* Fy, double* Fz, double* vp, uint32_t* flags);
+
+/* eigvals of jacobian6 (src/autogen/jacobian6.jl, compute_jacobian_eigenvalues.jl:18-50)
+ * for n 15-moment slices ([15][n]); outputs the 6 eigenvalues sorted by real part
+ * ([6][n] real, [6][n] imaginary). */
+int hyq_eig6_batch(const double* m15, int64_t n, double* wr, double* wi);
+
+/* collision35 (src/numerics/collision35.jl:17-45) on n cells. */
+int hyq_collision_batch(const double* M, int64_t n, double dt, double Kn, int32_t positivity,
+                        double* Mout);
+
+/* Text of the last error on the calling thread ("" if none). */
+const char* hyq_last_error(void);
+
+/* Library version string. */
+const char* hyq_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HYQMOM_H */
