@@ -1,0 +1,578 @@
+// HIP kernels and the C ABI of libhyqmom_hip.so (declared in include/hyqmom.h).
+//
+// Device layout: the state is 35 structure-of-arrays moment planes of N = Nx*Ny*Nz
+// doubles, cell index i + Nx*(j + Ny*k) -- the reference's own (Nx,Ny,Nz,35) Julia layout,
+// so host upload/download is a straight copy.  One step is four launches on the context
+// stream, none of which waits on the host:
+//   k_step_begin  : active = (t < tmax); nn += active                      (1 thread)
+//   k_pass1       : per cell: closure -> x/y/z wave speeds -> closure + fluxes -> 1D
+//                   speeds; writes Mr, Fx/Fy/Fz, vp, flags; wave max |vp| -> atomic max
+//   k_dt          : dt = min(CFL*min(dx,dy,dz)/vmax, min(dtmax,Kn), tmax-t); t += dt
+//   k_update      : per cell: x/y/z HLL updates with the reference's boundary rules,
+//                   combined as Mx+My+Mz-2M
+//   k_pass2       : per cell: closure -> x/y wave-speed corrections -> closure -> BGK
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hyqmom.h"
+#include "hyq_pipeline.h"
+
+using namespace hyq;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* what, hipError_t e = hipSuccess) {
+  g_err = what;
+  if (e != hipSuccess) {
+    g_err += ": ";
+    g_err += hipGetErrorString(e);
+  }
+  return code;
+}
+#define HIPCHK(x, msg)                                   \
+  do {                                                   \
+    hipError_t _e = (x);                                 \
+    if (_e != hipSuccess) return fail(HYQ_EHIP, msg, _e); \
+  } while (0)
+
+constexpr int kBlock = 128;
+
+struct DevScalars {  // device-resident step scalars
+  double t, dt, tmax, active;
+  unsigned long long vmax_bits;
+  long long nn, nnmax, pad;
+};
+
+// ---- pass 1 -------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long w = __shfl_xor(v, o);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+template <typename FT>
+__global__ __launch_bounds__(kBlock) void k_pass1(const double* __restrict__ M,
+                                                  double* __restrict__ Mr, FT* __restrict__ F,
+                                                  FT* __restrict__ vp, uint32_t* __restrict__ flags,
+                                                  DevScalars* sc, long long N, int flag2D,
+                                                  double Ma, int positivity) {
+  if (sc != nullptr && sc->active == 0.0) return;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long vb = 0;
+  if (c < N) {
+    double m[35], mr[35], v[6];
+#pragma unroll
+    for (int n = 0; n < 35; ++n) m[n] = M[(size_t)n * N + c];
+    uint32_t fl = 0;
+    pass1_cell<FT>(m, flag2D, Ma, positivity != 0, mr, F + c, F + (size_t)35 * N + c,
+                   F + (size_t)70 * N + c, (size_t)N, v, fl);
+#pragma unroll
+    for (int n = 0; n < 35; ++n) Mr[(size_t)n * N + c] = mr[n];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+      vp[(size_t)d * N + c] = (FT)v[d];
+      const unsigned long long b = __double_as_longlong(fabs(v[d]));
+      vb = b > vb ? b : vb;  // |vp| >= 0: bit order = value order; NaN sorts above +Inf
+    }
+    if (flags) flags[c] = fl;
+  }
+  if (sc != nullptr) {
+    vb = wave_max_u64(vb);
+    if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
+  }
+}
+
+// ---- dt (simulation_runner.jl:402-433, :496) -------------------------------------------
+__global__ void k_step_begin(DevScalars* sc) {
+  const bool act = (sc->t < sc->tmax) && (sc->nn < sc->nnmax);
+  sc->active = act ? 1.0 : 0.0;
+  if (act) sc->nn += 1;
+  sc->vmax_bits = 0ull;
+}
+__global__ void k_dt(DevScalars* sc, double CFL, double hmin, double dtmax, double Kn) {
+  if (sc->active == 0.0) return;
+  const double vmax = __longlong_as_double((long long)sc->vmax_bits);
+  double dt = jmin(CFL * hmin / vmax, jmin(dtmax, Kn));
+  dt = jmin(dt, sc->tmax - sc->t);
+  sc->dt = dt;
+  sc->t += dt;
+}
+
+// ---- HLL update (pas_HLL.jl:24-84, flux_HLL.jl:20-32, apply_flux_update_3d.jl) ---------
+template <typename FT>
+struct Line {
+  const double* M;
+  const FT* F;
+  const FT* lo;
+  const FT* hi;
+  long long st;  // stride between consecutive cells of the line
+  int n;
+  __device__ double m(int q) const { return M[(long long)q * st]; }
+  __device__ double f(int q) const { return (double)F[(long long)q * st]; }
+  __device__ double vl(int q) const { return (double)lo[(long long)q * st]; }
+  __device__ double vh(int q) const { return (double)hi[(long long)q * st]; }
+  // intermediate HLL state of face q (between cells q and q+1), F unmodified
+  __device__ double wstar(int q, double ll, double lr) const {
+    if (fabs(ll - lr) > 1e-10)
+      return (ll * m(q) - lr * m(q + 1)) / (ll - lr) - (f(q) - f(q + 1)) / (ll - lr);
+    return 0.0;
+  }
+  // numerical flux of face q, 0 <= q <= n-2, with the physical-BC copies at both ends
+  __device__ double flux(int q) const {
+    if (q == 0) {
+      // lleft[1] = lright[1] = 0; F[1] := F[2]; Wstar[1] := Wstar[2]
+      double ws = 0.0;
+      if (n >= 3) {
+        const double ll = jmin(vl(1), vl(2)), lr = jmax(vh(1), vh(2));
+        ws = wstar(1, ll, lr);
+      }
+      const double f1 = f(1);
+      return 0.5 * (f1 + f1) - 0.5 * (0.0 * (ws - m(0)) - 0.0 * (ws - m(1)));
+    }
+    const double ll = jmin(vl(q), vl(q + 1)), lr = jmax(vh(q), vh(q + 1));
+    const double ws = wstar(q, ll, lr);
+    const double fa = f(q), fb = (q == n - 2) ? f(q) : f(q + 1);  // F[Np] := F[Np-1]
+    return 0.5 * (fa + fb) - 0.5 * (fabs(ll) * (ws - m(q)) - fabs(lr) * (ws - m(q + 1)));
+  }
+  // Mp at cell s of the line
+  __device__ double updated(int s, double dtdx) const {
+    if (n == 1) return m(s);
+    if (n == 2) return m(1);
+    const int j = s < 1 ? 1 : (s > n - 2 ? n - 2 : s);
+    return m(j) - dtdx * (flux(j) - flux(j - 1));
+  }
+};
+
+template <typename FT>
+__global__ __launch_bounds__(kBlock) void k_update(const double* __restrict__ Mr,
+                                                   const FT* __restrict__ F,
+                                                   const FT* __restrict__ vp,
+                                                   double* __restrict__ Mout, const DevScalars* sc,
+                                                   int Nx, int Ny, int Nz, double dx, double dy,
+                                                   double dz) {
+  if (sc->active == 0.0) return;
+  const long long N = (long long)Nx * Ny * Nz;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const int i = (int)(c % Nx), j = (int)((c / Nx) % Ny), k = (int)(c / ((long long)Nx * Ny));
+  const double dt = sc->dt;
+  const long long bx = c - i, by = c - (long long)j * Nx, bz = c - (long long)k * Nx * Ny;
+  for (int n = 0; n < 35; ++n) {
+    const double* Mn = Mr + (size_t)n * N;
+    Line<FT> lx{Mn + bx, F + (size_t)n * N + bx, vp + 0 * N + bx, vp + 1 * (size_t)N + bx, 1, Nx};
+    Line<FT> ly{Mn + by, F + (size_t)(35 + n) * N + by, vp + 2 * (size_t)N + by,
+                vp + 3 * (size_t)N + by, Nx, Ny};
+    Line<FT> lz{Mn + bz, F + (size_t)(70 + n) * N + bz, vp + 4 * (size_t)N + bz,
+                vp + 5 * (size_t)N + bz, (long long)Nx * Ny, Nz};
+    const double m0 = Mn[c];
+    const double mx = lx.updated(i, dt / dx);
+    const double my = ly.updated(j, dt / dy);
+    const double mz = (Nz == 1) ? m0 : lz.updated(k, dt / dz);  // apply_flux_update_3d.jl:183
+    Mout[(size_t)n * N + c] = mx + my + mz - 2.0 * m0;
+  }
+}
+
+// ---- pass 2 + BGK ----------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_pass2(double* __restrict__ M, const DevScalars* sc,
+                                                  long long N, int flag2D, double Ma,
+                                                  int positivity, double Kn) {
+  if (sc->active == 0.0) return;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  double m[35], o[35];
+#pragma unroll
+  for (int n = 0; n < 35; ++n) m[n] = M[(size_t)n * N + c];
+  uint32_t fl = 0;
+  pass2_cell(m, flag2D, Ma, positivity != 0, sc->dt, Kn, o, fl);
+#pragma unroll
+  for (int n = 0; n < 35; ++n) M[(size_t)n * N + c] = o[n];
+}
+
+// ---- batch kernels ---------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_closure_batch(const double* __restrict__ M, long long N,
+                                                          double Ma, int flag2D, double* Mr,
+                                                          double* Fx, double* Fy, double* Fz,
+                                                          uint32_t* flags) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  double m[35], mr[35];
+#pragma unroll
+  for (int n = 0; n < 35; ++n) m[n] = M[(size_t)n * N + c];
+  uint32_t fl = 0;
+  if (Fx != nullptr && Fy != nullptr && Fz != nullptr) {
+    flux_closure35<true>(m, flag2D, Ma, mr, Fx + c, Fy + c, Fz + c, (size_t)N, fl);
+  } else {
+    flux_closure35<false, double>(m, flag2D, Ma, mr, nullptr, nullptr, nullptr, 0, fl);
+  }
+  if (Mr)
+#pragma unroll
+    for (int n = 0; n < 35; ++n) Mr[(size_t)n * N + c] = mr[n];
+  if (flags) flags[c] = fl;
+}
+
+__global__ __launch_bounds__(kBlock) void k_collision_batch(const double* __restrict__ M,
+                                                            long long N, double dt, double Kn,
+                                                            int positivity, double* Mo) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  double m[35], o[35];
+#pragma unroll
+  for (int n = 0; n < 35; ++n) m[n] = M[(size_t)n * N + c];
+  collision_cell(m, dt, Kn, positivity != 0, o);
+#pragma unroll
+  for (int n = 0; n < 35; ++n) Mo[(size_t)n * N + c] = o[n];
+}
+
+// eigvals of jacobian6 for a 15-moment slice (jacobian6.jl argument order), sorted by real
+// part; computed on the standardized plane and mapped back.
+__global__ __launch_bounds__(kBlock) void k_eig6_batch(const double* __restrict__ m15, long long N,
+                                                       double* wr_out, double* wi_out) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  double r[15];
+#pragma unroll
+  for (int t = 0; t < 15; ++t) r[t] = m15[(size_t)t * N + c];
+  // raw plane moments R[i][j] = m_ij / m00 (i: flux direction)
+  double R[5][5];
+#pragma unroll
+  for (int a = 0; a < 5; ++a)
+#pragma unroll
+    for (int b = 0; b < 5; ++b) R[a][b] = 0.0;
+  const double i0 = 1.0 / r[0];
+  R[0][0] = 1.0; R[0][1] = r[1] * i0; R[0][2] = r[2] * i0; R[0][3] = r[3] * i0; R[0][4] = r[4] * i0;
+  R[1][0] = r[5] * i0; R[1][1] = r[6] * i0; R[1][2] = r[7] * i0; R[1][3] = r[8] * i0;
+  R[2][0] = r[9] * i0; R[2][1] = r[10] * i0; R[2][2] = r[11] * i0; R[3][0] = r[12] * i0;
+  R[3][1] = r[13] * i0; R[4][0] = r[14] * i0;
+  // embed as a 3D packed array with the third axis empty and shift to central moments
+  double cc[35];
+#pragma unroll
+  for (int t = 0; t < 35; ++t) cc[t] = 0.0;
+#pragma unroll
+  for (int a = 0; a < 5; ++a)
+#pragma unroll
+    for (int b = 0; a + b < 5; ++b) cc[tpos(a, b, 0)] = R[a][b];
+  const double a = R[1][0], b = R[0][1];
+  taylor_shift<0, 4>(cc, -a);
+  taylor_shift<1, 4>(cc, -b);
+  Plane p = plane<0, 1>(cc, a, b, 0.0);
+  double H[36];
+  jacobian_std(p, H);
+  double wr[6], wi[6];
+  bool fin = p.ok;
+#pragma unroll
+  for (int t = 0; t < 36; ++t) fin = fin && (fabs(H[t]) < 1e300);
+  if (fin) {
+    balance_n<6>(H);
+    hessenberg_n<6>(H);
+    fin = hqr_n<6>(H, wr, wi);
+  }
+  for (int t = 0; t < 6; ++t) {
+    if (!fin) { wr[t] = NAN; wi[t] = NAN; }
+    else { wr[t] = p.a + p.sig * wr[t]; wi[t] = p.sig * wi[t]; }
+  }
+  // insertion sort by real part
+  for (int x = 1; x < 6; ++x)
+    for (int y = x; y > 0 && wr[y] < wr[y - 1]; --y) {
+      double tt = wr[y]; wr[y] = wr[y - 1]; wr[y - 1] = tt;
+      tt = wi[y]; wi[y] = wi[y - 1]; wi[y - 1] = tt;
+    }
+  for (int t = 0; t < 6; ++t) {
+    wr_out[(size_t)t * N + c] = wr[t];
+    wi_out[(size_t)t * N + c] = wi[t];
+  }
+}
+
+unsigned grid_for(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// Context
+struct hyq_ctx {
+  hyq_params p;
+  long long N;
+  double dx, dy, dz;
+  hipStream_t stream;
+  double* M;       // [35][N]
+  double* Mr;      // [35][N]
+  void* F;         // [105][N] double (FP64) or float (mixed)
+  void* vp;        // [6][N]
+  uint32_t* flags; // [N]
+  DevScalars* sc;
+  hipEvent_t ev[4];
+  double ms_pass1, ms_update;
+  long long timed_steps;
+};
+
+extern "C" {
+
+const char* hyq_last_error(void) { return g_err.c_str(); }
+const char* hyq_version(void) { return "hyqmom-mi355x 0.1 (gfx950)"; }
+
+int hyq_create(const hyq_params* params, hyq_ctx** out) {
+  if (!params || !out) return fail(HYQ_EINVAL, "null argument");
+  const hyq_params& p = *params;
+  if (p.Nmom != 35) return fail(HYQ_EINVAL, "Nmom must be 35");
+  if (p.Nx < 1 || p.Ny < 1 || p.Nz < 1) return fail(HYQ_EINVAL, "grid sizes must be >= 1");
+  if (p.precision < HYQ_PREC_FP64 || p.precision > HYQ_PREC_MIXED_LEARNED)
+    return fail(HYQ_EINVAL, "unknown precision mode");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(HYQ_ENODEV, "no HIP device");
+  hyq_ctx* c = new hyq_ctx();
+  std::memset(c, 0, sizeof(*c));
+  c->p = p;
+  c->N = (long long)p.Nx * p.Ny * p.Nz;
+  c->dx = (p.xmax - p.xmin) / p.Nx;  // simulation_runner.jl:133-139
+  c->dy = (p.ymax - p.ymin) / p.Ny;
+  c->dz = (p.zmax - p.zmin) / p.Nz;
+  const size_t fsz = p.precision == HYQ_PREC_FP64 ? sizeof(double) : sizeof(float);
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
+    delete c;
+    return fail(HYQ_EHIP, "hipStreamCreate", e);
+  }
+  bool ok = hipMalloc(&c->M, sizeof(double) * 35 * c->N) == hipSuccess &&
+            hipMalloc(&c->Mr, sizeof(double) * 35 * c->N) == hipSuccess &&
+            hipMalloc(&c->F, fsz * 105 * c->N) == hipSuccess &&
+            hipMalloc(&c->vp, fsz * 6 * c->N) == hipSuccess &&
+            hipMalloc(&c->flags, sizeof(uint32_t) * c->N) == hipSuccess &&
+            hipMalloc(&c->sc, sizeof(DevScalars)) == hipSuccess;
+  if (!ok) {
+    hyq_destroy(c);
+    return fail(HYQ_ENOMEM, "device allocation failed");
+  }
+  for (int i = 0; i < 4; ++i) hipEventCreate(&c->ev[i]);
+  *out = c;
+  return HYQ_OK;
+}
+
+int hyq_destroy(hyq_ctx* c) {
+  if (!c) return HYQ_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  hipFree(c->M);
+  hipFree(c->Mr);
+  hipFree(c->F);
+  hipFree(c->vp);
+  hipFree(c->flags);
+  hipFree(c->sc);
+  for (int i = 0; i < 4; ++i)
+    if (c->ev[i]) hipEventDestroy(c->ev[i]);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return HYQ_OK;
+}
+
+int hyq_set_state(hyq_ctx* c, const double* M) {
+  if (!c || !M) return fail(HYQ_EINVAL, "null argument");
+  HIPCHK(hipMemcpyAsync(c->M, M, sizeof(double) * 35 * c->N, hipMemcpyHostToDevice, c->stream),
+         "set_state copy");
+  HIPCHK(hipStreamSynchronize(c->stream), "set_state sync");
+  return HYQ_OK;
+}
+
+int hyq_get_state(hyq_ctx* c, double* M) {
+  if (!c || !M) return fail(HYQ_EINVAL, "null argument");
+  HIPCHK(hipMemcpyAsync(M, c->M, sizeof(double) * 35 * c->N, hipMemcpyDeviceToHost, c->stream),
+         "get_state copy");
+  HIPCHK(hipStreamSynchronize(c->stream), "get_state sync");
+  return HYQ_OK;
+}
+
+static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
+  const hyq_params& p = c->p;
+  const unsigned g = grid_for(c->N);
+  const double hmin = jmin(jmin(c->dx, c->dy), c->dz);
+  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, c->stream, c->sc);
+  if (ev) hipEventRecord(ev[0], c->stream);
+  if (p.precision == HYQ_PREC_FP64) {
+    hipLaunchKernelGGL(k_pass1<double>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
+                       (double*)c->F, (double*)c->vp, c->flags, c->sc, c->N, p.flag2D, p.Ma,
+                       p.positivity);
+  } else {
+    hipLaunchKernelGGL(k_pass1<float>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
+                       (float*)c->F, (float*)c->vp, c->flags, c->sc, c->N, p.flag2D, p.Ma,
+                       p.positivity);
+  }
+  if (ev) hipEventRecord(ev[1], c->stream);
+  hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, c->stream, c->sc, p.CFL, hmin, p.dtmax, p.Kn);
+  if (ev) hipEventRecord(ev[2], c->stream);
+  if (p.precision == HYQ_PREC_FP64) {
+    hipLaunchKernelGGL(k_update<double>, dim3(g), dim3(kBlock), 0, c->stream, c->Mr,
+                       (const double*)c->F, (const double*)c->vp, c->M, c->sc, p.Nx, p.Ny, p.Nz,
+                       c->dx, c->dy, c->dz);
+  } else {
+    hipLaunchKernelGGL(k_update<float>, dim3(g), dim3(kBlock), 0, c->stream, c->Mr,
+                       (const float*)c->F, (const float*)c->vp, c->M, c->sc, p.Nx, p.Ny, p.Nz,
+                       c->dx, c->dy, c->dz);
+  }
+  hipLaunchKernelGGL(k_pass2, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->sc, c->N, p.flag2D,
+                     p.Ma, p.positivity, p.Kn);
+  if (ev) hipEventRecord(ev[3], c->stream);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(HYQ_EHIP, "kernel launch", e);
+  return HYQ_OK;
+}
+
+static int load_scalars(hyq_ctx* c, double t, long long nn, double tmax, long long nnmax) {
+  DevScalars s;
+  std::memset(&s, 0, sizeof(s));
+  s.t = t;
+  s.tmax = tmax;
+  s.nn = nn;
+  s.nnmax = nnmax;
+  HIPCHK(hipMemcpyAsync(c->sc, &s, sizeof(s), hipMemcpyHostToDevice, c->stream), "scalars");
+  return HYQ_OK;
+}
+
+static int accumulate_times(hyq_ctx* c, hipEvent_t* ev) {
+  float a = 0, b = 0;
+  HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]), "event time");
+  HIPCHK(hipEventElapsedTime(&b, ev[2], ev[3]), "event time");
+  c->ms_pass1 += a;
+  c->ms_update += b;
+  c->timed_steps += 1;
+  return HYQ_OK;
+}
+
+int hyq_step(hyq_ctx* c, double* t_inout, double* dt_out, int64_t* nn_inout) {
+  if (!c || !t_inout || !nn_inout) return fail(HYQ_EINVAL, "null argument");
+  // one step regardless of tmax (the caller owns the loop condition)
+  // one pass of the loop body; a no-op if t >= tmax (the caller owns the loop condition)
+  int rc = load_scalars(c, *t_inout, *nn_inout, c->p.tmax, (long long)*nn_inout + 1);
+  if (rc) return rc;
+  rc = enqueue_step(c, c->ev);
+  if (rc) return rc;
+  DevScalars s;
+  HIPCHK(hipMemcpyAsync(&s, c->sc, sizeof(s), hipMemcpyDeviceToHost, c->stream), "scalars back");
+  HIPCHK(hipStreamSynchronize(c->stream), "step sync");
+  if (s.active != 0.0) {
+    rc = accumulate_times(c, c->ev);
+    if (rc) return rc;
+  }
+  *t_inout = s.t;
+  if (dt_out) *dt_out = s.active != 0.0 ? s.dt : 0.0;
+  *nn_inout = s.nn;
+  return HYQ_OK;
+}
+
+int hyq_run(hyq_ctx* c, double tmax, int64_t nnmax, double* t_inout, int64_t* nn_inout) {
+  if (!c || !t_inout || !nn_inout) return fail(HYQ_EINVAL, "null argument");
+  int rc = load_scalars(c, *t_inout, *nn_inout, tmax, nnmax);
+  if (rc) return rc;
+  // launch nnmax - nn steps back to back; steps after t >= tmax are no-ops on the device.
+  // HIP events bracket the pass-1 kernel and the update+pass-2 pair of every step.
+  const long long todo = (long long)nnmax - *nn_inout;
+  std::vector<hipEvent_t> ev(4 * (size_t)(todo > 0 ? todo : 0));
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e), "event create");
+  for (long long s = 0; s < todo; ++s) {
+    rc = enqueue_step(c, &ev[4 * s]);
+    if (rc) break;
+  }
+  DevScalars s;
+  if (rc == HYQ_OK) {
+    HIPCHK(hipMemcpyAsync(&s, c->sc, sizeof(s), hipMemcpyDeviceToHost, c->stream), "scalars back");
+    HIPCHK(hipStreamSynchronize(c->stream), "run sync");
+    const long long ran = s.nn - *nn_inout;
+    for (long long q = 0; q < ran && rc == HYQ_OK; ++q) rc = accumulate_times(c, &ev[4 * q]);
+  }
+  for (auto& e : ev) hipEventDestroy(e);
+  if (rc) return rc;
+  *t_inout = s.t;
+  *nn_inout = s.nn;
+  return HYQ_OK;
+}
+
+int hyq_get_wave_speeds(hyq_ctx* c, double* vp) {
+  if (!c || !vp) return fail(HYQ_EINVAL, "null argument");
+  if (c->p.precision == HYQ_PREC_FP64) {
+    HIPCHK(hipMemcpy(vp, c->vp, sizeof(double) * 6 * c->N, hipMemcpyDeviceToHost), "vp copy");
+  } else {
+    float* tmp = new float[6 * c->N];
+    const hipError_t e = hipMemcpy(tmp, c->vp, sizeof(float) * 6 * c->N, hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+      for (long long i = 0; i < 6 * c->N; ++i) vp[i] = tmp[i];
+    delete[] tmp;
+    HIPCHK(e, "vp copy");
+  }
+  return HYQ_OK;
+}
+
+int hyq_get_flags(hyq_ctx* c, uint32_t* flags) {
+  if (!c || !flags) return fail(HYQ_EINVAL, "null argument");
+  HIPCHK(hipMemcpy(flags, c->flags, sizeof(uint32_t) * c->N, hipMemcpyDeviceToHost), "flags copy");
+  return HYQ_OK;
+}
+
+int hyq_kernel_times(hyq_ctx* c, double* ms_pass1, double* ms_update, int64_t* nsteps) {
+  if (!c) return fail(HYQ_EINVAL, "null argument");
+  const double n = c->timed_steps > 0 ? (double)c->timed_steps : 1.0;
+  if (ms_pass1) *ms_pass1 = c->ms_pass1 / n;
+  if (ms_update) *ms_update = c->ms_update / n;
+  if (nsteps) *nsteps = c->timed_steps;
+  return HYQ_OK;
+}
+
+int hyq_reset_kernel_times(hyq_ctx* c) {
+  if (!c) return fail(HYQ_EINVAL, "null argument");
+  c->ms_pass1 = c->ms_update = 0.0;
+  c->timed_steps = 0;
+  return HYQ_OK;
+}
+
+int hyq_closure_batch(const double* M, int64_t n, double Ma, int32_t flag2D, double* Mr,
+                      double* Fx, double* Fy, double* Fz, uint32_t* flags) {
+  if (!M || n < 0) return fail(HYQ_EINVAL, "bad argument");
+  if (n == 0) return HYQ_OK;
+  hipLaunchKernelGGL(k_closure_batch, dim3(grid_for(n)), dim3(kBlock), 0, 0, M, (long long)n, Ma,
+                     flag2D, Mr, Fx, Fy, Fz, flags);
+  HIPCHK(hipGetLastError(), "closure launch");
+  HIPCHK(hipDeviceSynchronize(), "closure sync");
+  return HYQ_OK;
+}
+
+int hyq_wavespeed_batch(const double* M, int64_t n, double Ma, int32_t flag2D,
+                        int32_t positivity, int32_t precision, double* Mr, double* Fx,
+                        double* Fy, double* Fz, double* vp, uint32_t* flags) {
+  if (!M || !Mr || !Fx || !Fy || !Fz || !vp || n < 0) return fail(HYQ_EINVAL, "bad argument");
+  if (precision != HYQ_PREC_FP64) return fail(HYQ_EINVAL, "batch wave speeds are FP64");
+  if (n == 0) return HYQ_OK;
+  // the three flux outputs must be one [105][n] block for the pass-1 kernel
+  if (Fy != Fx + (size_t)35 * n || Fz != Fx + (size_t)70 * n)
+    return fail(HYQ_EINVAL, "Fx, Fy, Fz must be consecutive [35][n] blocks");
+  hipLaunchKernelGGL(k_pass1<double>, dim3(grid_for(n)), dim3(kBlock), 0, 0, M, Mr, Fx, vp, flags,
+                     (DevScalars*)nullptr, (long long)n, flag2D, Ma, positivity);
+  HIPCHK(hipGetLastError(), "pass1 launch");
+  HIPCHK(hipDeviceSynchronize(), "pass1 sync");
+  return HYQ_OK;
+}
+
+int hyq_eig6_batch(const double* m15, int64_t n, double* wr, double* wi) {
+  if (!m15 || !wr || !wi || n < 0) return fail(HYQ_EINVAL, "bad argument");
+  if (n == 0) return HYQ_OK;
+  hipLaunchKernelGGL(k_eig6_batch, dim3(grid_for(n)), dim3(kBlock), 0, 0, m15, (long long)n, wr, wi);
+  HIPCHK(hipGetLastError(), "eig6 launch");
+  HIPCHK(hipDeviceSynchronize(), "eig6 sync");
+  return HYQ_OK;
+}
+
+int hyq_collision_batch(const double* M, int64_t n, double dt, double Kn, int32_t positivity,
+                        double* Mout) {
+  if (!M || !Mout || n < 0) return fail(HYQ_EINVAL, "bad argument");
+  if (n == 0) return HYQ_OK;
+  hipLaunchKernelGGL(k_collision_batch, dim3(grid_for(n)), dim3(kBlock), 0, 0, M, (long long)n, dt,
+                     Kn, positivity, Mout);
+  HIPCHK(hipGetLastError(), "collision launch");
+  HIPCHK(hipDeviceSynchronize(), "collision sync");
+  return HYQ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,215 @@
+// The per-cell passes of one time step (simulation_runner.jl:352-380 and :526-556).
+#pragma once
+#include "hyq_eig.h"
+
+namespace hyq {
+
+// correct_{uv,uw,vw}_plane (eigenvalues6_hyperbolic_3D.jl:181-252, and the inlined copies
+// in eigenvalues6z_hyperbolic_3D.jl:225-283) on packed central moments c0 (kept) -> c.
+// Plane (P, Q): cross moments of orders 3 and 4 rebuilt from the plane correlation.
+template <int P, int Q>
+HD void correct_plane(const double* c0, double* c) {
+  auto id = [](int i, int j) {
+    int e[3] = {0, 0, 0};
+    e[P] += i;
+    e[Q] += j;
+    return tpos(e[0], e[1], e[2]);
+  };
+  const double sp = sqrt(jmax(EPS, c0[id(2, 0)])), sq = sqrt(jmax(EPS, c0[id(0, 2)]));
+  // standardized values as M2CS4_35 computes them: sqrt(max(C, eps))
+  const double Spq = c0[id(1, 1)] / (sp * sq), Sp3 = c0[id(3, 0)] / (sp * sp * sp),
+               Sq3 = c0[id(0, 3)] / (sq * sq * sq), Sp4 = c0[id(4, 0)] / (sp * sp * sp * sp),
+               Sq4 = c0[id(0, 4)] / (sq * sq * sq * sq);
+  double S22 = c0[id(2, 2)] / (sp * sp * sq * sq);
+  const double s22min = (2 + 5 * Sq3 * Spq * Sp3) / 6;
+  if (S22 < s22min) S22 = s22min;
+  c[id(2, 1)] = (Spq * Sp3) * sp * sp * sq;
+  c[id(1, 2)] = (Spq * Sq3) * sp * sq * sq;
+  c[id(3, 1)] = (Spq * Sp4) * sp * sp * sp * sq;
+  c[id(1, 3)] = (Spq * Sq4) * sp * sq * sq * sq;
+  c[id(2, 2)] = S22 * sp * sp * sq * sq;
+}
+
+HD void pack35(const double* c, double* M) {
+#pragma unroll
+  for (int n = 0; n < 35; ++n) M[n] = c[T35(n)];
+}
+
+// Minimum/maximum over two Jacobians with compute_jacobian_eigenvalues.jl's NaN guard: a
+// non-finite first Jacobian makes both NaN; a non-finite second one is skipped.
+HD void combine_guarded(const EigRes& A, const EigRes& B, double& vmin, double& vmax) {
+  vmin = A.ok ? A.vmin : NAN;
+  vmax = A.ok ? A.vmax : NAN;
+  if (B.ok) {
+    vmin = jmin(vmin, B.vmin);
+    vmax = jmax(vmax, B.vmax);
+  }
+}
+
+// eigenvalues6_hyperbolic_3D (src/numerics/eigenvalues6_hyperbolic_3D.jl:22-155).
+// AXIS 0: UV and UW Jacobians; AXIS 1: VU and VW.  In: M.  Out: Mr (may alias nothing).
+template <int AXIS>
+HD void eig6_axis(const double* M, int flag2D, double Ma, double& vmin, double& vmax,
+                  double* Mr, uint32_t& fl) {
+  double c[35];
+  double M000, um, vm, wm;
+  raw_to_central4(M, c, M000, um, vm, wm);
+  EigRes A, B;
+  if (AXIS == 0) {
+    A = plane_eigs(plane<0, 1>(c, um, vm, wm), fl);
+    B = plane_eigs(plane<0, 2>(c, um, vm, wm), fl);
+  } else {
+    A = plane_eigs(plane<1, 0>(c, um, vm, wm), fl);
+    B = plane_eigs(plane<1, 2>(c, um, vm, wm), fl);
+  }
+  combine_guarded(A, B, vmin, vmax);
+  if (A.cplx || B.cplx) {
+    fl |= AXIS == 0 ? F_ECX : F_ECY;
+    double cn[35];
+#pragma unroll
+    for (int t = 0; t < 35; ++t) cn[t] = c[t];
+    if (A.cplx) correct_plane<0, 1>(c, cn);
+    if (B.cplx) {
+      if (AXIS == 0) correct_plane<0, 2>(c, cn);
+      else correct_plane<1, 2>(c, cn);
+    }
+    double h[35];
+#pragma unroll
+    for (int t = 0; t < 35; ++t) h[t] = cn[t];
+    central_to_raw<4>(h, M000, um, vm, wm);
+    double Mh[35];
+    pack35(h, Mh);
+    flux_closure35<false, double>(Mh, flag2D, Ma, Mr, nullptr, nullptr, nullptr, 0, fl);
+    raw_to_central4(Mr, c, M000, um, vm, wm);
+    if (AXIS == 0) {
+      A = plane_eigs(plane<0, 1>(c, um, vm, wm), fl);
+      B = plane_eigs(plane<0, 2>(c, um, vm, wm), fl);
+    } else {
+      A = plane_eigs(plane<1, 0>(c, um, vm, wm), fl);
+      B = plane_eigs(plane<1, 2>(c, um, vm, wm), fl);
+    }
+    combine_guarded(A, B, vmin, vmax);
+  } else {
+#pragma unroll
+    for (int n = 0; n < 35; ++n) Mr[n] = M[n];
+  }
+}
+
+// eigenvalues6z_hyperbolic_3D (src/numerics/eigenvalues6z_hyperbolic_3D.jl:23-365): the WU
+// Jacobian (flux along z, second variable x) and, as the reference maps it, the VW one.
+HD void eig6_z(const double* M, int flag2D, double Ma, bool positivity, double& vmin,
+               double& vmax, double* Mr, uint32_t& fl) {
+#pragma unroll
+  for (int n = 0; n < 35; ++n) Mr[n] = M[n];
+  if (positivity && M[0] <= 0.0) {
+    Mr[0] = jmax(M[0], 1e-14);
+    fl |= F_RHO;
+  }
+  double W[35];
+#pragma unroll
+  for (int n = 0; n < 35; ++n) W[n] = M[n];
+  W[0] = Mr[0];
+  if (positivity && W[0] > 1e-15) {
+    const double m0 = W[0];
+    const double vx = M[2] / m0 - (M[1] / m0) * (M[1] / m0);
+    const double vy = M[9] / m0 - (M[5] / m0) * (M[5] / m0);
+    const double vz = M[19] / m0 - (M[15] / m0) * (M[15] / m0);
+    if (vx < 0.0 || vy < 0.0 || vz < 0.0) {
+      fl |= F_ZNEG;
+      flux_closure35<false, double>(M, flag2D, Ma, Mr, nullptr, nullptr, nullptr, 0, fl);
+#pragma unroll
+      for (int n = 0; n < 35; ++n) W[n] = Mr[n];
+    }
+  }
+  double c[35];
+  double M000, um, vm, wm;
+  raw_to_central4(W, c, M000, um, vm, wm);
+  EigRes A = plane_eigs(plane<2, 0>(c, um, vm, wm), fl);
+  EigRes B = plane_eigs(plane<1, 2>(c, um, vm, wm), fl);
+  if (!A.ok || !B.ok) fl |= F_NONFIN;  // the reference's eigvals throws here
+  vmin = jmin(A.vmin, B.vmin);
+  vmax = jmax(A.vmax, B.vmax);
+  if (A.cplx || B.cplx) {
+    fl |= F_ECZ;
+    raw_to_central4(M, c, M000, um, vm, wm);  // corrections start from the input M
+    double cn[35];
+#pragma unroll
+    for (int t = 0; t < 35; ++t) cn[t] = c[t];
+    if (A.cplx) correct_plane<0, 2>(c, cn);
+    if (B.cplx) correct_plane<1, 2>(c, cn);
+    central_to_raw<4>(cn, M000, um, vm, wm);
+    double Mh[35];
+    pack35(cn, Mh);
+    flux_closure35<false, double>(Mh, flag2D, Ma, Mr, nullptr, nullptr, nullptr, 0, fl);
+    raw_to_central4(Mr, c, M000, um, vm, wm);
+    A = plane_eigs(plane<2, 0>(c, um, vm, wm), fl);
+    B = plane_eigs(plane<1, 2>(c, um, vm, wm), fl);
+    vmin = jmin(A.vmin, B.vmin);
+    vmax = jmax(A.vmax, B.vmax);
+  }
+}
+
+// Pass 1 of the step for one cell (simulation_runner.jl:356-379).  Outputs the projected
+// state Mr, the fluxes (strided), and the six wave speeds.
+template <typename FT = double>
+HD void pass1_cell(const double* M, int flag2D, double Ma, bool positivity, double* Mr, FT* Fx,
+                   FT* Fy, FT* Fz, size_t st, double vp[6], uint32_t& fl) {
+  double A[35], B[35];
+  double xmn, xmx, ymn, ymx, zmn, zmx;
+  flux_closure35<false, double>(M, flag2D, Ma, A, nullptr, nullptr, nullptr, 0, fl);
+  eig6_axis<0>(A, flag2D, Ma, xmn, xmx, B, fl);
+  eig6_axis<1>(B, flag2D, Ma, ymn, ymx, A, fl);
+  eig6_z(A, flag2D, Ma, positivity, zmn, zmx, B, fl);
+  flux_closure35<true, FT>(B, flag2D, Ma, Mr, Fx, Fy, Fz, st, fl);
+  double a, b;
+  closure1d_speeds(Mr[0], Mr[1], Mr[2], Mr[3], Mr[4], a, b);
+  vp[0] = jmin(a, xmn);
+  vp[1] = jmax(b, xmx);
+  closure1d_speeds(Mr[0], Mr[5], Mr[9], Mr[12], Mr[14], a, b);
+  vp[2] = jmin(a, ymn);
+  vp[3] = jmax(b, ymx);
+  closure1d_speeds(Mr[0], Mr[15], Mr[19], Mr[22], Mr[24], a, b);
+  vp[4] = jmin(a, zmn);
+  vp[5] = jmax(b, zmx);
+}
+
+// collision35 (src/numerics/collision35.jl:17-45): relaxation toward the isotropic
+// Maxwellian with the same density, velocity and trace temperature.
+HD void collision_cell(const double* M, double dt, double Kn, bool positivity, double* Mo) {
+  const double rho = M[0], r = 1.0 / rho;
+  const double um = M[1] / rho, vm = M[5] / rho, wm = M[15] / rho;
+  const double C200 = M[2] / rho - um * um, C020 = M[9] / rho - vm * vm,
+               C002 = M[19] / rho - wm * wm;
+  (void)r;
+  double Th = (C200 + C020 + C002) / 3;
+  if (positivity) Th = jmax(Th, 1e-14);
+  const double tc = Kn / (rho * sqrt(Th) * 2);
+  double g[35];
+#pragma unroll
+  for (int t = 0; t < 35; ++t) g[t] = 0.0;
+  g[tpos(2, 0, 0)] = Th; g[tpos(0, 2, 0)] = Th; g[tpos(0, 0, 2)] = Th;
+  g[tpos(4, 0, 0)] = 3 * Th * Th; g[tpos(0, 4, 0)] = 3 * Th * Th; g[tpos(0, 0, 4)] = 3 * Th * Th;
+  g[tpos(2, 2, 0)] = Th * Th; g[tpos(2, 0, 2)] = Th * Th; g[tpos(0, 2, 2)] = Th * Th;
+  central_to_raw<4>(g, rho, um, vm, wm);
+  const double e = exp(-dt / tc);
+#pragma unroll
+  for (int n = 0; n < 35; ++n) {
+    const double MG = g[T35(n)];
+    Mo[n] = MG - e * (MG - M[n]);
+  }
+}
+
+// Pass 2 for one cell (simulation_runner.jl:526-556): realizability, x and y wave-speed
+// corrections, realizability, then BGK.
+HD void pass2_cell(const double* M, int flag2D, double Ma, bool positivity, double dt,
+                   double Kn, double* Mo, uint32_t& fl) {
+  double A[35], B[35];
+  double d0, d1;
+  flux_closure35<false, double>(M, flag2D, Ma, A, nullptr, nullptr, nullptr, 0, fl);
+  eig6_axis<0>(A, flag2D, Ma, d0, d1, B, fl);
+  eig6_axis<1>(B, flag2D, Ma, d0, d1, A, fl);
+  flux_closure35<false, double>(A, flag2D, Ma, B, nullptr, nullptr, nullptr, 0, fl);
+  collision_cell(B, dt, Kn, positivity, Mo);
+}
+
+}  // namespace hyq

@@ -1,0 +1,80 @@
+"""ctypes binding of libhyqmom_hip.so (include/hyqmom.h).
+
+The library is built in-tree by hyqmom.jl_amd/build.py (hipcc, gfx950).  Loading fails
+loudly when it is missing: there is no CPU fallback for the device step.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(os.path.dirname(HERE), "build", "libhyqmom_hip.so")
+
+HYQ_OK = 0
+ERRORS = {-1: "HYQ_EINVAL", -2: "HYQ_ENOMEM", -3: "HYQ_EHIP", -4: "HYQ_ENONFINITE",
+          -5: "HYQ_ENOTCONV", -6: "HYQ_ENODEV"}
+PREC = {"fp64": 0, "mixed": 1, "mixed_learned": 2}
+
+FLAG_NAMES = ["UNIVARIATE", "EDGE_CORNER", "FACES", "FLAG220", "ROOTS2D", "ROOTS4", "EIGCORR_X",
+              "EIGCORR_Y", "EIGCORR_Z", "ZNEGVAR", "NONFINITE", "RHOCLAMP", "EIG_FALLBACK",
+              "QR_NOCONV"]
+
+
+class HyqParams(ctypes.Structure):
+    _fields_ = [("Nx", ctypes.c_int32), ("Ny", ctypes.c_int32), ("Nz", ctypes.c_int32),
+                ("Nmom", ctypes.c_int32), ("flag2D", ctypes.c_int32), ("positivity", ctypes.c_int32),
+                ("precision", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("tmax", ctypes.c_double), ("Kn", ctypes.c_double), ("Ma", ctypes.c_double),
+                ("CFL", ctypes.c_double), ("dtmax", ctypes.c_double),
+                ("xmin", ctypes.c_double), ("xmax", ctypes.c_double), ("ymin", ctypes.c_double),
+                ("ymax", ctypes.c_double), ("zmin", ctypes.c_double), ("zmax", ctypes.c_double)]
+
+
+class HyqError(RuntimeError):
+    pass
+
+
+_LIB = None
+D = ctypes.POINTER(ctypes.c_double)
+VP = ctypes.c_void_p
+I64 = ctypes.c_int64
+
+# exported symbol -> (argtypes, restype)
+SIGNATURES = {
+    "hyq_create": ([ctypes.POINTER(HyqParams), ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "hyq_destroy": ([VP], ctypes.c_int),
+    "hyq_set_state": ([VP, D], ctypes.c_int),
+    "hyq_get_state": ([VP, D], ctypes.c_int),
+    "hyq_step": ([VP, D, D, ctypes.POINTER(I64)], ctypes.c_int),
+    "hyq_run": ([VP, ctypes.c_double, I64, D, ctypes.POINTER(I64)], ctypes.c_int),
+    "hyq_get_wave_speeds": ([VP, D], ctypes.c_int),
+    "hyq_get_flags": ([VP, ctypes.POINTER(ctypes.c_uint32)], ctypes.c_int),
+    "hyq_kernel_times": ([VP, D, D, ctypes.POINTER(I64)], ctypes.c_int),
+    "hyq_reset_kernel_times": ([VP], ctypes.c_int),
+    "hyq_closure_batch": ([VP, I64, ctypes.c_double, ctypes.c_int32, VP, VP, VP, VP, VP], ctypes.c_int),
+    "hyq_wavespeed_batch": ([VP, I64, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                             VP, VP, VP, VP, VP, VP], ctypes.c_int),
+    "hyq_eig6_batch": ([VP, I64, VP, VP], ctypes.c_int),
+    "hyq_collision_batch": ([VP, I64, ctypes.c_double, ctypes.c_double, ctypes.c_int32, VP], ctypes.c_int),
+    "hyq_last_error": ([], ctypes.c_char_p),
+    "hyq_version": ([], ctypes.c_char_p),
+}
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIBPATH):
+            raise HyqError("libhyqmom_hip.so is not built (%s): run __graft_entry__.build()" % LIBPATH)
+        L = ctypes.CDLL(LIBPATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _LIB = L
+    return _LIB
+
+
+def check(rc, what=""):
+    if rc != HYQ_OK:
+        msg = lib().hyq_last_error().decode()
+        raise HyqError("%s failed: %s (%s)" % (what, ERRORS.get(rc, rc), msg))

@@ -1,0 +1,213 @@
+"""simulation_runner(params) and run_simulation_with_snapshots(params; snapshot_interval):
+the reference's Julia API for the 3D step (src/simulation_runner.jl:63-759, 994-1011),
+mirrored over the C ABI of libhyqmom_hip.so.
+
+params is a dict (or any mapping / object with attributes) with the reference's field
+names: Nx, Ny, Nz, tmax, Kn, Ma, flag2D, CFL, Nmom, nnmax, dtmax, rhol, rhor, T, r110,
+r101, r011, symmetry_check_interval, homogeneous_z, debug_output; optional positivity,
+snapshot_interval, snapshot_filename, xmin..zmax, use_custom_ic, ic_background, ic_jets,
+and (this build) precision in {"fp64", "mixed", "mixed_learned"}.
+"""
+import ctypes
+import math
+import time
+
+import numpy as np
+
+from . import _lib
+from .initial_conditions import cell_centers, initialize_moment_field
+from .moments import maxwellian35
+
+REQUIRED = ["Nx", "Ny", "Nz", "tmax", "Kn", "Ma", "flag2D", "CFL", "Nmom", "nnmax", "dtmax",
+            "rhol", "rhor", "T", "r110", "r101", "r011"]
+
+
+def _get(params, key, default=None):
+    if isinstance(params, dict):
+        return params.get(key, default)
+    return getattr(params, key, default)
+
+
+def hardcoded_jets(Nx, Ny, Nz, rhol, rhor, T, Ma, r110=0.0, r101=0.0, r011=0.0):
+    """The original crossing-jets IC (src/simulation_runner.jl:200-271)."""
+    C = np.array([[T, r110 * math.sqrt(T * T), r101 * math.sqrt(T * T)],
+                  [r110 * math.sqrt(T * T), T, r011 * math.sqrt(T * T)],
+                  [r101 * math.sqrt(T * T), r011 * math.sqrt(T * T), T]])
+    Uc = Ma / math.sqrt(2.0)
+    bg = maxwellian35(rhor, (0.0, 0.0, 0.0), C)
+    Mt = maxwellian35(rhol, (-Uc, -Uc, 0.0), C)
+    Mb = maxwellian35(rhol, (Uc, Uc, 0.0), C)
+    Cx, Cy = int(math.floor(0.1 * Nx)), int(math.floor(0.1 * Ny))
+    Mint_x, Maxt_x, Minb_x, Maxb_x = Nx // 2 + 1, Nx // 2 + 1 + Cx, Nx // 2 - Cx, Nx // 2
+    Mint_y, Maxt_y, Minb_y, Maxb_y = Ny // 2 + 1, Ny // 2 + 1 + Cy, Ny // 2 - Cy, Ny // 2
+    Cz = min(Cx, Cy)
+    Mint_z, Maxt_z = Nz // 2 + 1 - Cz // 2, Nz // 2 + 1 + Cz // 2
+    gi = np.arange(1, Nx + 1)[:, None, None]
+    gj = np.arange(1, Ny + 1)[None, :, None]
+    gk = np.arange(1, Nz + 1)[None, None, :]
+    M = np.empty((Nx, Ny, Nz, 35), order="F")
+    M[...] = bg
+    zin = (gk >= Mint_z) & (gk <= Maxt_z)
+    bot = (gi >= Minb_x) & (gi <= Maxb_x) & (gj >= Minb_y) & (gj <= Maxb_y) & zin
+    top = (gi >= Mint_x) & (gi <= Maxt_x) & (gj >= Mint_y) & (gj <= Maxt_y) & zin
+    M[bot] = Mb
+    M[top] = Mt
+    return M
+
+
+def initial_state(params):
+    Nx, Ny, Nz = int(_get(params, "Nx")), int(_get(params, "Ny")), int(_get(params, "Nz"))
+    r = [float(_get(params, k, 0.0)) for k in ("r110", "r101", "r011")]
+    if _get(params, "use_custom_ic", False):
+        M, _ = initialize_moment_field(Nx, Ny, Nz, _get(params, "ic_background"), _get(params, "ic_jets"),
+                                       *r, xmin=_get(params, "xmin", -0.5), xmax=_get(params, "xmax", 0.5),
+                                       ymin=_get(params, "ymin", -0.5), ymax=_get(params, "ymax", 0.5),
+                                       zmin=_get(params, "zmin", -0.5), zmax=_get(params, "zmax", 0.5))
+        return M
+    return hardcoded_jets(Nx, Ny, Nz, float(_get(params, "rhol")), float(_get(params, "rhor")),
+                          float(_get(params, "T")), float(_get(params, "Ma")), *r)
+
+
+class Solver:
+    """One device context: state, step and run on the GPU."""
+
+    def __init__(self, params):
+        for k in REQUIRED:
+            if _get(params, k) is None:
+                raise ValueError("params is missing required field %s" % k)
+        if int(_get(params, "Nmom")) != 35:
+            raise ValueError("Nmom must be 35")
+        self.params = params
+        p = _lib.HyqParams()
+        p.Nx, p.Ny, p.Nz = int(_get(params, "Nx")), int(_get(params, "Ny")), int(_get(params, "Nz"))
+        p.Nmom = 35
+        p.flag2D = int(_get(params, "flag2D"))
+        p.positivity = 1 if _get(params, "positivity", True) else 0
+        p.precision = _lib.PREC[_get(params, "precision", "fp64")]
+        p.tmax = float(_get(params, "tmax"))
+        p.Kn = float(_get(params, "Kn"))
+        p.Ma = float(_get(params, "Ma"))
+        p.CFL = float(_get(params, "CFL"))
+        p.dtmax = float(_get(params, "dtmax"))
+        p.xmin, p.xmax = float(_get(params, "xmin", -0.5)), float(_get(params, "xmax", 0.5))
+        p.ymin, p.ymax = float(_get(params, "ymin", -0.5)), float(_get(params, "ymax", 0.5))
+        p.zmin, p.zmax = float(_get(params, "zmin", -0.5)), float(_get(params, "zmax", 0.5))
+        self.p = p
+        self.shape = (p.Nx, p.Ny, p.Nz, 35)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().hyq_create(ctypes.byref(p), ctypes.byref(h)), "hyq_create")
+        self.h = h
+        self.t = 0.0
+        self.nn = 0
+
+    def close(self):
+        if self.h:
+            _lib.lib().hyq_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, M):
+        buf = np.asfortranarray(np.asarray(M, dtype=np.float64).reshape(self.shape, order="F"))
+        flat = buf.ravel(order="F")
+        _lib.check(_lib.lib().hyq_set_state(self.h, flat.ctypes.data_as(_lib.D)), "hyq_set_state")
+
+    def get_state(self):
+        flat = np.empty(int(np.prod(self.shape)), dtype=np.float64)
+        _lib.check(_lib.lib().hyq_get_state(self.h, flat.ctypes.data_as(_lib.D)), "hyq_get_state")
+        return flat.reshape(self.shape, order="F")
+
+    def step(self):
+        t = ctypes.c_double(self.t); dt = ctypes.c_double(); nn = _lib.I64(self.nn)
+        _lib.check(_lib.lib().hyq_step(self.h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(nn)),
+                   "hyq_step")
+        self.t, self.nn = t.value, nn.value
+        return dt.value
+
+    def run(self, tmax, nnmax):
+        t = ctypes.c_double(self.t); nn = _lib.I64(self.nn)
+        _lib.check(_lib.lib().hyq_run(self.h, ctypes.c_double(tmax), _lib.I64(int(nnmax)), ctypes.byref(t),
+                                      ctypes.byref(nn)), "hyq_run")
+        self.t, self.nn = t.value, nn.value
+
+    def wave_speeds(self):
+        n = self.p.Nx * self.p.Ny * self.p.Nz
+        out = np.empty(6 * n)
+        _lib.check(_lib.lib().hyq_get_wave_speeds(self.h, out.ctypes.data_as(_lib.D)), "wave speeds")
+        return out.reshape(6, n)
+
+    def flags(self):
+        n = self.p.Nx * self.p.Ny * self.p.Nz
+        out = np.empty(n, dtype=np.uint32)
+        _lib.check(_lib.lib().hyq_get_flags(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))),
+                   "flags")
+        return out
+
+    def kernel_times(self):
+        a = ctypes.c_double(); b = ctypes.c_double(); n = _lib.I64()
+        _lib.check(_lib.lib().hyq_kernel_times(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n)),
+                   "kernel times")
+        return a.value, b.value, n.value
+
+    def reset_kernel_times(self):
+        _lib.check(_lib.lib().hyq_reset_kernel_times(self.h), "reset times")
+
+
+def grid(params):
+    """grid_out of simulation_runner.jl:720-730."""
+    Nx, Ny, Nz = int(_get(params, "Nx")), int(_get(params, "Ny")), int(_get(params, "Nz"))
+    lo = [float(_get(params, k, d)) for k, d in (("xmin", -.5), ("ymin", -.5), ("zmin", -.5))]
+    hi = [float(_get(params, k, d)) for k, d in (("xmax", .5), ("ymax", .5), ("zmax", .5))]
+    out = {}
+    for ax, n, a, b in zip("xyz", (Nx, Ny, Nz), lo, hi):
+        out[ax] = np.linspace(a, b, n + 1)
+        out["d" + ax] = (b - a) / n
+        out[ax + "m"] = cell_centers(a, b, n)[0]
+    return out
+
+
+def simulation_runner(params, verbose=False):
+    """Returns (M_final (Nx,Ny,Nz,35), final_time, time_steps, grid_out) as the reference
+    (src/simulation_runner.jl:63-759), or (snapshot_filename, grid_out) in snapshot mode."""
+    snap = int(_get(params, "snapshot_interval", 0) or 0)
+    solver = Solver(params)
+    try:
+        solver.set_state(initial_state(params))
+        tmax = float(_get(params, "tmax"))
+        nnmax = int(_get(params, "nnmax"))
+        if snap > 0:
+            from .snapshots import SnapshotWriter
+            fname = _get(params, "snapshot_filename") or "snapshots.npz"
+            w = SnapshotWriter(fname, params, snap)
+            w.write(solver.get_state(), solver.t, solver.nn)
+            while solver.t < tmax and solver.nn < nnmax:
+                solver.run(tmax, min(nnmax, solver.nn + snap))
+                w.write(solver.get_state(), solver.t, solver.nn)
+            w.close(grid(params))
+            return fname, grid(params)
+        t0 = time.time()
+        while solver.t < tmax and solver.nn < nnmax:
+            solver.step()
+            if verbose:
+                n = params_cells(params)
+                el = time.time() - t0
+                print("step %d t=%.6e wall=%.3fs s/pt=%.3e" % (solver.nn, solver.t, el, el / n))
+                t0 = time.time()
+        return solver.get_state(), solver.t, solver.nn, grid(params)
+    finally:
+        solver.close()
+
+
+def params_cells(params):
+    return int(_get(params, "Nx")) * int(_get(params, "Ny")) * int(_get(params, "Nz"))
+
+
+def run_simulation_with_snapshots(params, snapshot_interval=2):
+    """src/simulation_runner.jl:994-1011"""
+    p = dict(params) if isinstance(params, dict) else dict(vars(params))
+    p["snapshot_interval"] = snapshot_interval
+    return simulation_runner(p)

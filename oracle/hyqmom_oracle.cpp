@@ -1225,14 +1225,17 @@ static void flux_closure35(const double* M4, int flag2D, double Ma, double* Fx, 
   const double A022 = std::sqrt((H020 + s.S030 * s.S030) * (H002 + s.S003 * s.S003));
   s.S022 = realizablity_S220(s.S011, s.S022, A022);
 
+  // check2D_all_planes (:83-88) always runs; its outputs are read only on the edge/corner
+  // branch, so its branch flags are recorded only there.
+  const double R110 = 1 - s.S110 * s.S110, R101 = 1 - s.S101 * s.S101, R011 = 1 - s.S011 * s.S011;
+  uint32_t* f2d = (R110 <= 0 || R101 <= 0 || R011 <= 0) ? flags : nullptr;
   double p1[10], p2[10], p3[10];
   check2D(s.S300, s.S400, s.S110, s.S210, s.S310, s.S120, s.S220, s.S030, s.S130, s.S040, p1,
-          flags);
+          f2d);
   check2D(s.S300, s.S400, s.S101, s.S201, s.S301, s.S102, s.S202, s.S003, s.S103, s.S004, p2,
-          flags);
+          f2d);
   check2D(s.S030, s.S040, s.S011, s.S021, s.S031, s.S012, s.S022, s.S003, s.S013, s.S004, p3,
-          flags);
-  const double R110 = 1 - s.S110 * s.S110, R101 = 1 - s.S101 * s.S101, R011 = 1 - s.S011 * s.S011;
+          f2d);
   if (R110 <= 0 || R101 <= 0 || R011 <= 0) {
     if (flags) *flags |= HYQ_FLAG_EDGE_CORNER;
     EdgeIn q;

@@ -1,0 +1,67 @@
+"""ctypes binding of tests/native/_build/libcellhost.so (device cell numerics built for the
+host; CPU tests only)."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_L = None
+D = ctypes.POINTER(ctypes.c_double)
+
+
+def lib():
+    global _L
+    if _L is None:
+        subprocess.run(["make", "-s", "-C", os.path.join(HERE, "native")], check=True)
+        _L = ctypes.CDLL(os.path.join(HERE, "native", "_build", "libcellhost.so"))
+    return _L
+
+
+def _p(a):
+    return a.ctypes.data_as(D)
+
+
+def _f(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+def flux_closure35(M, flag2D=0, Ma=0.0):
+    M = _f(M); Fx, Fy, Fz, Mr = (np.zeros(35) for _ in range(4)); fl = ctypes.c_uint32()
+    lib().cell_flux_closure35(_p(M), flag2D, ctypes.c_double(Ma), _p(Fx), _p(Fy), _p(Fz), _p(Mr),
+                              ctypes.byref(fl))
+    return Fx, Fy, Fz, Mr, fl.value
+
+
+def eigenvalues6(M, axis, flag2D=0, Ma=0.0):
+    M = _f(M); Mr = np.zeros(35); a = ctypes.c_double(); b = ctypes.c_double(); fl = ctypes.c_uint32()
+    lib().cell_eig6(_p(M), axis, flag2D, ctypes.c_double(Ma), ctypes.byref(a), ctypes.byref(b), _p(Mr),
+                    ctypes.byref(fl))
+    return a.value, b.value, Mr, fl.value
+
+
+def pass1(M, flag2D=0, Ma=0.0, positivity=True):
+    M = _f(M); Mr, Fx, Fy, Fz = (np.zeros(35) for _ in range(4)); vp = np.zeros(6); fl = ctypes.c_uint32()
+    lib().cell_pass1(_p(M), flag2D, ctypes.c_double(Ma), int(positivity), _p(Mr), _p(Fx), _p(Fy), _p(Fz),
+                     _p(vp), ctypes.byref(fl))
+    return Mr, Fx, Fy, Fz, vp, fl.value
+
+
+def pass2(M, dt, Kn, flag2D=0, Ma=0.0, positivity=True):
+    M = _f(M); Mo = np.zeros(35); fl = ctypes.c_uint32()
+    lib().cell_pass2(_p(M), flag2D, ctypes.c_double(Ma), int(positivity), ctypes.c_double(dt),
+                     ctypes.c_double(Kn), _p(Mo), ctypes.byref(fl))
+    return Mo, fl.value
+
+
+def collision35(M, dt, Kn, positivity=True):
+    M = _f(M); Mo = np.zeros(35)
+    lib().cell_collision(_p(M), ctypes.c_double(dt), ctypes.c_double(Kn), int(positivity), _p(Mo))
+    return Mo
+
+
+def closure1d(m5):
+    m = _f(m5); a = ctypes.c_double(); b = ctypes.c_double()
+    lib().cell_closure1d(_p(m), ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value

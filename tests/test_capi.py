@@ -1,0 +1,39 @@
+"""The C ABI library loads and exports every symbol include/hyqmom.h declares (CPU)."""
+import ctypes
+import os
+import re
+
+from _common import HERE
+
+ROOT = os.path.dirname(HERE)
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "hyqmom.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(hyq_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_header_symbols():
+    from hyqmom_amd import _lib
+    L = ctypes.CDLL(_lib.LIBPATH)
+    syms = declared_symbols()
+    assert len(syms) >= 15
+    for s in syms:
+        assert hasattr(L, s), s
+    assert set(syms) == set(_lib.SIGNATURES), set(syms) ^ set(_lib.SIGNATURES)
+
+
+def test_version_and_error_without_gpu():
+    from hyqmom_amd import _lib
+    L = _lib.lib()
+    assert b"gfx950" in L.hyq_version()
+    assert L.hyq_closure_batch(None, 1, 0.0, 0, None, None, None, None, None) == -1
+    assert b"bad argument" in L.hyq_last_error()
+
+
+def test_invalid_params_rejected():
+    from hyqmom_amd import _lib
+    L = _lib.lib()
+    p = _lib.HyqParams(Nx=4, Ny=4, Nz=4, Nmom=34)
+    h = ctypes.c_void_p()
+    assert L.hyq_create(ctypes.byref(p), ctypes.byref(h)) == -1

@@ -1,0 +1,154 @@
+"""GPU parity: libhyqmom_hip.so (through the C ABI) against the CPU oracle.
+
+FP64 bar: on cells whose branch flags match, every output agrees with the oracle to
+1e-12 relative to the field's magnitude (the per-moment maximum over the batch), except
+where the reference computation itself is ill-conditioned (flagged flips are counted and
+bounded separately)."""
+import ctypes
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from _common import gaussian_raw, mixture_raw, to35
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-12
+
+
+def _lib():
+    from hyqmom_amd import _lib
+    return _lib
+
+
+def cells(n, seed, spread=1.5, lam_lo=0.01):
+    rng = np.random.default_rng(seed)
+    out = [to35(mixture_raw(rng, spread=spread, lam_lo=lam_lo)) for _ in range(n - 4)]
+    for u, T in [((0.1, 0.2, 0.3), 1.0), ((-0.7, 0.4, 1.1), 2.5), ((0.0, 0.0, 0.0), 1.0), ((1.06, 1.06, 0), 1.0)]:
+        out.append(to35(gaussian_raw(0.9, u, np.eye(3) * T, 4)))
+    return np.array(out)
+
+
+def soa_dev(a):
+    from gpu_util import to_dev
+    return to_dev(np.ascontiguousarray(a.T))  # [35][n]
+
+
+def scaled_err(a, b):
+    """max over values of |a-b| / per-field scale; arrays [n][k]."""
+    scale = np.maximum(np.abs(b).max(axis=0), 1e-300)
+    return (np.abs(a - b) / scale).max()
+
+
+def test_closure_batch():
+    from gpu_util import empty_dev, to_host
+    L = _lib()
+    M = cells(256, 1)
+    n = len(M)
+    Md = soa_dev(M)
+    Mr, Fx, Fy, Fz = (empty_dev((35, n)) for _ in range(4))
+    fl = empty_dev((n,), "int32")
+    L.check(L.lib().hyq_closure_batch(Md.data_ptr(), n, 1.5, 0, Mr.data_ptr(), Fx.data_ptr(), Fy.data_ptr(),
+                                      Fz.data_ptr(), fl.data_ptr()), "closure")
+    g = [to_host(x).T for x in (Fx, Fy, Fz, Mr)]
+    gf = to_host(fl).view(np.uint32)
+    ref = [O.flux_closure35(M[c], 0, 1.5) for c in range(n)]
+    of = np.array([r[4] for r in ref], dtype=np.uint32)
+    match = gf == of
+    assert match.mean() > 0.98, (match.mean(), gf[~match], of[~match])
+    for k in range(4):
+        o = np.array([r[k] for r in ref])
+        e = scaled_err(g[k][match], o[match])
+        assert e < RTOL * 10, (k, e)
+
+
+def test_pass1_batch():
+    from gpu_util import empty_dev, to_host
+    L = _lib()
+    M = cells(200, 2)
+    n = len(M)
+    Md = soa_dev(M)
+    Mr = empty_dev((35, n)); F = empty_dev((105, n)); vp = empty_dev((6, n)); fl = empty_dev((n,), "int32")
+    p = F.data_ptr()
+    L.check(L.lib().hyq_wavespeed_batch(Md.data_ptr(), n, 1.5, 0, 1, 0, Mr.data_ptr(), p, p + 35 * n * 8,
+                                        p + 70 * n * 8, vp.data_ptr(), fl.data_ptr()), "wavespeed")
+    gMr, gF, gvp = to_host(Mr).T, to_host(F), to_host(vp).T
+    gf = to_host(fl).view(np.uint32)
+    oMr, oFx, oFy, oFz, ovp, ofl = O.pass1_batch(M, 0, 1.5, True)
+    match = gf == ofl
+    assert match.mean() > 0.97, match.mean()
+    assert scaled_err(gMr[match], oMr[match]) < RTOL * 10
+    for k, o in enumerate((oFx, oFy, oFz)):
+        assert scaled_err(gF[35 * k:35 * (k + 1)].T[match], o[match]) < RTOL * 10
+    # wave speeds: absolute error relative to the speed scale
+    assert np.abs(gvp[match] - ovp[match]).max() / np.abs(ovp).max() < 1e-10
+
+
+def test_eig6_batch_against_lapack():
+    from gpu_util import empty_dev, to_dev, to_host
+    L = _lib()
+    M = cells(128, 3)
+    sl = [1, 6, 10, 13, 15, 2, 7, 11, 14, 3, 8, 12, 4, 9, 5]
+    m15 = np.array([[m[i - 1] for i in sl] for m in M])
+    n = len(m15)
+    d = to_dev(np.ascontiguousarray(m15.T))
+    wr = empty_dev((6, n)); wi = empty_dev((6, n))
+    L.check(L.lib().hyq_eig6_batch(d.data_ptr(), n, wr.data_ptr(), wi.data_ptr()), "eig6")
+    g = to_host(wr).T + 1j * to_host(wi).T
+    for c in range(n):
+        ref = np.linalg.eigvals(O.jacobian6(m15[c]))
+        np.testing.assert_allclose(np.sort(g[c].real), np.sort(ref.real), atol=1e-7 * max(1, np.abs(ref).max()))
+
+
+def test_collision_batch():
+    from gpu_util import empty_dev, to_host
+    L = _lib()
+    M = cells(128, 4)
+    n = len(M)
+    Md = soa_dev(M)
+    out = empty_dev((35, n))
+    L.check(L.lib().hyq_collision_batch(Md.data_ptr(), n, 0.002, 0.01, 1, out.data_ptr()), "collision")
+    ref = np.array([O.collision35(m, 0.002, 0.01) for m in M])
+    assert scaled_err(to_host(out).T, ref) < RTOL
+
+
+def _params(N, Ma, Kn, nnmax, tmax, precision="fp64"):
+    import hyqmom_amd as H
+    bg, jets = H.crossing_jets(Ma)
+    return dict(Nx=N[0], Ny=N[1], Nz=N[2], tmax=tmax, Kn=Kn, Ma=Ma, flag2D=0, CFL=0.7, Nmom=35,
+                nnmax=nnmax, dtmax=1e-2, rhol=1.0, rhor=0.1, T=1.0, r110=0.0, r101=0.0, r011=0.0,
+                symmetry_check_interval=0, homogeneous_z=False, debug_output=False,
+                use_custom_ic=True, ic_background=bg, ic_jets=jets, precision=precision)
+
+
+def test_step_config1():
+    """Config 1 (20x20x10, Ma=1, Kn=1, tmax=0.01): one step, whole state."""
+    import hyqmom_amd as H
+    p = _params((20, 20, 10), 1.0, 1.0, 100000, 0.01)
+    M0 = H.initial_state(p)
+    Mg, t, nn, _ = H.simulation_runner(p)
+    Mo, to, nno, _, rc = O.run(M0, 20, 20, 10, 1.0, 1.0, 0, 0.7, 1e-2, 0.05, 0.05, 0.1, True, 0.01, 100000)
+    assert rc == 0 and nn == nno and t == to
+    scale = np.abs(Mo).max(axis=(0, 1, 2))
+    assert (np.abs(Mg - Mo).max(axis=(0, 1, 2)) / scale).max() < 1e-11
+
+
+def test_multistep_small_grid():
+    """12x12x8 crossing jets, Ma=1.5, Kn=0.01: 8 steps through hyq_run."""
+    import hyqmom_amd as H
+    p = _params((12, 12, 8), 1.5, 0.01, 8, 10.0)
+    M0 = H.initial_state(p)
+    s = H.Solver(p)
+    s.set_state(M0)
+    s.run(10.0, 8)
+    Mg = s.get_state()
+    tg, ng = s.t, s.nn
+    s.close()
+    Mo, to, nno, dts, rc = O.run(M0, 12, 12, 8, 0.01, 1.5, 0, 0.7, 1e-2, 1 / 12, 1 / 12, 1 / 8, True, 10.0, 8)
+    assert rc == 0 and ng == nno == 8
+    assert abs(tg - to) < 1e-12 * to
+    scale = np.abs(Mo).max(axis=(0, 1, 2))
+    err = (np.abs(Mg - Mo).max(axis=(0, 1, 2)) / scale).max()
+    assert err < 1e-9, err
