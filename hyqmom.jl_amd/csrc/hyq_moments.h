@@ -25,8 +25,8 @@ constexpr double EPS = 2.220446049250313e-16;
 HD constexpr int tpos(int i, int j, int k) {
   return (i + j + k) * (i + j + k + 1) * (i + j + k + 2) / 6 + (j + k) * (j + k + 1) / 2 + k;
 }
-HD constexpr double binom(int n, int k) {
-  double r = 1.0;
+HD constexpr int binom(int n, int k) {
+  int r = 1;
   for (int t = 1; t <= k; ++t) r = r * (n - k + t) / t;
   return r;
 }

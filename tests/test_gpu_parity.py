@@ -135,20 +135,22 @@ def test_step_config1():
     assert (np.abs(Mg - Mo).max(axis=(0, 1, 2)) / scale).max() < 1e-11
 
 
-def test_multistep_small_grid():
-    """12x12x8 crossing jets, Ma=1.5, Kn=0.01: 8 steps through hyq_run."""
+@pytest.mark.parametrize("N,Ma,Kn,steps", [((20, 20, 10), 1.0, 1.0, 8), ((24, 24, 12), 1.5, 0.01, 5)])
+def test_multistep_small_grid(N, Ma, Kn, steps):
+    """Crossing jets on small grids, several steps through hyq_run, whole state."""
     import hyqmom_amd as H
-    p = _params((12, 12, 8), 1.5, 0.01, 8, 10.0)
+    p = _params(N, Ma, Kn, steps, 10.0)
     M0 = H.initial_state(p)
     s = H.Solver(p)
     s.set_state(M0)
-    s.run(10.0, 8)
+    s.run(10.0, steps)
     Mg = s.get_state()
     tg, ng = s.t, s.nn
     s.close()
-    Mo, to, nno, dts, rc = O.run(M0, 12, 12, 8, 0.01, 1.5, 0, 0.7, 1e-2, 1 / 12, 1 / 12, 1 / 8, True, 10.0, 8)
-    assert rc == 0 and ng == nno == 8
+    Mo, to, nno, dts, rc = O.run(M0, *N, Kn, Ma, 0, 0.7, 1e-2, 1 / N[0], 1 / N[1], 1 / N[2], True, 10.0, steps)
+    assert rc == 0 and ng == nno == steps
     assert abs(tg - to) < 1e-12 * to
     scale = np.abs(Mo).max(axis=(0, 1, 2))
     err = (np.abs(Mg - Mo).max(axis=(0, 1, 2)) / scale).max()
+    print("max moment-scaled error after %d steps: %.3e" % (steps, err))
     assert err < 1e-9, err
