@@ -20,6 +20,7 @@
 
 #include "../../include/hyqmom.h"
 #include "hyq_pipeline.h"
+#include "hyq_update.h"
 
 using namespace hyq;
 
@@ -107,50 +108,30 @@ __global__ void k_dt(DevScalars* sc, double CFL, double hmin, double dtmax, doub
   sc->t += dt;
 }
 
-// ---- HLL update (pas_HLL.jl:24-84, flux_HLL.jl:20-32, apply_flux_update_3d.jl) ---------
-template <typename FT>
-struct Line {
-  const double* M;
-  const FT* F;
-  const FT* lo;
-  const FT* hi;
-  long long st;  // stride between consecutive cells of the line
-  int n;
-  __device__ double m(int q) const { return M[(long long)q * st]; }
-  __device__ double f(int q) const { return (double)F[(long long)q * st]; }
-  __device__ double vl(int q) const { return (double)lo[(long long)q * st]; }
-  __device__ double vh(int q) const { return (double)hi[(long long)q * st]; }
-  // intermediate HLL state of face q (between cells q and q+1), F unmodified
-  __device__ double wstar(int q, double ll, double lr) const {
-    if (fabs(ll - lr) > 1e-10)
-      return (ll * m(q) - lr * m(q + 1)) / (ll - lr) - (f(q) - f(q + 1)) / (ll - lr);
-    return 0.0;
+// ---- HLL update kernels (the line logic is in hyq_update.h) ----------------------
+__global__ __launch_bounds__(kBlock) void k_update_mixed(const double* __restrict__ Mr,
+                                                         const float* __restrict__ F,
+                                                         const float* __restrict__ vp,
+                                                         double* __restrict__ Mout,
+                                                         const DevScalars* sc, int Nx, int Ny,
+                                                         int Nz, double dx, double dy, double dz) {
+  if (sc->active == 0.0) return;
+  const long long N = (long long)Nx * Ny * Nz;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const int i = (int)(c % Nx), j = (int)((c / Nx) % Ny), k = (int)(c / ((long long)Nx * Ny));
+  const double dt = sc->dt;
+  const long long bx = c - i, by = c - (long long)j * Nx, bz = c - (long long)k * Nx * Ny;
+  for (int n = 0; n < 35; ++n) {
+    const double* Mn = Mr + (size_t)n * N;
+    LineF lx{Mn + bx, F + (size_t)n * N + bx, vp + bx, vp + (size_t)N + bx, 1, Nx};
+    LineF ly{Mn + by, F + (size_t)(35 + n) * N + by, vp + 2 * (size_t)N + by,
+             vp + 3 * (size_t)N + by, Nx, Ny};
+    LineF lz{Mn + bz, F + (size_t)(70 + n) * N + bz, vp + 4 * (size_t)N + bz,
+             vp + 5 * (size_t)N + bz, (long long)Nx * Ny, Nz};
+    Mout[(size_t)n * N + c] = update_cell(lx, ly, lz, Mn[c], i, j, k, Nz, dt / dx, dt / dy, dt / dz);
   }
-  // numerical flux of face q, 0 <= q <= n-2, with the physical-BC copies at both ends
-  __device__ double flux(int q) const {
-    if (q == 0) {
-      // lleft[1] = lright[1] = 0; F[1] := F[2]; Wstar[1] := Wstar[2]
-      double ws = 0.0;
-      if (n >= 3) {
-        const double ll = jmin(vl(1), vl(2)), lr = jmax(vh(1), vh(2));
-        ws = wstar(1, ll, lr);
-      }
-      const double f1 = f(1);
-      return 0.5 * (f1 + f1) - 0.5 * (0.0 * (ws - m(0)) - 0.0 * (ws - m(1)));
-    }
-    const double ll = jmin(vl(q), vl(q + 1)), lr = jmax(vh(q), vh(q + 1));
-    const double ws = wstar(q, ll, lr);
-    const double fa = f(q), fb = (q == n - 2) ? f(q) : f(q + 1);  // F[Np] := F[Np-1]
-    return 0.5 * (fa + fb) - 0.5 * (fabs(ll) * (ws - m(q)) - fabs(lr) * (ws - m(q + 1)));
-  }
-  // Mp at cell s of the line
-  __device__ double updated(int s, double dtdx) const {
-    if (n == 1) return m(s);
-    if (n == 2) return m(1);
-    const int j = s < 1 ? 1 : (s > n - 2 ? n - 2 : s);
-    return m(j) - dtdx * (flux(j) - flux(j - 1));
-  }
-};
+}
 
 template <typename FT>
 __global__ __launch_bounds__(kBlock) void k_update(const double* __restrict__ Mr,
@@ -173,15 +154,12 @@ __global__ __launch_bounds__(kBlock) void k_update(const double* __restrict__ Mr
                 vp + 3 * (size_t)N + by, Nx, Ny};
     Line<FT> lz{Mn + bz, F + (size_t)(70 + n) * N + bz, vp + 4 * (size_t)N + bz,
                 vp + 5 * (size_t)N + bz, (long long)Nx * Ny, Nz};
-    const double m0 = Mn[c];
-    const double mx = lx.updated(i, dt / dx);
-    const double my = ly.updated(j, dt / dy);
-    const double mz = (Nz == 1) ? m0 : lz.updated(k, dt / dz);  // apply_flux_update_3d.jl:183
-    Mout[(size_t)n * N + c] = mx + my + mz - 2.0 * m0;
+    Mout[(size_t)n * N + c] = update_cell(lx, ly, lz, Mn[c], i, j, k, Nz, dt / dx, dt / dy, dt / dz);
   }
 }
 
 // ---- pass 2 + BGK ----------------------------------------------------------------------
+template <bool MIXED>
 __global__ __launch_bounds__(kBlock) void k_pass2(double* __restrict__ M, const DevScalars* sc,
                                                   long long N, int flag2D, double Ma,
                                                   int positivity, double Kn) {
@@ -192,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void k_pass2(double* __restrict__ M, const 
 #pragma unroll
   for (int n = 0; n < 35; ++n) m[n] = M[(size_t)n * N + c];
   uint32_t fl = 0;
-  pass2_cell(m, flag2D, Ma, positivity != 0, sc->dt, Kn, o, fl);
+  pass2_cell<MIXED>(m, flag2D, Ma, positivity != 0, sc->dt, Kn, o, fl);
 #pragma unroll
   for (int n = 0; n < 35; ++n) M[(size_t)n * N + c] = o[n];
 }
@@ -410,12 +388,17 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
                        (const double*)c->F, (const double*)c->vp, c->M, c->sc, p.Nx, p.Ny, p.Nz,
                        c->dx, c->dy, c->dz);
   } else {
-    hipLaunchKernelGGL(k_update<float>, dim3(g), dim3(kBlock), 0, c->stream, c->Mr,
+    hipLaunchKernelGGL(k_update_mixed, dim3(g), dim3(kBlock), 0, c->stream, c->Mr,
                        (const float*)c->F, (const float*)c->vp, c->M, c->sc, p.Nx, p.Ny, p.Nz,
                        c->dx, c->dy, c->dz);
   }
-  hipLaunchKernelGGL(k_pass2, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->sc, c->N, p.flag2D,
-                     p.Ma, p.positivity, p.Kn);
+  if (p.precision == HYQ_PREC_FP64) {
+    hipLaunchKernelGGL(k_pass2<false>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->sc, c->N,
+                       p.flag2D, p.Ma, p.positivity, p.Kn);
+  } else {
+    hipLaunchKernelGGL(k_pass2<true>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->sc, c->N,
+                       p.flag2D, p.Ma, p.positivity, p.Kn);
+  }
   if (ev) hipEventRecord(ev[3], c->stream);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(HYQ_EHIP, "kernel launch", e);

@@ -175,6 +175,7 @@ HD void pass1_cell(const double* M, int flag2D, double Ma, bool positivity, doub
 
 // collision35 (src/numerics/collision35.jl:17-45): relaxation toward the isotropic
 // Maxwellian with the same density, velocity and trace temperature.
+template <bool MIXED = false>
 HD void collision_cell(const double* M, double dt, double Kn, bool positivity, double* Mo) {
   const double rho = M[0], r = 1.0 / rho;
   const double um = M[1] / rho, vm = M[5] / rho, wm = M[15] / rho;
@@ -191,16 +192,24 @@ HD void collision_cell(const double* M, double dt, double Kn, bool positivity, d
   g[tpos(4, 0, 0)] = 3 * Th * Th; g[tpos(0, 4, 0)] = 3 * Th * Th; g[tpos(0, 0, 4)] = 3 * Th * Th;
   g[tpos(2, 2, 0)] = Th * Th; g[tpos(2, 0, 2)] = Th * Th; g[tpos(0, 2, 2)] = Th * Th;
   central_to_raw<4>(g, rho, um, vm, wm);
-  const double e = exp(-dt / tc);
+  if (MIXED) {
+    // relaxation in FP32: M + (1 - e) (MG - M), the difference formed in FP64
+    const float a = 1.0f - expf((float)(-dt / tc));
 #pragma unroll
-  for (int n = 0; n < 35; ++n) {
-    const double MG = g[T35(n)];
-    Mo[n] = MG - e * (MG - M[n]);
+    for (int n = 0; n < 35; ++n) Mo[n] = M[n] + (double)(a * (float)(g[T35(n)] - M[n]));
+  } else {
+    const double e = exp(-dt / tc);
+#pragma unroll
+    for (int n = 0; n < 35; ++n) {
+      const double MG = g[T35(n)];
+      Mo[n] = MG - e * (MG - M[n]);
+    }
   }
 }
 
 // Pass 2 for one cell (simulation_runner.jl:526-556): realizability, x and y wave-speed
 // corrections, realizability, then BGK.
+template <bool MIXED = false>
 HD void pass2_cell(const double* M, int flag2D, double Ma, bool positivity, double dt,
                    double Kn, double* Mo, uint32_t& fl) {
   double A[35], B[35];
@@ -209,7 +218,7 @@ HD void pass2_cell(const double* M, int flag2D, double Ma, bool positivity, doub
   eig6_axis<0>(A, flag2D, Ma, d0, d1, B, fl);
   eig6_axis<1>(B, flag2D, Ma, d0, d1, A, fl);
   flux_closure35<false, double>(A, flag2D, Ma, B, nullptr, nullptr, nullptr, 0, fl);
-  collision_cell(B, dt, Kn, positivity, Mo);
+  collision_cell<MIXED>(B, dt, Kn, positivity, Mo);
 }
 
 }  // namespace hyq

@@ -3,6 +3,9 @@
 // of the product; the product is libhyqmom_hip.so.
 #include <cstring>
 #include "../../hyqmom.jl_amd/csrc/hyq_pipeline.h"
+#include "../../hyqmom.jl_amd/csrc/hyq_update.h"
+#include <vector>
+#include <cmath>
 
 using namespace hyq;
 extern "C" {
@@ -54,4 +57,56 @@ extern "C" void cell_delta2star(const double* in28, double* E21) {
   Sym6 E;
   delta2star(s, E);
   std::memcpy(E21, E.e, sizeof(E.e));
+}
+// Host emulation of one device step (k_pass1, k_dt, k_update, k_pass2) on the SoA state
+// M[35][N]; returns dt.  Same cell functions and line logic as the kernels.
+extern "C" double host_step(double* M, int Nx, int Ny, int Nz, double Kn, double Ma, int flag2D,
+                            double CFL, double dtmax, double dx, double dy, double dz,
+                            double tmax, double t, uint32_t* flags) {
+  const long long N = (long long)Nx * Ny * Nz;
+  std::vector<double> Mr(35 * N), F(105 * N), vp(6 * N);
+  double vmax = 0.0;
+  bool nan = false;
+  for (long long c = 0; c < N; ++c) {
+    double m[35], mr[35], v[6];
+    for (int n = 0; n < 35; ++n) m[n] = M[n * N + c];
+    uint32_t fl = 0;
+    pass1_cell<double>(m, flag2D, Ma, true, mr, &F[c], &F[35 * N + c], &F[70 * N + c], (size_t)N, v, fl);
+    for (int n = 0; n < 35; ++n) Mr[n * N + c] = mr[n];
+    for (int d = 0; d < 6; ++d) {
+      vp[d * N + c] = v[d];
+      if (std::isnan(v[d])) nan = true;
+      vmax = std::fmax(vmax, std::fabs(v[d]));
+    }
+    if (flags) flags[c] = fl;
+  }
+  if (nan) vmax = NAN;
+  double dt = jmin(CFL * jmin(jmin(dx, dy), dz) / vmax, jmin(dtmax, Kn));
+  dt = jmin(dt, tmax - t);
+  for (long long c = 0; c < N; ++c) {
+    const int i = (int)(c % Nx), j = (int)((c / Nx) % Ny), k = (int)(c / ((long long)Nx * Ny));
+    const long long bx = c - i, by = c - (long long)j * Nx, bz = c - (long long)k * Nx * Ny;
+    for (int n = 0; n < 35; ++n) {
+      const double* Mn = &Mr[n * N];
+      Line<double> lx{Mn + bx, &F[n * N] + bx, &vp[0] + bx, &vp[N] + bx, 1, Nx};
+      Line<double> ly{Mn + by, &F[(35 + n) * N] + by, &vp[2 * N] + by, &vp[3 * N] + by, Nx, Ny};
+      Line<double> lz{Mn + bz, &F[(70 + n) * N] + bz, &vp[4 * N] + bz, &vp[5 * N] + bz,
+                      (long long)Nx * Ny, Nz};
+      M[n * N + c] = update_cell(lx, ly, lz, Mn[c], i, j, k, Nz, dt / dx, dt / dy, dt / dz);
+    }
+  }
+  for (long long c = 0; c < N; ++c) {
+    double m[35], o[35];
+    for (int n = 0; n < 35; ++n) m[n] = M[n * N + c];
+    uint32_t fl = 0;
+    pass2_cell<false>(m, flag2D, Ma, true, dt, Kn, o, fl);
+    for (int n = 0; n < 35; ++n) M[n * N + c] = o[n];
+  }
+  return dt;
+}
+extern "C" void cell_pass2_flags(const double* M, int flag2D, double Ma, double dt, double Kn,
+                                 double* Mo, uint32_t* fl_out) {
+  uint32_t fl = 0;
+  pass2_cell<false>(M, flag2D, Ma, true, dt, Kn, Mo, fl);
+  *fl_out = fl;
 }
