@@ -15,7 +15,8 @@ def build(verbose=True):
     os.makedirs(out, exist_ok=True)
     target = os.path.join(out, "libhyqmom_hip.so")
     tmp = target + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-o", tmp, os.path.join(HERE, "csrc", "hyq_kernels.hip")]
+    srcs = [os.path.join(HERE, "csrc", f) for f in ("hyq_kernels.hip", "hyq_mlp.hip")]
+    cmd = [HIPCC] + FLAGS + ["-o", tmp] + srcs
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

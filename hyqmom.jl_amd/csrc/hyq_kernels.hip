@@ -24,6 +24,9 @@
 
 using namespace hyq;
 
+extern "C" int hyq_mlp_infer_async(hyq_mlp* m, const float* X, int64_t n, float* P,
+                                   hipStream_t stream);
+
 namespace {
 
 thread_local std::string g_err;
@@ -60,7 +63,7 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
   return v;
 }
 
-template <typename FT>
+template <typename FT, int EM>
 __global__ __launch_bounds__(kBlock) void k_pass1(const double* __restrict__ M,
                                                   double* __restrict__ Mr, FT* __restrict__ F,
                                                   FT* __restrict__ vp, uint32_t* __restrict__ flags,
@@ -74,8 +77,8 @@ __global__ __launch_bounds__(kBlock) void k_pass1(const double* __restrict__ M,
 #pragma unroll
     for (int n = 0; n < 35; ++n) m[n] = M[(size_t)n * N + c];
     uint32_t fl = 0;
-    pass1_cell<FT>(m, flag2D, Ma, positivity != 0, mr, F + c, F + (size_t)35 * N + c,
-                   F + (size_t)70 * N + c, (size_t)N, v, fl);
+    pass1_cell<FT, EM>(m, flag2D, Ma, positivity != 0, mr, F + c, F + (size_t)35 * N + c,
+                       F + (size_t)70 * N + c, (size_t)N, v, fl);
 #pragma unroll
     for (int n = 0; n < 35; ++n) Mr[(size_t)n * N + c] = mr[n];
 #pragma unroll
@@ -159,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const double* __restrict__ Mr
 }
 
 // ---- pass 2 + BGK ----------------------------------------------------------------------
-template <bool MIXED>
+template <bool MIXED, int EM>
 __global__ __launch_bounds__(kBlock) void k_pass2(double* __restrict__ M, const DevScalars* sc,
                                                   long long N, int flag2D, double Ma,
                                                   int positivity, double Kn) {
@@ -170,7 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_pass2(double* __restrict__ M, const 
 #pragma unroll
   for (int n = 0; n < 35; ++n) m[n] = M[(size_t)n * N + c];
   uint32_t fl = 0;
-  pass2_cell<MIXED>(m, flag2D, Ma, positivity != 0, sc->dt, Kn, o, fl);
+  pass2_cell<MIXED, EM>(m, flag2D, Ma, positivity != 0, sc->dt, Kn, o, fl);
 #pragma unroll
   for (int n = 0; n < 35; ++n) M[(size_t)n * N + c] = o[n];
 }
@@ -269,6 +272,114 @@ __global__ __launch_bounds__(kBlock) void k_eig6_batch(const double* __restrict_
   }
 }
 
+// ---- learned warm start: plane features and the split pass 1 -------------------------
+// The five distinct plane Jacobians of a cell, in warm-start slot order: UV, UW, VU, VW, WU.
+// Feature row of a plane: its 15 standardized moments in jacobian6.jl argument order
+// (S00 S01 S02 S03 S04 S10 S11 S12 S13 S20 S21 S22 S30 S31 S40), padded to 16.
+__device__ void plane_feature(const Plane& p, float* x) {
+  x[0] = 1.f; x[1] = 0.f; x[2] = 1.f; x[3] = (float)p.S03; x[4] = (float)p.S04;
+  x[5] = 0.f; x[6] = (float)p.S11; x[7] = (float)p.S12; x[8] = (float)p.S13;
+  x[9] = 1.f; x[10] = (float)p.S21; x[11] = (float)p.S22; x[12] = (float)p.S30;
+  x[13] = (float)p.S31; x[14] = (float)p.S40; x[15] = 0.f;
+}
+__device__ void cell_planes(const double* M, Plane pl[5]) {
+  double c[35], M000, um, vm, wm;
+  raw_to_central4(M, c, M000, um, vm, wm);
+  pl[0] = plane<0, 1>(c, um, vm, wm);
+  pl[1] = plane<0, 2>(c, um, vm, wm);
+  pl[2] = plane<1, 0>(c, um, vm, wm);
+  pl[3] = plane<1, 2>(c, um, vm, wm);
+  pl[4] = plane<2, 0>(c, um, vm, wm);
+}
+// Features (and, with L != null, labels from the full FP64 solve: standardized lmin, lmax and
+// a validity mask -- finite and real) of the five planes of n cells; sample = plane*n + cell.
+__global__ __launch_bounds__(kBlock) void k_plane_dataset(const double* __restrict__ M, long long n,
+                                                          float* __restrict__ X, float* L,
+                                                          uint8_t* valid) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double m[35];
+#pragma unroll
+  for (int t = 0; t < 35; ++t) m[t] = M[(size_t)t * n + c];
+  Plane pl[5];
+  cell_planes(m, pl);
+  for (int q = 0; q < 5; ++q) {
+    float x[16];
+    plane_feature(pl[q], x);
+    float4* dst = reinterpret_cast<float4*>(X + ((size_t)q * n + c) * 16);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) dst[t] = make_float4(x[4 * t], x[4 * t + 1], x[4 * t + 2], x[4 * t + 3]);
+    if (L) {
+      uint32_t fl = 0;
+      Plane ps = pl[q];
+      ps.a = 0.0;
+      ps.sig = 1.0;  // standardized eigenvalues
+      const EigRes r = plane_eigs(ps, fl);
+      L[((size_t)q * n + c) * 2] = (float)r.vmin;
+      L[((size_t)q * n + c) * 2 + 1] = (float)r.vmax;
+      valid[(size_t)q * n + c] = (r.ok && !r.cplx) ? 1 : 0;
+    }
+  }
+}
+// Pass 1, first half: projection (state written to A) and the plane features.
+__global__ __launch_bounds__(kBlock) void k_pass1a(const double* __restrict__ M, double* __restrict__ A,
+                                                   float* __restrict__ X, DevScalars* sc, long long N,
+                                                   int flag2D, double Ma) {
+  if (sc->active == 0.0) return;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  double m[35], a[35];
+#pragma unroll
+  for (int t = 0; t < 35; ++t) m[t] = M[(size_t)t * N + c];
+  uint32_t fl = 0;
+  pass1a_cell(m, flag2D, Ma, a, fl);
+#pragma unroll
+  for (int t = 0; t < 35; ++t) A[(size_t)t * N + c] = a[t];
+  Plane pl[5];
+  cell_planes(a, pl);
+  for (int q = 0; q < 5; ++q) {
+    float x[16];
+    plane_feature(pl[q], x);
+    float4* dst = reinterpret_cast<float4*>(X + ((size_t)q * N + c) * 16);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) dst[t] = make_float4(x[4 * t], x[4 * t + 1], x[4 * t + 2], x[4 * t + 3]);
+  }
+}
+// Pass 1, second half, with the network's predictions P [5N][2] as Newton warm starts.
+__global__ __launch_bounds__(kBlock) void k_pass1b(double* __restrict__ AMr, float* __restrict__ F,
+                                                   float* __restrict__ vp, uint32_t* __restrict__ flags,
+                                                   const float* __restrict__ P, DevScalars* sc,
+                                                   long long N, int flag2D, double Ma, int positivity) {
+  if (sc->active == 0.0) return;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long vb = 0;
+  if (c < N) {
+    double a[35], mr[35], v[6];
+#pragma unroll
+    for (int t = 0; t < 35; ++t) a[t] = AMr[(size_t)t * N + c];
+    Warm w;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      w.w[2 * q] = P[((size_t)q * N + c) * 2];
+      w.w[2 * q + 1] = P[((size_t)q * N + c) * 2 + 1];
+    }
+    uint32_t fl = 0;
+    pass1b_cell<float, 1>(a, flag2D, Ma, positivity != 0, mr, F + c, F + (size_t)35 * N + c,
+                          F + (size_t)70 * N + c, (size_t)N, v, fl, w);
+#pragma unroll
+    for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = mr[t];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+      vp[(size_t)d * N + c] = (float)v[d];
+      const unsigned long long b = __double_as_longlong(fabs(v[d]));
+      vb = b > vb ? b : vb;
+    }
+    flags[c] = fl;
+  }
+  vb = wave_max_u64(vb);
+  if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
+}
+
 unsigned grid_for(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -289,6 +400,9 @@ struct hyq_ctx {
   hipEvent_t ev[4];
   double ms_pass1, ms_update;
   long long timed_steps;
+  hyq_mlp* mlp;  // learned mode: warm-start network (not owned)
+  float* X;      // [5N][16] plane features
+  float* P;      // [5N][2] predictions
 };
 
 extern "C" {
@@ -301,7 +415,7 @@ int hyq_create(const hyq_params* params, hyq_ctx** out) {
   const hyq_params& p = *params;
   if (p.Nmom != 35) return fail(HYQ_EINVAL, "Nmom must be 35");
   if (p.Nx < 1 || p.Ny < 1 || p.Nz < 1) return fail(HYQ_EINVAL, "grid sizes must be >= 1");
-  if (p.precision < HYQ_PREC_FP64 || p.precision > HYQ_PREC_MIXED_LEARNED)
+  if (p.precision < HYQ_PREC_FP64 || p.precision > HYQ_PREC_MIXED_NEWTON)
     return fail(HYQ_EINVAL, "unknown precision mode");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(HYQ_ENODEV, "no HIP device");
@@ -324,6 +438,9 @@ int hyq_create(const hyq_params* params, hyq_ctx** out) {
             hipMalloc(&c->vp, fsz * 6 * c->N) == hipSuccess &&
             hipMalloc(&c->flags, sizeof(uint32_t) * c->N) == hipSuccess &&
             hipMalloc(&c->sc, sizeof(DevScalars)) == hipSuccess;
+  if (ok && p.precision == HYQ_PREC_MIXED_LEARNED)
+    ok = hipMalloc(&c->X, sizeof(float) * 16 * 5 * c->N) == hipSuccess &&
+         hipMalloc(&c->P, sizeof(float) * 2 * 5 * c->N) == hipSuccess;
   if (!ok) {
     hyq_destroy(c);
     return fail(HYQ_ENOMEM, "device allocation failed");
@@ -342,6 +459,8 @@ int hyq_destroy(hyq_ctx* c) {
   hipFree(c->vp);
   hipFree(c->flags);
   hipFree(c->sc);
+  hipFree(c->X);
+  hipFree(c->P);
   for (int i = 0; i < 4; ++i)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -372,13 +491,25 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
   hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, c->stream, c->sc);
   if (ev) hipEventRecord(ev[0], c->stream);
   if (p.precision == HYQ_PREC_FP64) {
-    hipLaunchKernelGGL(k_pass1<double>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
+    hipLaunchKernelGGL((k_pass1<double, 0>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
                        (double*)c->F, (double*)c->vp, c->flags, c->sc, c->N, p.flag2D, p.Ma,
                        p.positivity);
-  } else {
-    hipLaunchKernelGGL(k_pass1<float>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
+  } else if (p.precision == HYQ_PREC_MIXED) {
+    hipLaunchKernelGGL((k_pass1<float, 0>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
                        (float*)c->F, (float*)c->vp, c->flags, c->sc, c->N, p.flag2D, p.Ma,
                        p.positivity);
+  } else if (p.precision == HYQ_PREC_MIXED_NEWTON || c->mlp == nullptr) {
+    hipLaunchKernelGGL((k_pass1<float, 1>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
+                       (float*)c->F, (float*)c->vp, c->flags, c->sc, c->N, p.flag2D, p.Ma,
+                       p.positivity);
+  } else {
+    // learned: projection + features, network warm starts, wave speeds + fluxes
+    hipLaunchKernelGGL(k_pass1a, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr, c->X, c->sc,
+                       c->N, p.flag2D, p.Ma);
+    const int rc = hyq_mlp_infer_async(c->mlp, c->X, 5 * c->N, c->P, c->stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pass1b, dim3(g), dim3(kBlock), 0, c->stream, c->Mr, (float*)c->F,
+                       (float*)c->vp, c->flags, c->P, c->sc, c->N, p.flag2D, p.Ma, p.positivity);
   }
   if (ev) hipEventRecord(ev[1], c->stream);
   hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, c->stream, c->sc, p.CFL, hmin, p.dtmax, p.Kn);
@@ -393,11 +524,14 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
                        c->dx, c->dy, c->dz);
   }
   if (p.precision == HYQ_PREC_FP64) {
-    hipLaunchKernelGGL(k_pass2<false>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->sc, c->N,
-                       p.flag2D, p.Ma, p.positivity, p.Kn);
+    hipLaunchKernelGGL((k_pass2<false, 0>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->sc,
+                       c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
+  } else if (p.precision == HYQ_PREC_MIXED) {
+    hipLaunchKernelGGL((k_pass2<true, 0>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->sc,
+                       c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
   } else {
-    hipLaunchKernelGGL(k_pass2<true>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->sc, c->N,
-                       p.flag2D, p.Ma, p.positivity, p.Kn);
+    hipLaunchKernelGGL((k_pass2<true, 1>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->sc,
+                       c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
   }
   if (ev) hipEventRecord(ev[3], c->stream);
   const hipError_t e = hipGetLastError();
@@ -531,7 +665,7 @@ int hyq_wavespeed_batch(const double* M, int64_t n, double Ma, int32_t flag2D,
   // the three flux outputs must be one [105][n] block for the pass-1 kernel
   if (Fy != Fx + (size_t)35 * n || Fz != Fx + (size_t)70 * n)
     return fail(HYQ_EINVAL, "Fx, Fy, Fz must be consecutive [35][n] blocks");
-  hipLaunchKernelGGL(k_pass1<double>, dim3(grid_for(n)), dim3(kBlock), 0, 0, M, Mr, Fx, vp, flags,
+  hipLaunchKernelGGL((k_pass1<double, 0>), dim3(grid_for(n)), dim3(kBlock), 0, 0, M, Mr, Fx, vp, flags,
                      (DevScalars*)nullptr, (long long)n, flag2D, Ma, positivity);
   HIPCHK(hipGetLastError(), "pass1 launch");
   HIPCHK(hipDeviceSynchronize(), "pass1 sync");
@@ -544,6 +678,25 @@ int hyq_eig6_batch(const double* m15, int64_t n, double* wr, double* wi) {
   hipLaunchKernelGGL(k_eig6_batch, dim3(grid_for(n)), dim3(kBlock), 0, 0, m15, (long long)n, wr, wi);
   HIPCHK(hipGetLastError(), "eig6 launch");
   HIPCHK(hipDeviceSynchronize(), "eig6 sync");
+  return HYQ_OK;
+}
+
+int hyq_set_mlp(hyq_ctx* c, hyq_mlp* mlp) {
+  if (!c) return fail(HYQ_EINVAL, "null argument");
+  if (mlp && c->p.precision != HYQ_PREC_MIXED_LEARNED)
+    return fail(HYQ_EINVAL, "a warm-start network needs precision HYQ_PREC_MIXED_LEARNED");
+  c->mlp = mlp;
+  return HYQ_OK;
+}
+
+int hyq_plane_dataset(const double* M, int64_t n, float* X, float* L, uint8_t* valid) {
+  if (!M || !X || n < 0 || ((L == nullptr) != (valid == nullptr)))
+    return fail(HYQ_EINVAL, "bad argument");
+  if (n == 0) return HYQ_OK;
+  hipLaunchKernelGGL(k_plane_dataset, dim3(grid_for(n)), dim3(kBlock), 0, 0, M, (long long)n, X, L,
+                     valid);
+  HIPCHK(hipGetLastError(), "dataset launch");
+  HIPCHK(hipDeviceSynchronize(), "dataset sync");
   return HYQ_OK;
 }
 

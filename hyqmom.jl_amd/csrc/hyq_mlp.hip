@@ -1,0 +1,401 @@
+// Learned warm start: a 15 -> 128 -> 128 -> 128 -> 2 MLP (GELU) that predicts the extreme
+// eigenvalues of the standardized flux Jacobian from the 15 standardized moments of a plane.
+// Matrix products run on MFMA (v_mfma_f32_16x16x16_bf16: bf16 operands, FP32 accumulate);
+// weights are kept as FP32 master copies and rounded to bf16 as they are loaded.  Training
+// (forward, backward, Adam) and inference use the kernels of this file only.
+//
+// Parameter block (FP32, this order): W1 [16][128] (input padded to 16), b1 [128],
+// W2 [128][128], b2 [128], W3 [128][128], b3 [128], W4 [128][16] (outputs padded to 16,
+// columns >= 2 held at zero), b4 [16], then input mean/std [16]+[16] and output mean/std
+// [2]+[2] used to normalize features and labels.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hyqmom.h"
+
+namespace {
+
+thread_local std::string g_mlp_err;
+int mfail(int code, const char* what, hipError_t e = hipSuccess) {
+  g_mlp_err = what;
+  if (e != hipSuccess) {
+    g_mlp_err += ": ";
+    g_mlp_err += hipGetErrorString(e);
+  }
+  return code;
+}
+#define MCHK(x, msg)                                       \
+  do {                                                     \
+    hipError_t _e = (x);                                   \
+    if (_e != hipSuccess) return mfail(HYQ_EHIP, msg, _e); \
+  } while (0)
+
+constexpr int IN = 16, HID = 128, OUT = 16;
+constexpr size_t OFF_W1 = 0, OFF_B1 = OFF_W1 + IN * HID, OFF_W2 = OFF_B1 + HID,
+                 OFF_B2 = OFF_W2 + HID * HID, OFF_W3 = OFF_B2 + HID, OFF_B3 = OFF_W3 + HID * HID,
+                 OFF_W4 = OFF_B3 + HID, OFF_B4 = OFF_W4 + HID * OUT, NPARAM = OFF_B4 + OUT;
+constexpr size_t OFF_XMU = NPARAM, OFF_XSD = OFF_XMU + 16, OFF_YMU = OFF_XSD + 16,
+                 OFF_YSD = OFF_YMU + 2, NBLOB = OFF_YSD + 2;
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short f2bf(float f) {  // round to nearest even
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+
+// C[M x N] = A[M x K] . B[K x N] (+ bias), A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn].
+// One wave computes rows [16 r, 16 r + 16) x columns [16 NT c, 16 NT (c+1)); lane l feeds
+// A rows l%16 and B columns l%16 with the four k values 4(l/16)..+3 of each 16-wide k step,
+// and receives D rows 4(l/16)..+3 of column l%16.
+// Epilogue: act 0 = none, 1 = GELU; Z gets the pre-activation, H the activation (either may
+// be null); stores only rows < M and columns < Nst.
+template <int NT>
+__global__ __launch_bounds__(256) void k_gemm(const float* __restrict__ A, long sam, long sak,
+                                              const float* __restrict__ B, long sbk, long sbn,
+                                              int M, int N, int K, const float* __restrict__ bias,
+                                              float* Z, float* H, long ldc, int Nst, int act) {
+  const int lane = threadIdx.x & 63;
+  const long rt = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int ct = blockIdx.y;
+  if (rt * 16 >= M) return;
+  const int r = lane & 15, q = lane >> 4;
+  const long row = rt * 16 + r;
+  f4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    s4 a;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int k = k0 + 4 * q + e;
+      const float v = (row < M && k < K) ? A[row * sam + (long)k * sak] : 0.f;
+      a[e] = f2bf(v);
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const long col = (long)(ct * NT + t) * 16 + r;
+      s4 b;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int k = k0 + 4 * q + e;
+        const float v = (col < N && k < K) ? B[(long)k * sbk + col * sbn] : 0.f;
+        b[e] = f2bf(v);
+      }
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const long col = (long)(ct * NT + t) * 16 + r;
+    if (col >= Nst) continue;
+    const float bv = bias ? bias[col] : 0.f;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const long orow = rt * 16 + 4 * q + e;
+      if (orow >= M) continue;
+      const float z = acc[t][e] + bv;
+      if (Z) Z[orow * ldc + col] = z;
+      if (H) H[orow * ldc + col] = act == 1 ? 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)) : z;
+    }
+  }
+}
+
+// dZ = dH * GELU'(Z)
+__global__ void k_gelu_bwd(const float* __restrict__ Z, float* __restrict__ dH, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float z = Z[i];
+  const float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
+  const float pdf = 0.3989422804014327f * expf(-0.5f * z * z);
+  dH[i] *= cdf + z * pdf;
+}
+// normalized features: Xn[b][j] = (X[b][j] - mu[j]) / sd[j]  (j < 15), 0 for j = 15
+__global__ void k_normalize(const float* __restrict__ X, const float* __restrict__ stat,
+                            float* __restrict__ Xn, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * 16) return;
+  const int j = (int)(i & 15);
+  Xn[i] = j < 15 ? (X[i] - stat[j]) / stat[16 + j] : 0.0f;
+}
+// loss gradient of L = 1/(2B) sum (Y - Tn)^2 on the two outputs; Y rows of 16.
+__global__ void k_mse_grad(const float* __restrict__ Y, const float* __restrict__ T,
+                           const float* __restrict__ ystat, float* __restrict__ dY, long n,
+                           float* loss) {
+  const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.f;
+  if (b < n) {
+    for (int o = 0; o < 16; ++o) dY[b * 16 + o] = 0.f;
+    for (int o = 0; o < 2; ++o) {
+      const float tn = (T[b * 2 + o] - ystat[o]) / ystat[2 + o];
+      const float d = Y[b * 16 + o] - tn;
+      dY[b * 16 + o] = d / (float)n;
+      l += 0.5f * d * d / (float)n;
+    }
+  }
+  // wave reduction then one atomic per wave
+  for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
+  if ((threadIdx.x & 63) == 0) atomicAdd(loss, l);
+}
+__global__ void k_colsum(const float* __restrict__ D, long n, int ncol, float* __restrict__ out) {
+  const int c = blockIdx.x;
+  float s = 0.f;
+  for (long b = threadIdx.x; b < n; b += blockDim.x) s += D[b * ncol + c];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  __shared__ float red[16];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float t = 0.f;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+    out[c] = t;
+  }
+}
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, long n, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi, vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+// outputs columns >= 2 of layer 4 are held at zero
+__global__ void k_mask_w4(float* __restrict__ p) {
+  const int i = threadIdx.x + blockIdx.x * blockDim.x;
+  if (i >= HID * OUT + OUT) return;
+  if (i < HID * OUT) {
+    if ((i % OUT) >= 2) p[OFF_W4 + i] = 0.f;
+  } else if (i - HID * OUT >= 2) {
+    p[OFF_B4 + (i - HID * OUT)] = 0.f;
+  }
+}
+// de-normalized predictions (lmin, lmax) from the padded output rows
+__global__ void k_denorm(const float* __restrict__ Y, const float* __restrict__ ystat,
+                         float* __restrict__ P, long n) {
+  const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  P[2 * b] = Y[b * 16] * ystat[2] + ystat[0];
+  P[2 * b + 1] = Y[b * 16 + 1] * ystat[3] + ystat[1];
+}
+
+unsigned g1(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+template <int NT>
+void gemm(hipStream_t s, const float* A, long sam, long sak, const float* B, long sbk, long sbn,
+          long M, int N, long K, const float* bias, float* Z, float* H, long ldc, int Nst, int act) {
+  dim3 grid((unsigned)((M + 63) / 64), (unsigned)((N + 16 * NT - 1) / (16 * NT)));
+  hipLaunchKernelGGL((k_gemm<NT>), grid, dim3(256), 0, s, A, sam, sak, B, sbk, sbn, (int)M, N,
+                     (int)K, bias, Z, H, ldc, Nst, act);
+}
+
+}  // namespace
+
+struct hyq_mlp {
+  float* blob;   // parameters + normalization (device)
+  float* grad;   // NPARAM
+  float* m1;     // Adam moments
+  float* m2;
+  long cap;      // training activation capacity (samples)
+  float *Xn, *Z1, *H1, *Z2, *H2, *Z3, *H3, *Y, *dY, *dA, *dB;
+  long icap;     // inference capacity: normalized input, two hidden ping-pong buffers, output
+  float *iX, *iA, *iB, *iY;
+  float* loss;
+  long step;
+  hipStream_t stream;
+};
+
+static int ensure_icap(hyq_mlp* m, long n) {
+  if (n <= m->icap) return HYQ_OK;
+  float** bufs[] = {&m->iX, &m->iA, &m->iB, &m->iY};
+  const long width[] = {IN, HID, HID, OUT};
+  for (int i = 0; i < 4; ++i) {
+    (void)hipFree(*bufs[i]);
+    *bufs[i] = nullptr;
+    if (hipMalloc(bufs[i], sizeof(float) * width[i] * n) != hipSuccess)
+      return mfail(HYQ_ENOMEM, "mlp inference allocation");
+  }
+  m->icap = n;
+  return HYQ_OK;
+}
+
+static int ensure_cap(hyq_mlp* m, long n) {
+  if (n <= m->cap) return HYQ_OK;
+  float** bufs[] = {&m->Xn, &m->Z1, &m->H1, &m->Z2, &m->H2, &m->Z3, &m->H3, &m->Y, &m->dY, &m->dA, &m->dB};
+  const long width[] = {IN, HID, HID, HID, HID, HID, HID, OUT, OUT, HID, HID};
+  for (int i = 0; i < 11; ++i) {
+    (void)hipFree(*bufs[i]);
+    *bufs[i] = nullptr;
+    if (hipMalloc(bufs[i], sizeof(float) * width[i] * n) != hipSuccess)
+      return mfail(HYQ_ENOMEM, "mlp activation allocation");
+  }
+  m->cap = n;
+  return HYQ_OK;
+}
+
+// forward on normalized inputs m->Xn (n rows); fills Z*, H*, Y
+static void forward(hyq_mlp* m, long n) {
+  const float* p = m->blob;
+  gemm<8>(m->stream, m->Xn, IN, 1, p + OFF_W1, HID, 1, n, HID, IN, p + OFF_B1, m->Z1, m->H1, HID, HID, 1);
+  gemm<8>(m->stream, m->H1, HID, 1, p + OFF_W2, HID, 1, n, HID, HID, p + OFF_B2, m->Z2, m->H2, HID, HID, 1);
+  gemm<8>(m->stream, m->H2, HID, 1, p + OFF_W3, HID, 1, n, HID, HID, p + OFF_B3, m->Z3, m->H3, HID, HID, 1);
+  gemm<1>(m->stream, m->H3, HID, 1, p + OFF_W4, OUT, 1, n, OUT, HID, p + OFF_B4, nullptr, m->Y, OUT, OUT, 0);
+}
+
+// gradients of the loss into m->grad (forward must have run)
+static void backward(hyq_mlp* m, long n) {
+  const float* p = m->blob;
+  float* g = m->grad;
+  hipStream_t s = m->stream;
+  // layer 4
+  gemm<1>(s, m->H3, 1, HID, m->dY, OUT, 1, HID, OUT, n, nullptr, nullptr, g + OFF_W4, OUT, OUT, 0);
+  hipLaunchKernelGGL(k_colsum, dim3(OUT), dim3(256), 0, s, m->dY, n, OUT, g + OFF_B4);
+  gemm<8>(s, m->dY, OUT, 1, p + OFF_W4, 1, OUT, n, HID, OUT, nullptr, nullptr, m->dA, HID, HID, 0);
+  hipLaunchKernelGGL(k_gelu_bwd, dim3(g1(n * HID)), dim3(256), 0, s, m->Z3, m->dA, n * HID);
+  // layer 3
+  gemm<8>(s, m->H2, 1, HID, m->dA, HID, 1, HID, HID, n, nullptr, nullptr, g + OFF_W3, HID, HID, 0);
+  hipLaunchKernelGGL(k_colsum, dim3(HID), dim3(256), 0, s, m->dA, n, HID, g + OFF_B3);
+  gemm<8>(s, m->dA, HID, 1, p + OFF_W3, 1, HID, n, HID, HID, nullptr, nullptr, m->dB, HID, HID, 0);
+  hipLaunchKernelGGL(k_gelu_bwd, dim3(g1(n * HID)), dim3(256), 0, s, m->Z2, m->dB, n * HID);
+  // layer 2
+  gemm<8>(s, m->H1, 1, HID, m->dB, HID, 1, HID, HID, n, nullptr, nullptr, g + OFF_W2, HID, HID, 0);
+  hipLaunchKernelGGL(k_colsum, dim3(HID), dim3(256), 0, s, m->dB, n, HID, g + OFF_B2);
+  gemm<8>(s, m->dB, HID, 1, p + OFF_W2, 1, HID, n, HID, HID, nullptr, nullptr, m->dA, HID, HID, 0);
+  hipLaunchKernelGGL(k_gelu_bwd, dim3(g1(n * HID)), dim3(256), 0, s, m->Z1, m->dA, n * HID);
+  // layer 1
+  gemm<8>(s, m->Xn, 1, IN, m->dA, HID, 1, IN, HID, n, nullptr, nullptr, g + OFF_W1, HID, HID, 0);
+  hipLaunchKernelGGL(k_colsum, dim3(HID), dim3(256), 0, s, m->dA, n, HID, g + OFF_B1);
+}
+
+extern "C" {
+
+const char* hyq_mlp_last_error(void) { return g_mlp_err.c_str(); }
+int64_t hyq_mlp_blob_size(void) { return (int64_t)NBLOB; }
+int64_t hyq_mlp_param_count(void) { return (int64_t)NPARAM; }
+
+int hyq_mlp_create(const float* blob_host, hyq_mlp** out) {
+  if (!out) return mfail(HYQ_EINVAL, "null argument");
+  hyq_mlp* m = new hyq_mlp();
+  std::memset(m, 0, sizeof(*m));
+  if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete m;
+    return mfail(HYQ_EHIP, "mlp stream");
+  }
+  bool ok = hipMalloc(&m->blob, sizeof(float) * NBLOB) == hipSuccess &&
+            hipMalloc(&m->grad, sizeof(float) * NPARAM) == hipSuccess &&
+            hipMalloc(&m->m1, sizeof(float) * NPARAM) == hipSuccess &&
+            hipMalloc(&m->m2, sizeof(float) * NPARAM) == hipSuccess &&
+            hipMalloc(&m->loss, sizeof(float)) == hipSuccess;
+  if (!ok) {
+    delete m;
+    return mfail(HYQ_ENOMEM, "mlp allocation");
+  }
+  if (blob_host) {
+    MCHK(hipMemcpy(m->blob, blob_host, sizeof(float) * NBLOB, hipMemcpyHostToDevice), "mlp blob");
+  } else {
+    MCHK(hipMemset(m->blob, 0, sizeof(float) * NBLOB), "mlp blob");
+  }
+  MCHK(hipMemset(m->m1, 0, sizeof(float) * NPARAM), "adam m");
+  MCHK(hipMemset(m->m2, 0, sizeof(float) * NPARAM), "adam v");
+  *out = m;
+  return HYQ_OK;
+}
+
+int hyq_mlp_destroy(hyq_mlp* m) {
+  if (!m) return HYQ_OK;
+  (void)hipStreamSynchronize(m->stream);
+  float* bufs[] = {m->blob, m->grad, m->m1, m->m2, m->loss, m->Xn, m->Z1, m->H1, m->Z2,
+                   m->H2, m->Z3, m->H3, m->Y, m->dY, m->dA, m->dB, m->iX, m->iA, m->iB, m->iY};
+  for (float* b : bufs) (void)hipFree(b);
+  (void)hipStreamDestroy(m->stream);
+  delete m;
+  return HYQ_OK;
+}
+
+int hyq_mlp_get_blob(hyq_mlp* m, float* blob_host) {
+  if (!m || !blob_host) return mfail(HYQ_EINVAL, "null argument");
+  MCHK(hipStreamSynchronize(m->stream), "mlp sync");
+  MCHK(hipMemcpy(blob_host, m->blob, sizeof(float) * NBLOB, hipMemcpyDeviceToHost), "mlp blob");
+  return HYQ_OK;
+}
+int hyq_mlp_set_blob(hyq_mlp* m, const float* blob_host) {
+  if (!m || !blob_host) return mfail(HYQ_EINVAL, "null argument");
+  MCHK(hipStreamSynchronize(m->stream), "mlp sync");
+  MCHK(hipMemcpy(m->blob, blob_host, sizeof(float) * NBLOB, hipMemcpyHostToDevice), "mlp blob");
+  m->step = 0;
+  MCHK(hipMemset(m->m1, 0, sizeof(float) * NPARAM), "adam m");
+  MCHK(hipMemset(m->m2, 0, sizeof(float) * NPARAM), "adam v");
+  return HYQ_OK;
+}
+
+// Predicted (lmin, lmax) for n feature rows X [n][16] (device) into P [n][2] (device),
+// enqueued on `stream` without synchronizing.
+int hyq_mlp_infer_async(hyq_mlp* m, const float* X, int64_t n, float* P, hipStream_t stream) {
+  if (!m || !X || !P || n < 0) return mfail(HYQ_EINVAL, "bad argument");
+  if (n == 0) return HYQ_OK;
+  int rc = ensure_icap(m, n);
+  if (rc) return rc;
+  const float* p = m->blob;
+  hipLaunchKernelGGL(k_normalize, dim3(g1(n * 16)), dim3(256), 0, stream, X, p + OFF_XMU, m->iX, (long)n);
+  gemm<8>(stream, m->iX, IN, 1, p + OFF_W1, HID, 1, n, HID, IN, p + OFF_B1, nullptr, m->iA, HID, HID, 1);
+  gemm<8>(stream, m->iA, HID, 1, p + OFF_W2, HID, 1, n, HID, HID, p + OFF_B2, nullptr, m->iB, HID, HID, 1);
+  gemm<8>(stream, m->iB, HID, 1, p + OFF_W3, HID, 1, n, HID, HID, p + OFF_B3, nullptr, m->iA, HID, HID, 1);
+  gemm<1>(stream, m->iA, HID, 1, p + OFF_W4, OUT, 1, n, OUT, HID, p + OFF_B4, nullptr, m->iY, OUT, OUT, 0);
+  hipLaunchKernelGGL(k_denorm, dim3(g1(n)), dim3(256), 0, stream, m->iY, p + OFF_YMU, P, (long)n);
+  MCHK(hipGetLastError(), "mlp infer launch");
+  return HYQ_OK;
+}
+int hyq_mlp_infer(hyq_mlp* m, const float* X, int64_t n, float* P) {
+  if (!m) return mfail(HYQ_EINVAL, "null argument");
+  int rc = hyq_mlp_infer_async(m, X, n, P, m->stream);
+  if (rc) return rc;
+  MCHK(hipStreamSynchronize(m->stream), "mlp infer sync");
+  return HYQ_OK;
+}
+
+// Loss and parameter gradients on one batch (X [n][16], T [n][2], device); grad_host may
+// be null.  No update.
+int hyq_mlp_grad(hyq_mlp* m, const float* X, const float* T, int64_t n, float* loss_out,
+                 float* grad_host) {
+  if (!m || !X || !T || n <= 0) return mfail(HYQ_EINVAL, "bad argument");
+  int rc = ensure_cap(m, n);
+  if (rc) return rc;
+  MCHK(hipMemsetAsync(m->loss, 0, sizeof(float), m->stream), "loss reset");
+  hipLaunchKernelGGL(k_normalize, dim3(g1(n * 16)), dim3(256), 0, m->stream, X, m->blob + OFF_XMU,
+                     m->Xn, (long)n);
+  forward(m, n);
+  hipLaunchKernelGGL(k_mse_grad, dim3(g1(n)), dim3(256), 0, m->stream, m->Y, T, m->blob + OFF_YMU,
+                     m->dY, (long)n, m->loss);
+  backward(m, n);
+  MCHK(hipGetLastError(), "mlp grad launch");
+  MCHK(hipStreamSynchronize(m->stream), "mlp grad sync");
+  if (loss_out) MCHK(hipMemcpy(loss_out, m->loss, sizeof(float), hipMemcpyDeviceToHost), "loss");
+  if (grad_host)
+    MCHK(hipMemcpy(grad_host, m->grad, sizeof(float) * NPARAM, hipMemcpyDeviceToHost), "grad");
+  return HYQ_OK;
+}
+
+// One Adam step on a batch.
+int hyq_mlp_train_step(hyq_mlp* m, const float* X, const float* T, int64_t n, float lr,
+                       float* loss_out) {
+  int rc = hyq_mlp_grad(m, X, T, n, loss_out, nullptr);
+  if (rc) return rc;
+  m->step += 1;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - powf(b1, (float)m->step), c2 = 1.f - powf(b2, (float)m->step);
+  hipLaunchKernelGGL(k_adam, dim3(g1(NPARAM)), dim3(256), 0, m->stream, m->blob, m->grad, m->m1,
+                     m->m2, (long)NPARAM, lr, b1, b2, 1e-8f, c1, c2);
+  hipLaunchKernelGGL(k_mask_w4, dim3(g1(HID * OUT + OUT)), dim3(256), 0, m->stream, m->blob);
+  MCHK(hipGetLastError(), "adam launch");
+  return HYQ_OK;
+}
+
+}  // extern "C"

@@ -1,6 +1,6 @@
 // The per-cell passes of one time step (simulation_runner.jl:352-380 and :526-556).
 #pragma once
-#include "hyq_eig.h"
+#include "hyq_fasteig.h"
 
 namespace hyq {
 
@@ -35,6 +35,45 @@ HD void pack35(const double* c, double* M) {
   for (int n = 0; n < 35; ++n) M[n] = c[T35(n)];
 }
 
+// Eigen solve of one plane in mode EM: 0 = full FP64 solve (QR); 1 = Sturm test + Newton
+// from warm starts (wmin/wmax in standardized units, NaN = analytic bounds), falling back
+// to the full solve when hyperbolicity is not proven or Newton does not converge.
+template <int EM>
+HD EigRes plane_solve(const Plane& p, float wmin, float wmax, bool need_vals, uint32_t& fl) {
+  if (EM == 0) return plane_eigs(p, fl);
+  if (p.ok) {
+    double J[36];
+    jacobian_std(p, J);
+    bool fin = true;
+#pragma unroll
+    for (int t = 0; t < 36; ++t) fin = fin && (fabs(J[t]) < 1e300);
+    if (fin) {
+      const FastEig fe = fast_eigs_std(J, (double)wmin, (double)wmax, need_vals);
+      if (fe.real) {
+        EigRes r;
+        r.vmin = need_vals ? p.a + p.sig * fe.lmin : 0.0;
+        r.vmax = need_vals ? p.a + p.sig * fe.lmax : 0.0;
+        r.cplx = false;
+        r.ok = true;
+        return r;
+      }
+    }
+  }
+  fl |= F_FALLBACK;
+  return plane_eigs(p, fl);
+}
+// Warm starts of the five distinct plane Jacobians of a cell (UV, UW, VU, VW, WU), each
+// (min, max); NaN when absent.
+struct Warm {
+  float w[10];
+};
+HD Warm no_warm() {
+  Warm w;
+#pragma unroll
+  for (int t = 0; t < 10; ++t) w.w[t] = NAN;
+  return w;
+}
+
 // Minimum/maximum over two Jacobians with compute_jacobian_eigenvalues.jl's NaN guard: a
 // non-finite first Jacobian makes both NaN; a non-finite second one is skipped.
 HD void combine_guarded(const EigRes& A, const EigRes& B, double& vmin, double& vmax) {
@@ -48,19 +87,20 @@ HD void combine_guarded(const EigRes& A, const EigRes& B, double& vmin, double& 
 
 // eigenvalues6_hyperbolic_3D (src/numerics/eigenvalues6_hyperbolic_3D.jl:22-155).
 // AXIS 0: UV and UW Jacobians; AXIS 1: VU and VW.  In: M.  Out: Mr (may alias nothing).
-template <int AXIS>
+template <int AXIS, int EM = 0>
 HD void eig6_axis(const double* M, int flag2D, double Ma, double& vmin, double& vmax,
-                  double* Mr, uint32_t& fl) {
+                  double* Mr, uint32_t& fl, const Warm& wm_ = no_warm(), bool need_vals = true) {
   double c[35];
   double M000, um, vm, wm;
   raw_to_central4(M, c, M000, um, vm, wm);
   EigRes A, B;
+  const int ia = AXIS == 0 ? 0 : 2, ib = AXIS == 0 ? 1 : 3;  // Warm slots: UV UW VU VW WU
   if (AXIS == 0) {
-    A = plane_eigs(plane<0, 1>(c, um, vm, wm), fl);
-    B = plane_eigs(plane<0, 2>(c, um, vm, wm), fl);
+    A = plane_solve<EM>(plane<0, 1>(c, um, vm, wm), wm_.w[2 * ia], wm_.w[2 * ia + 1], need_vals, fl);
+    B = plane_solve<EM>(plane<0, 2>(c, um, vm, wm), wm_.w[2 * ib], wm_.w[2 * ib + 1], need_vals, fl);
   } else {
-    A = plane_eigs(plane<1, 0>(c, um, vm, wm), fl);
-    B = plane_eigs(plane<1, 2>(c, um, vm, wm), fl);
+    A = plane_solve<EM>(plane<1, 0>(c, um, vm, wm), wm_.w[2 * ia], wm_.w[2 * ia + 1], need_vals, fl);
+    B = plane_solve<EM>(plane<1, 2>(c, um, vm, wm), wm_.w[2 * ib], wm_.w[2 * ib + 1], need_vals, fl);
   }
   combine_guarded(A, B, vmin, vmax);
   if (A.cplx || B.cplx) {
@@ -82,11 +122,11 @@ HD void eig6_axis(const double* M, int flag2D, double Ma, double& vmin, double& 
     flux_closure35<false, double>(Mh, flag2D, Ma, Mr, nullptr, nullptr, nullptr, 0, fl);
     raw_to_central4(Mr, c, M000, um, vm, wm);
     if (AXIS == 0) {
-      A = plane_eigs(plane<0, 1>(c, um, vm, wm), fl);
-      B = plane_eigs(plane<0, 2>(c, um, vm, wm), fl);
+      A = plane_solve<EM>(plane<0, 1>(c, um, vm, wm), wm_.w[2 * ia], wm_.w[2 * ia + 1], need_vals, fl);
+      B = plane_solve<EM>(plane<0, 2>(c, um, vm, wm), wm_.w[2 * ib], wm_.w[2 * ib + 1], need_vals, fl);
     } else {
-      A = plane_eigs(plane<1, 0>(c, um, vm, wm), fl);
-      B = plane_eigs(plane<1, 2>(c, um, vm, wm), fl);
+      A = plane_solve<EM>(plane<1, 0>(c, um, vm, wm), wm_.w[2 * ia], wm_.w[2 * ia + 1], need_vals, fl);
+      B = plane_solve<EM>(plane<1, 2>(c, um, vm, wm), wm_.w[2 * ib], wm_.w[2 * ib + 1], need_vals, fl);
     }
     combine_guarded(A, B, vmin, vmax);
   } else {
@@ -97,8 +137,9 @@ HD void eig6_axis(const double* M, int flag2D, double Ma, double& vmin, double& 
 
 // eigenvalues6z_hyperbolic_3D (src/numerics/eigenvalues6z_hyperbolic_3D.jl:23-365): the WU
 // Jacobian (flux along z, second variable x) and, as the reference maps it, the VW one.
+template <int EM = 0>
 HD void eig6_z(const double* M, int flag2D, double Ma, bool positivity, double& vmin,
-               double& vmax, double* Mr, uint32_t& fl) {
+               double& vmax, double* Mr, uint32_t& fl, const Warm& wm_ = no_warm()) {
 #pragma unroll
   for (int n = 0; n < 35; ++n) Mr[n] = M[n];
   if (positivity && M[0] <= 0.0) {
@@ -124,8 +165,8 @@ HD void eig6_z(const double* M, int flag2D, double Ma, bool positivity, double& 
   double c[35];
   double M000, um, vm, wm;
   raw_to_central4(W, c, M000, um, vm, wm);
-  EigRes A = plane_eigs(plane<2, 0>(c, um, vm, wm), fl);
-  EigRes B = plane_eigs(plane<1, 2>(c, um, vm, wm), fl);
+  EigRes A = plane_solve<EM>(plane<2, 0>(c, um, vm, wm), wm_.w[8], wm_.w[9], true, fl);
+  EigRes B = plane_solve<EM>(plane<1, 2>(c, um, vm, wm), wm_.w[6], wm_.w[7], true, fl);
   if (!A.ok || !B.ok) fl |= F_NONFIN;  // the reference's eigvals throws here
   vmin = jmin(A.vmin, B.vmin);
   vmax = jmax(A.vmax, B.vmax);
@@ -142,8 +183,8 @@ HD void eig6_z(const double* M, int flag2D, double Ma, bool positivity, double& 
     pack35(cn, Mh);
     flux_closure35<false, double>(Mh, flag2D, Ma, Mr, nullptr, nullptr, nullptr, 0, fl);
     raw_to_central4(Mr, c, M000, um, vm, wm);
-    A = plane_eigs(plane<2, 0>(c, um, vm, wm), fl);
-    B = plane_eigs(plane<1, 2>(c, um, vm, wm), fl);
+    A = plane_solve<EM>(plane<2, 0>(c, um, vm, wm), wm_.w[8], wm_.w[9], true, fl);
+    B = plane_solve<EM>(plane<1, 2>(c, um, vm, wm), wm_.w[6], wm_.w[7], true, fl);
     vmin = jmin(A.vmin, B.vmin);
     vmax = jmax(A.vmax, B.vmax);
   }
@@ -151,15 +192,19 @@ HD void eig6_z(const double* M, int flag2D, double Ma, bool positivity, double& 
 
 // Pass 1 of the step for one cell (simulation_runner.jl:356-379).  Outputs the projected
 // state Mr, the fluxes (strided), and the six wave speeds.
-template <typename FT = double>
-HD void pass1_cell(const double* M, int flag2D, double Ma, bool positivity, double* Mr, FT* Fx,
-                   FT* Fy, FT* Fz, size_t st, double vp[6], uint32_t& fl) {
-  double A[35], B[35];
-  double xmn, xmx, ymn, ymx, zmn, zmx;
+// First half of pass 1: the projection, returned in A (the state the wave speeds see).
+HD void pass1a_cell(const double* M, int flag2D, double Ma, double* A, uint32_t& fl) {
   flux_closure35<false, double>(M, flag2D, Ma, A, nullptr, nullptr, nullptr, 0, fl);
-  eig6_axis<0>(A, flag2D, Ma, xmn, xmx, B, fl);
-  eig6_axis<1>(B, flag2D, Ma, ymn, ymx, A, fl);
-  eig6_z(A, flag2D, Ma, positivity, zmn, zmx, B, fl);
+}
+// Second half of pass 1 from the projected state A.
+template <typename FT = double, int EM = 0>
+HD void pass1b_cell(double* A, int flag2D, double Ma, bool positivity, double* Mr, FT* Fx,
+                    FT* Fy, FT* Fz, size_t st, double vp[6], uint32_t& fl, const Warm& w) {
+  double B[35];
+  double xmn, xmx, ymn, ymx, zmn, zmx;
+  eig6_axis<0, EM>(A, flag2D, Ma, xmn, xmx, B, fl, w);
+  eig6_axis<1, EM>(B, flag2D, Ma, ymn, ymx, A, fl, w);
+  eig6_z<EM>(A, flag2D, Ma, positivity, zmn, zmx, B, fl, w);
   flux_closure35<true, FT>(B, flag2D, Ma, Mr, Fx, Fy, Fz, st, fl);
   double a, b;
   closure1d_speeds(Mr[0], Mr[1], Mr[2], Mr[3], Mr[4], a, b);
@@ -171,6 +216,13 @@ HD void pass1_cell(const double* M, int flag2D, double Ma, bool positivity, doub
   closure1d_speeds(Mr[0], Mr[15], Mr[19], Mr[22], Mr[24], a, b);
   vp[4] = jmin(a, zmn);
   vp[5] = jmax(b, zmx);
+}
+template <typename FT = double, int EM = 0>
+HD void pass1_cell(const double* M, int flag2D, double Ma, bool positivity, double* Mr, FT* Fx,
+                   FT* Fy, FT* Fz, size_t st, double vp[6], uint32_t& fl, const Warm& w = no_warm()) {
+  double A[35];
+  pass1a_cell(M, flag2D, Ma, A, fl);
+  pass1b_cell<FT, EM>(A, flag2D, Ma, positivity, Mr, Fx, Fy, Fz, st, vp, fl, w);
 }
 
 // collision35 (src/numerics/collision35.jl:17-45): relaxation toward the isotropic
@@ -209,14 +261,15 @@ HD void collision_cell(const double* M, double dt, double Kn, bool positivity, d
 
 // Pass 2 for one cell (simulation_runner.jl:526-556): realizability, x and y wave-speed
 // corrections, realizability, then BGK.
-template <bool MIXED = false>
+template <bool MIXED = false, int EM = 0>
 HD void pass2_cell(const double* M, int flag2D, double Ma, bool positivity, double dt,
                    double Kn, double* Mo, uint32_t& fl) {
   double A[35], B[35];
   double d0, d1;
   flux_closure35<false, double>(M, flag2D, Ma, A, nullptr, nullptr, nullptr, 0, fl);
-  eig6_axis<0>(A, flag2D, Ma, d0, d1, B, fl);
-  eig6_axis<1>(B, flag2D, Ma, d0, d1, A, fl);
+  // the wave speeds of pass 2 are not used (only the complex-eigenvalue corrections are)
+  eig6_axis<0, EM>(A, flag2D, Ma, d0, d1, B, fl, no_warm(), EM == 0);
+  eig6_axis<1, EM>(B, flag2D, Ma, d0, d1, A, fl, no_warm(), EM == 0);
   flux_closure35<false, double>(A, flag2D, Ma, B, nullptr, nullptr, nullptr, 0, fl);
   collision_cell<MIXED>(B, dt, Kn, positivity, Mo);
 }

@@ -12,7 +12,7 @@ LIBPATH = os.path.join(os.path.dirname(HERE), "build", "libhyqmom_hip.so")
 HYQ_OK = 0
 ERRORS = {-1: "HYQ_EINVAL", -2: "HYQ_ENOMEM", -3: "HYQ_EHIP", -4: "HYQ_ENONFINITE",
           -5: "HYQ_ENOTCONV", -6: "HYQ_ENODEV"}
-PREC = {"fp64": 0, "mixed": 1, "mixed_learned": 2}
+PREC = {"fp64": 0, "mixed": 1, "mixed_learned": 2, "mixed_newton": 3}
 
 FLAG_NAMES = ["UNIVARIATE", "EDGE_CORNER", "FACES", "FLAG220", "ROOTS2D", "ROOTS4", "EIGCORR_X",
               "EIGCORR_Y", "EIGCORR_Z", "ZNEGVAR", "NONFINITE", "RHOCLAMP", "EIG_FALLBACK",
@@ -55,6 +55,18 @@ SIGNATURES = {
                              VP, VP, VP, VP, VP, VP], ctypes.c_int),
     "hyq_eig6_batch": ([VP, I64, VP, VP], ctypes.c_int),
     "hyq_collision_batch": ([VP, I64, ctypes.c_double, ctypes.c_double, ctypes.c_int32, VP], ctypes.c_int),
+    "hyq_mlp_blob_size": ([], ctypes.c_int64),
+    "hyq_mlp_param_count": ([], ctypes.c_int64),
+    "hyq_mlp_create": ([VP, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "hyq_mlp_destroy": ([VP], ctypes.c_int),
+    "hyq_mlp_get_blob": ([VP, VP], ctypes.c_int),
+    "hyq_mlp_set_blob": ([VP, VP], ctypes.c_int),
+    "hyq_mlp_infer": ([VP, VP, I64, VP], ctypes.c_int),
+    "hyq_mlp_grad": ([VP, VP, VP, I64, VP, VP], ctypes.c_int),
+    "hyq_mlp_train_step": ([VP, VP, VP, I64, ctypes.c_float, VP], ctypes.c_int),
+    "hyq_mlp_last_error": ([], ctypes.c_char_p),
+    "hyq_set_mlp": ([VP, VP], ctypes.c_int),
+    "hyq_plane_dataset": ([VP, I64, VP, VP, VP], ctypes.c_int),
     "hyq_last_error": ([], ctypes.c_char_p),
     "hyq_version": ([], ctypes.c_char_p),
 }

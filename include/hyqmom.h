@@ -63,7 +63,9 @@ extern "C" {
 /* precision modes */
 #define HYQ_PREC_FP64 0           /* everything FP64 */
 #define HYQ_PREC_MIXED 1          /* FP32 fluxes, HLL update and BGK relaxation */
-#define HYQ_PREC_MIXED_LEARNED 2  /* MIXED + MLP warm-started wave speeds */
+#define HYQ_PREC_MIXED_LEARNED 2  /* MIXED + MLP warm-started Newton wave speeds */
+#define HYQ_PREC_MIXED_NEWTON 3   /* MIXED + Newton wave speeds from analytic bounds (the
+                                     learned mode without its network, for comparison) */
 
 /* Mirrors the params NamedTuple of simulation_runner (src/simulation_runner.jl:20-37,
  * 74-113).  Fields the device step does not need (snapshot, symmetry, debug, MPI) stay
@@ -131,6 +133,36 @@ int hyq_eig6_batch(const double* m15, int64_t n, double* wr, double* wi);
 /* collision35 (src/numerics/collision35.jl:17-45) on n cells. */
 int hyq_collision_batch(const double* M, int64_t n, double dt, double Kn, int32_t positivity,
                         double* Mout);
+
+/* ---- learned warm start (a 15 -> 128 -> 128 -> 128 -> 2 GELU MLP on bf16 MFMA) ----------
+ * No counterpart in the reference: the network predicts the extreme eigenvalues of the
+ * standardized flux Jacobian of a plane from its 15 standardized moments; the predictions
+ * only seed a Newton solve whose result is exact to FP64 or falls back to the full solve.
+ * Parameters travel as one FP32 blob of hyq_mlp_blob_size() floats (weights, biases, input
+ * and label normalization). */
+typedef struct hyq_mlp hyq_mlp;
+int64_t hyq_mlp_blob_size(void);
+int64_t hyq_mlp_param_count(void);
+int hyq_mlp_create(const float* blob_host /* NULL: zeros */, hyq_mlp** out);
+int hyq_mlp_destroy(hyq_mlp* m);
+int hyq_mlp_get_blob(hyq_mlp* m, float* blob_host);
+int hyq_mlp_set_blob(hyq_mlp* m, const float* blob_host);
+/* X [n][16] features (device), P [n][2] predictions (device). */
+int hyq_mlp_infer(hyq_mlp* m, const float* X, int64_t n, float* P);
+/* loss and gradients (grad_host: hyq_mlp_param_count() floats, may be NULL); T [n][2]. */
+int hyq_mlp_grad(hyq_mlp* m, const float* X, const float* T, int64_t n, float* loss_out,
+                 float* grad_host);
+/* one Adam step (beta 0.9/0.999, eps 1e-8) on a batch */
+int hyq_mlp_train_step(hyq_mlp* m, const float* X, const float* T, int64_t n, float lr,
+                       float* loss_out);
+const char* hyq_mlp_last_error(void);
+/* attach a network to a HYQ_PREC_MIXED_LEARNED context (not owned; NULL detaches, which
+ * makes the context use analytic Newton starts) */
+int hyq_set_mlp(hyq_ctx* ctx, hyq_mlp* mlp);
+/* training data: features X [5n][16] of the five plane Jacobians (UV, UW, VU, VW, WU; sample
+ * = plane*n + cell) of n states M [35][n], and optionally labels L [5n][2] (standardized
+ * extreme eigenvalues from the full FP64 solve) with valid [5n] (1: real and finite). */
+int hyq_plane_dataset(const double* M, int64_t n, float* X, float* L, uint8_t* valid);
 
 /* Text of the last error on the calling thread ("" if none). */
 const char* hyq_last_error(void);

@@ -4,6 +4,7 @@
 #include <cstring>
 #include "../../hyqmom.jl_amd/csrc/hyq_pipeline.h"
 #include "../../hyqmom.jl_amd/csrc/hyq_update.h"
+#include "../../hyqmom.jl_amd/csrc/hyq_fasteig.h"
 #include <vector>
 #include <cmath>
 
@@ -109,4 +110,25 @@ extern "C" void cell_pass2_flags(const double* M, int flag2D, double Ma, double 
   uint32_t fl = 0;
   pass2_cell<false>(M, flag2D, Ma, true, dt, Kn, Mo, fl);
   *fl_out = fl;
+}
+// Plane solve both ways on the standardized Jacobian of plane (f,g) of a 35-moment state.
+// out: [qr_min, qr_max, qr_cplx, fast_min, fast_max, fast_real, fast_iters] (raw scale)
+extern "C" void cell_plane_compare(const double* M, int f, int g, double wmin, double wmax,
+                                   double* out) {
+  double c[35], M000, um, vm, wm;
+  raw_to_central4(M, c, M000, um, vm, wm);
+  Plane p;
+  if (f == 0 && g == 1) p = plane<0, 1>(c, um, vm, wm);
+  else if (f == 0 && g == 2) p = plane<0, 2>(c, um, vm, wm);
+  else if (f == 1 && g == 0) p = plane<1, 0>(c, um, vm, wm);
+  else if (f == 1 && g == 2) p = plane<1, 2>(c, um, vm, wm);
+  else p = plane<2, 0>(c, um, vm, wm);
+  uint32_t fl = 0;
+  const EigRes q = plane_eigs(p, fl);
+  double J[36];
+  jacobian_std(p, J);
+  const FastEig fe = fast_eigs_std(J, wmin, wmax, true);
+  out[0] = q.vmin; out[1] = q.vmax; out[2] = q.cplx ? 1 : 0;
+  out[3] = p.a + p.sig * fe.lmin; out[4] = p.a + p.sig * fe.lmax; out[5] = fe.real ? 1 : 0;
+  out[6] = fe.iters;
 }
