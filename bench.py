@@ -27,7 +27,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "mixed_learned"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "mixed_newton", "mixed_learned"])
+    ap.add_argument("--model", default=None, help="warm-start network blob (mixed_learned)")
     ap.add_argument("--grid", default="120,120,60")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     return ap.parse_args()
@@ -64,6 +65,8 @@ def main():
                   nnmax=10**9, dtmax=1e-2, rhol=1.0, rhor=0.1, T=1.0, r110=0.0, r101=0.0, r011=0.0,
                   symmetry_check_interval=0, homogeneous_z=False, debug_output=False,
                   use_custom_ic=True, ic_background=bg, ic_jets=jets, precision=a.precision)
+    if a.model:
+        params["warmstart_model"] = a.model
     M0 = H.initial_state(params)
     s = H.Solver(params)
     s.set_state(M0)

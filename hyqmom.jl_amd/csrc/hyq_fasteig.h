@@ -159,13 +159,13 @@ HD FastEig fast_eigs_std(const double* J, double wmin, double wmax, bool need_va
     if (g < x1 && sturm_roots_above(S, g) == 0) x1 = g;
   }
   const int k0 = newton_extreme(c, x0, 60), k1 = newton_extreme(c, x1, 60);
+  r.iters = (k0 < 0 ? 60 : k0) + (k1 < 0 ? 60 : k1);
   if (k0 < 0 || k1 < 0) {
     r.real = false;  // not converged: the caller falls back to the full solve
     return r;
   }
   r.lmin = x0;
   r.lmax = x1;
-  r.iters = k0 + k1;
   return r;
 }
 

@@ -49,6 +49,9 @@ HD EigRes plane_solve(const Plane& p, float wmin, float wmax, bool need_vals, ui
     for (int t = 0; t < 36; ++t) fin = fin && (fabs(J[t]) < 1e300);
     if (fin) {
       const FastEig fe = fast_eigs_std(J, (double)wmin, (double)wmax, need_vals);
+      // Newton iterations, summed over the cell's plane solves, in flag bits 16..27
+      const uint32_t it = (fl >> 16) + (uint32_t)fe.iters;
+      fl = (fl & 0xFFFFu) | ((it < 0xFFFu ? it : 0xFFFu) << 16);
       if (fe.real) {
         EigRes r;
         r.vmin = need_vals ? p.a + p.sig * fe.lmin : 0.0;

@@ -19,6 +19,11 @@ FLAG_NAMES = ["UNIVARIATE", "EDGE_CORNER", "FACES", "FLAG220", "ROOTS2D", "ROOTS
               "QR_NOCONV"]
 
 
+def newton_iterations(flags):
+    """Newton iterations per cell (flag bits 16..27) of the fast wave-speed modes."""
+    return (flags >> 16) & 0xFFF
+
+
 class HyqParams(ctypes.Structure):
     _fields_ = [("Nx", ctypes.c_int32), ("Ny", ctypes.c_int32), ("Nz", ctypes.c_int32),
                 ("Nmom", ctypes.c_int32), ("flag2D", ctypes.c_int32), ("positivity", ctypes.c_int32),

@@ -6,10 +6,13 @@ params is a dict (or any mapping / object with attributes) with the reference's 
 names: Nx, Ny, Nz, tmax, Kn, Ma, flag2D, CFL, Nmom, nnmax, dtmax, rhol, rhor, T, r110,
 r101, r011, symmetry_check_interval, homogeneous_z, debug_output; optional positivity,
 snapshot_interval, snapshot_filename, xmin..zmax, use_custom_ic, ic_background, ic_jets,
-and (this build) precision in {"fp64", "mixed", "mixed_learned"}.
+and (this build) precision in {"fp64", "mixed", "mixed_newton", "mixed_learned"} with
+warmstart_model (a warmstart.WarmStartMLP or a path; default models/warmstart_mlp.f32 when
+present) for "mixed_learned".
 """
 import ctypes
 import math
+import os
 import time
 
 import numpy as np
@@ -99,6 +102,22 @@ class Solver:
         self.h = h
         self.t = 0.0
         self.nn = 0
+        self._mlp = None
+        if p.precision == _lib.PREC["mixed_learned"]:
+            model = _get(params, "warmstart_model")
+            if model is None or isinstance(model, str):
+                from .warmstart import DEFAULT_MODEL, WarmStartMLP
+                path = model or DEFAULT_MODEL
+                model = WarmStartMLP.load(path) if os.path.exists(path) else None
+            if model is not None:
+                self.set_warm_start(model)
+
+    def set_warm_start(self, model):
+        """Attach a warmstart.WarmStartMLP (precision "mixed_learned"); None detaches and the
+        Newton solve then starts from the analytic root bounds."""
+        _lib.check(_lib.lib().hyq_set_mlp(self.h, model.handle if model is not None else None),
+                   "hyq_set_mlp")
+        self._mlp = model
 
     def close(self):
         if self.h:

@@ -59,6 +59,10 @@ extern "C" {
 #define HYQ_FLAG_RHOCLAMP (1u << 11)    /* density clamp in the z eigen path */
 #define HYQ_FLAG_EIG_FALLBACK (1u << 12) /* learned warm start fell back to the full solve */
 #define HYQ_FLAG_QR_NOCONV (1u << 13)   /* QR iteration limit reached */
+/* bits 16..27: Newton iterations summed over the cell's fast plane solves (precision
+   HYQ_PREC_MIXED_NEWTON / HYQ_PREC_MIXED_LEARNED; saturates at 4095) */
+#define HYQ_FLAG_NEWTON_SHIFT 16
+#define HYQ_FLAG_NEWTON_MASK 0xFFFu
 
 /* precision modes */
 #define HYQ_PREC_FP64 0           /* everything FP64 */

@@ -1,0 +1,149 @@
+"""The warm-start network on the GPU (bf16 MFMA kernels) against its CPU reference, the
+training dataset against the oracle's eigen solve, and the learned / Newton precision
+modes against the plain mixed step."""
+import numpy as np
+import pytest
+
+import mlp_ref as R
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a, dtype=np.float32):
+    from gpu_util import to_dev
+    return to_dev(np.ascontiguousarray(a, dtype=dtype))
+
+
+@pytest.mark.parametrize("n", [1, 37, 1000])
+def test_forward_matches_reference(n):
+    """Random weights: any mistake in the MFMA operand/result lane layout or the strides
+    gives O(1) errors; bf16 rounding of the operands is reproduced by the reference, so
+    the remaining difference is FP32 accumulation order (and rare bf16 rounding flips)."""
+    from hyqmom_amd.warmstart import WarmStartMLP
+    blob = R.random_blob(11)
+    X = np.random.default_rng(12).standard_normal((n, 16)).astype(np.float32)
+    m = WarmStartMLP(blob)
+    P = m.infer(_dev(X)).cpu().numpy()
+    ref = R.predict(blob, X, use_bf16=True)
+    ysd = R.unpack(blob)["ysd"]
+    err = np.abs(P - ref) / ysd
+    print("n=%d forward max err %.3e mean %.3e" % (n, err.max(), err.mean()))
+    assert err.max() < 2e-2 and err.mean() < 2e-3
+    # and the reference differs from the unrounded FP64 network by more than that
+    ref64 = R.predict(blob, X, use_bf16=False)
+    assert np.abs(P - ref64).max() / ysd.max() > 0 or n == 1
+
+
+def test_gradient_matches_reference():
+    from hyqmom_amd.warmstart import WarmStartMLP
+    blob = R.random_blob(21)
+    rng = np.random.default_rng(22)
+    n = 777
+    X = rng.standard_normal((n, 16)).astype(np.float32)
+    T = rng.standard_normal((n, 2)).astype(np.float32) * 0.5 + np.array([-2.0, 2.0], np.float32)
+    m = WarmStartMLP(blob)
+    loss, g = m.grad(_dev(X), _dev(T))
+    lref, gref = R.loss_and_grad(blob, X, T, use_bf16=True)
+    assert abs(loss - lref) <= 1e-3 * lref, (loss, lref)
+    o = 0
+    for name, shp in R.SHAPES:
+        k = int(np.prod(shp))
+        a, b = g[o:o + k], gref[o:o + k]
+        o += k
+        nb = np.linalg.norm(b)
+        if nb == 0:
+            assert np.all(a == 0), name
+            continue
+        rel = np.linalg.norm(a - b) / nb
+        print("%s grad rel err %.3e" % (name, rel))
+        assert rel < 1e-2, (name, rel)
+
+
+def test_training_reduces_loss():
+    """Adam on a smooth synthetic regression: the loss falls by an order of magnitude."""
+    import torch
+    from hyqmom_amd import warmstart as W
+    rng = np.random.default_rng(31)
+    n = 20000
+    X = rng.standard_normal((n, 16)).astype(np.float32)
+    X[:, 15] = 0.0
+    T = np.stack([-1.5 - 0.3 * np.tanh(X[:, 0] * X[:, 3]) - 0.2 * X[:, 4] ** 2,
+                  1.5 + 0.3 * np.sin(X[:, 6]) + 0.2 * X[:, 7] * X[:, 8]], axis=1).astype(np.float32)
+    Xd, Td = _dev(X), _dev(T)
+    m, hist = W.train(Xd, Td, steps=400, batch=2048, lr=2e-3, lr_final=2e-4, seed=1)
+    print("loss history", hist[0], hist[-1])
+    assert hist[-1][1] < 0.1 * hist[0][1]
+    b1 = m.blob()
+    m2 = W.WarmStartMLP(b1)
+    assert np.array_equal(m2.blob(), b1)
+    P1 = m.infer(Xd[:100]).cpu().numpy()
+    P2 = m2.infer(Xd[:100]).cpu().numpy()
+    assert np.array_equal(P1, P2)
+
+
+def test_plane_dataset_labels_match_oracle():
+    """Features are the 15 standardized plane moments in jacobian6's argument order; labels
+    are the extreme eigenvalues of that Jacobian (oracle: LAPACK-family QR)."""
+    from hyqmom_amd.warmstart import plane_dataset
+    from _common import mixture_raw, to35
+    rng = np.random.default_rng(41)
+    M = np.array([to35(mixture_raw(rng, spread=1.5, lam_lo=0.05)) for _ in range(200)])
+    X, L, V = plane_dataset(_dev(M.T, np.float64))
+    X, L, V = X.cpu().numpy(), L.cpu().numpy(), V.cpu().numpy()
+    n = len(M)
+    assert X.shape == (5 * n, 16) and np.all(X[:, 15] == 0)
+    assert np.all(X[:, [0, 2, 9]] == 1) and np.all(X[:, [1, 5]] == 0)
+    checked = 0
+    for s in range(5 * n):
+        ev = O.eigvals(O.jacobian6(X[s, :15].astype(np.float64)))
+        real = np.max(np.abs(ev.imag)) <= 1e-6 and np.all(np.isfinite(ev))
+        if V[s]:
+            assert real
+            r = np.sort(ev.real)
+            assert abs(L[s, 0] - r[0]) <= 1e-4 * (1 + abs(r[0])), (s, L[s], r)
+            assert abs(L[s, 1] - r[-1]) <= 1e-4 * (1 + abs(r[-1])), (s, L[s], r)
+            checked += 1
+        elif np.max(np.abs(ev.imag)) > 1e-3:
+            pass  # complex: invalid on both sides
+    assert checked > 0.5 * 5 * n
+
+
+def _params(N, Ma, Kn, prec):
+    import hyqmom_amd as H
+    bg, jets = H.crossing_jets(Ma)
+    return dict(Nx=N[0], Ny=N[1], Nz=N[2], tmax=10.0, Kn=Kn, Ma=Ma, flag2D=0, CFL=0.7, Nmom=35,
+                nnmax=100, dtmax=1e-2, rhol=1.0, rhor=0.1, T=1.0, r110=0.0, r101=0.0, r011=0.0,
+                symmetry_check_interval=0, homogeneous_z=False, debug_output=False,
+                use_custom_ic=True, ic_background=bg, ic_jets=jets, precision=prec)
+
+
+def test_newton_and_learned_modes_match_mixed():
+    """From one state, mixed_newton and mixed_learned (an untrained network: any warm start
+    is only a Newton seed) give the wave speeds of the full solve to the Newton tolerance
+    (so dt agrees to 1e-12), hence the same step as the mixed mode up to FP32 rounding."""
+    import hyqmom_amd as H
+    from hyqmom_amd.warmstart import WarmStartMLP
+    N = (24, 24, 12)
+    p = _params(N, 1.5, 0.01, "mixed")
+    M0 = np.asfortranarray(H.initial_state(p))
+    out = {}
+    for prec in ("mixed", "mixed_newton", "mixed_learned"):
+        q = dict(p, precision=prec)
+        if prec == "mixed_learned":
+            q["warmstart_model"] = WarmStartMLP(R.random_blob(51))
+        s = H.Solver(q)
+        s.set_state(M0)
+        dt = s.step()
+        out[prec] = (s.get_state(), s.wave_speeds(), dt, s.flags())
+        s.close()
+    Mm, vm, dtm, _ = out["mixed"]
+    for prec in ("mixed_newton", "mixed_learned"):
+        Mx, vx, dtx, fx = out[prec]
+        assert abs(dtx - dtm) <= 1e-12 * dtm, (prec, dtx, dtm)
+        ev = np.abs(vx - vm) / np.maximum(np.abs(vm).max(axis=1, keepdims=True), 1e-300)
+        print(prec, "wave-speed max rel diff %.2e, fallback cells %d" % (ev.max(), ((fx >> 12) & 1).sum()))
+        assert ev.max() < 2e-7  # mixed modes keep the wave speeds in FP32
+        scale = np.abs(Mm).max(axis=(0, 1, 2))
+        e = (np.abs(Mx - Mm) / scale).max(axis=3)
+        assert e.max() < 1e-6, (prec, e.max())
