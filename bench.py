@@ -1,9 +1,18 @@
 """Benchmark: cell-steps/s of the HyQMOM 3D step at 120x120x60 (crossing jets, Ma=1.5,
-Kn=0.01, CFL=0.7) on one MI355X, through the C ABI of libhyqmom_hip.so.
+Kn=0.01, CFL=0.7, 200 steps) on one MI355X, through the C ABI of libhyqmom_hip.so.
 
-Prints one JSON line.  `value` = Nx*Ny*Nz*steps / wall time of the timed steps (state
-resident in HBM; no host copies inside the timed region).  The roofline object refers to
-the dominant kernel (pass 1), timed with HIP events on the context stream.
+Prints one JSON line.  `value` = Nx*Ny*Nz*steps / wall time of the timed steps of the
+headline mode (default: mixed precision with the learned warm start), state resident in
+HBM, no host copies inside the timed region.  The same workload in plain FP64 is timed
+beside it ("fp64"), and the headline mode's one-step error against FP64 from the same
+state is reported ("accuracy").
+
+Roofline: per kernel (pass 1, update, pass 2) the average duration from HIP events on the
+context stream over the timed steps; pass 1 and pass 2 are priced as FP64 work
+(algorithmic FLOPs counted on the states of the run by the op-counting host build of the
+same cell code) against the 78.6 TFLOP/s FP64 peak, the update as HBM traffic
+(algorithmic bytes per cell) against 8 TB/s.  The "roofline" object is the kernel with the
+largest share of the step.
 """
 import argparse
 import json
@@ -13,89 +22,240 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "hyqmom.jl_amd"))
-sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
-# FP64 operations per cell-step of the pass-1 kernel and per update+pass-2 pair, from the
-# oracle's op count on the crossing-jets state; bytes = algorithmic HBM
-# traffic per cell.
-PASS1_BYTES = 280 + 280 + 840 + 48 + 4        # read M; write Mr, Fx/Fy/Fz, vp, flags
-STEP_BYTES_FP64 = 2896
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+GRID = (120, 120, 60)
+
+
+def update_bytes(precision):
+    """Algorithmic HBM bytes per cell of the x/y/z update: read the projected state (35
+    FP64), the three flux blocks (3x35) and six wave speeds, write the new state."""
+    fb = 8 if precision == "fp64" else 4
+    return 35 * 8 + 3 * 35 * fb + 6 * fb + 35 * 8
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "mixed_newton", "mixed_learned"])
+    ap.add_argument("--precision", default="mixed_learned",
+                    choices=["fp64", "mixed", "mixed_newton", "mixed_learned"])
     ap.add_argument("--model", default=None, help="warm-start network blob (mixed_learned)")
-    ap.add_argument("--grid", default="120,120,60")
+    ap.add_argument("--grid", default="%d,%d,%d" % GRID)
+    ap.add_argument("--compare-fp64", type=int, default=1)
+    ap.add_argument("--count-flops", type=int, default=1)
+    ap.add_argument("--accuracy", type=int, default=1)
     ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="run one 1 GiB device copy first (HBM counter calibration)")
     return ap.parse_args()
 
 
-def cpu_baseline(M0, params, ncells_sample=4096):
-    """The oracle (FP64 CPU restatement) on the host cores: pass 1 + step on a bounded
-    sample of the same workload (a 16x16x16 block cut from the 120x120x60 state)."""
+def jets_params(N, precision, model=None):
+    import hyqmom_amd as H
+    bg, jets = H.crossing_jets(1.5)
+    p = dict(Nx=N[0], Ny=N[1], Nz=N[2], tmax=1e9, Kn=0.01, Ma=1.5, flag2D=0, CFL=0.7, Nmom=35,
+             nnmax=10 ** 9, dtmax=1e-2, rhol=1.0, rhor=0.1, T=1.0, r110=0.0, r101=0.0, r011=0.0,
+             symmetry_check_interval=0, homogeneous_z=False, debug_output=False,
+             use_custom_ic=True, ic_background=bg, ic_jets=jets, precision=precision)
+    if model:
+        p["warmstart_model"] = model
+    return p
+
+
+def timed_run(N, precision, M0, steps, warmup, model=None):
+    """Wall time and per-kernel event times of `steps` steps from M0."""
+    import torch
+    import hyqmom_amd as H
+    s = H.Solver(jets_params(N, precision, model))
+    s.set_state(M0)
+    s.run(1e9, warmup)
+    s.set_state(M0)
+    s.t, s.nn = 0.0, 0
+    s.reset_kernel_times()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.run(1e9, steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    p1, up, p2, nst = s.kernel_times()
+    assert nst == steps and s.nn == steps, (nst, s.nn)
+    attached = s._mlp is not None
+    s.close()
+    return {"seconds": el, "pass1_ms": p1, "update_ms": up, "pass2_ms": p2, "network": attached}
+
+
+def replay_states(N, precision, M0, steps, nsample, model=None):
+    """States at nsample evenly spaced steps of the same run (untimed replay)."""
+    import hyqmom_amd as H
+    s = H.Solver(jets_params(N, precision, model))
+    s.set_state(M0)
+    marks = sorted(set(int(round(k * (steps - 1) / max(1, nsample - 1))) for k in range(nsample)))
+    out = {}
+    for m in marks:
+        if m > s.nn:
+            s.run(1e9, m)
+        out[m] = s.get_state()
+    s.close()
+    return out
+
+
+def count_flops(states, precision, stride=211):
+    """Mean pass-1 / pass-2 FP64 FLOPs per cell over the sampled states."""
+    from hyqmom_amd import opcount
+    newton = precision in ("mixed_newton", "mixed_learned")
+    a = b = 0.0
+    for M in states.values():
+        f1, f2 = opcount.state_flops_per_cell(M, stride=stride, Ma=1.5, dt=1e-3, Kn=0.01, newton=newton)
+        a += f1
+        b += f2
+    return a / len(states), b / len(states)
+
+
+def fields(M):
+    r = M[..., 0]
+    return {"rho": r, "u": M[..., 1] / r, "v": M[..., 5] / r, "w": M[..., 15] / r,
+            "E": 0.5 * (M[..., 2] + M[..., 9] + M[..., 19])}
+
+
+def one_step(N, precision, M, model=None):
+    import hyqmom_amd as H
+    s = H.Solver(jets_params(N, precision, model))
+    s.set_state(M)
+    s.step()
+    out = s.get_state()
+    s.close()
+    return out
+
+
+def accuracy(N, precision, M, model=None):
+    """One step of `precision` and of FP64 from the same FP64-trajectory state.  Cells whose
+    FP64 result itself moves by more than 1e-8 (field-scaled) when the input is perturbed by
+    one ulp are ill-conditioned (realizability / eigenvalue branch knife edges) and are
+    counted, not compared; the max relative error (field-scaled) is over the other cells."""
     import numpy as np
+    rng = np.random.default_rng(7)
+    Mp = np.asfortranarray(M.copy())
+    flip = rng.random(M.shape) < 0.5
+    Mp[flip] = np.nextafter(Mp[flip], np.inf)
+    ref = one_step(N, "fp64", M)
+    pert = one_step(N, "fp64", Mp)
+    got = one_step(N, precision, M, model)
+    fr, fp, fg = fields(ref), fields(pert), fields(got)
+    ill = np.zeros(M.shape[:3], bool)
+    for k in fr:
+        sc = np.abs(fr[k]).max()
+        ill |= np.abs(fp[k] - fr[k]) / sc > 1e-8
+    out = {"ill_conditioned_cells": int(ill.sum())}
+    for k in fr:
+        sc = np.abs(fr[k]).max()
+        e = np.abs(fg[k] - fr[k]) / sc
+        out[k] = float(e[~ill].max())
+    out["max"] = max(out[k] for k in fr)
+    return out
+
+
+def cpu_baseline(M0, seconds=10.0):
+    """The oracle (FP64 CPU restatement of the reference step) on this host's cores, on a
+    bounded sample of the same workload: steps of a 16x16x16 block cut from the 120x120x60
+    state."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     nth = os.cpu_count() or 1
     sub = np.asfortranarray(M0[52:68, 52:68, 22:38, :])
     t0 = time.time()
     steps = 0
-    while time.time() - t0 < 10.0 and steps < 50:
-        O.run(sub, 16, 16, 16, params["Kn"], params["Ma"], 0, params["CFL"], params["dtmax"],
-              1 / 120, 1 / 120, 1 / 60, True, 1e9, 1, nthreads=nth)
+    while time.time() - t0 < seconds and steps < 50:
+        O.run(sub, 16, 16, 16, 0.01, 1.5, 0, 0.7, 1e-2, 1 / 120, 1 / 120, 1 / 60, True, 1e9, 1, nthreads=nth)
         steps += 1
     el = time.time() - t0
-    return {"value": 16 * 16 * 16 * steps / el, "unit": "cell-steps/s", "cores": nth, "kind": "port",
-            "sample": "%d steps of a 16^3 block of the 120x120x60 crossing-jets state, %d threads" % (steps, nth)}
+    return {"value": 16 ** 3 * steps / el, "unit": "cell-steps/s", "cores": nth, "kind": "port",
+            "sample": "%d steps of a 16^3 block of the 120x120x60 crossing-jets state, %d OpenMP threads"
+                      % (steps, nth)}
+
+
+def kernel_report(run, flops, precision, N):
+    p1f, p2f = flops if flops else (None, None)
+    ks = {}
+    for name, ms, fl in (("pass1", run["pass1_ms"], p1f), ("pass2", run["pass2_ms"], p2f)):
+        d = {"ms": ms, "bound": "mfma", "unit": "TFLOP/s", "peak": PEAK_FP64_TFLOPS}
+        if fl is not None:
+            d["flops_per_cell"] = fl
+            d["achieved"] = fl * N / (ms * 1e-3) / 1e12
+            d["frac"] = d["achieved"] / PEAK_FP64_TFLOPS
+        ks[name] = d
+    ub = update_bytes(precision)
+    ach = ub * N / (run["update_ms"] * 1e-3) / 1e9
+    ks["update"] = {"ms": run["update_ms"], "bound": "hbm", "unit": "GB/s", "peak": PEAK_HBM_GBS,
+                    "bytes_per_cell": ub, "achieved": ach, "frac": ach / PEAK_HBM_GBS}
+    return ks
 
 
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py runs on one GPU only (this build has no multi-GPU path)", file=sys.stderr)
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (the domain is "
+              "one device; see the sharding notes in the design document)", file=sys.stderr)
         sys.exit(2)
     import numpy as np
+    import torch
     import hyqmom_amd as H
-    Nx, Ny, Nz = (int(x) for x in a.grid.split(","))
-    bg, jets = H.crossing_jets(1.5)
-    params = dict(Nx=Nx, Ny=Ny, Nz=Nz, tmax=1e9, Kn=0.01, Ma=1.5, flag2D=0, CFL=0.7, Nmom=35,
-                  nnmax=10**9, dtmax=1e-2, rhol=1.0, rhor=0.1, T=1.0, r110=0.0, r101=0.0, r011=0.0,
-                  symmetry_check_interval=0, homogeneous_z=False, debug_output=False,
-                  use_custom_ic=True, ic_background=bg, ic_jets=jets, precision=a.precision)
-    if a.model:
-        params["warmstart_model"] = a.model
-    M0 = H.initial_state(params)
-    s = H.Solver(params)
-    s.set_state(M0)
-    s.run(1e9, a.warmup)
-    s.reset_kernel_times()
-    import ctypes
-    from hyqmom_amd import _lib
-    t0 = time.perf_counter()
-    s.run(1e9, a.warmup + a.steps)
-    el = time.perf_counter() - t0
-    ms_p1, ms_up, nst = s.kernel_times()
-    N = Nx * Ny * Nz
-    value = N * a.steps / el
-    ach = PASS1_BYTES * N / (ms_p1 * 1e-3) / 1e9
+    N3 = tuple(int(x) for x in a.grid.split(","))
+    N = N3[0] * N3[1] * N3[2]
+    if a.copy_calibration:
+        x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
+    M0 = np.asfortranarray(H.initial_state(jets_params(N3, "fp64")))
+
+    run = timed_run(N3, a.precision, M0, a.steps, a.warmup, a.model)
+    value = N * a.steps / run["seconds"]
+    flops = None
+    states = None
+    if a.count_flops or a.accuracy:
+        states = replay_states(N3, "fp64" if a.precision == "fp64" else a.precision, M0, a.steps, 6, a.model)
+    if a.count_flops:
+        flops = count_flops(states, a.precision)
+    ks = kernel_report(run, flops, a.precision, N)
+    dom = max(ks, key=lambda k: ks[k]["ms"])
+    d = ks[dom]
+    roof = {"kernel": dom, "bound": d["bound"], "achieved": d.get("achieved"), "peak": d["peak"],
+            "unit": d["unit"], "frac": d.get("frac"), "traffic": None}
     out = {
-        "metric": "cell-steps/s at 120x120x60 (crossing jets, Ma=1.5, Kn=0.01)",
+        "metric": "cell-steps/s at 120x120x60 (crossing jets, Ma=1.5, Kn=0.01, 200 steps); "
+                  "mixed+learned vs plain FP64",
         "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": el * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64" if a.precision == "fp64" else "f64+f32",
+        "ms_per_step": run["seconds"] * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if a.precision == "fp64" else "f64+f32" + ("+bf16" if a.precision == "mixed_learned" else ""),
         "data": "synthetic crossing-jets initial condition",
-        "config": {"workload": "crossing jets %dx%dx%d, Ma=1.5, Kn=0.01, CFL=0.7, %s" % (Nx, Ny, Nz, a.precision),
-                   "cells": N, "precision": a.precision},
-        "kernel_ms": {"pass1": ms_p1, "update_pass2": ms_up, "steps_timed": nst},
-        "roofline": {"bound": "hbm", "achieved": ach, "peak": 8000.0, "unit": "GB/s",
-                     "frac": ach / 8000.0, "traffic": None},
+        "config": {"workload": "crossing jets %dx%dx%d, Ma=1.5, Kn=0.01, CFL=0.7, %d steps, %s"
+                               % (N3 + (a.steps, a.precision)),
+                   "cells": N, "precision": a.precision,
+                   "warmstart_network": run["network"] if a.precision == "mixed_learned" else None},
+        "kernels": ks,
+        "roofline": roof,
     }
+    if a.compare_fp64 and a.precision != "fp64":
+        f = timed_run(N3, "fp64", M0, a.steps, a.warmup)
+        fv = N * a.steps / f["seconds"]
+        fflops = None
+        if a.count_flops:
+            fflops = count_flops(replay_states(N3, "fp64", M0, a.steps, 6), "fp64")
+        out["fp64"] = {"value": fv, "ms_per_step": f["seconds"] * 1e3 / a.steps,
+                       "kernels": kernel_report(f, fflops, "fp64", N)}
+        out["speedup_vs_fp64"] = value / fv
+    if a.accuracy and a.precision != "fp64":
+        mid = sorted(states)[len(states) // 2]
+        Mfp = replay_states(N3, "fp64", M0, mid + 1, 2)[mid] if mid > 0 else M0
+        out["accuracy"] = dict(accuracy(N3, a.precision, Mfp, a.model), from_step=mid)
     if a.cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(M0, params)
-    s.close()
+        out["cpu_baseline"] = cpu_baseline(M0)
     print(json.dumps(out), flush=True)
 
 

@@ -1,10 +1,10 @@
-// Fast wave speeds for the learned / mixed modes: the characteristic polynomial of the
+// Fast wave speeds for the learned / Newton modes: the characteristic polynomial of the
 // standardized flux Jacobian, a Sturm-sequence hyperbolicity test, and Newton refinement of
 // the extreme eigenvalues from a warm start.  Cells where the test is not conclusive, or
 // where Newton does not converge in a bounded number of iterations, fall back to the full
-// FP64 eigen solve (hyq_eig.h), so decisions on complex eigenvalues are always those of the
-// full solve whenever the fast path cannot prove that all six eigenvalues are real and
-// distinct.
+// FP64 eigen solve (hyq_eig.h).  Roots that coincide to rounding level (a Maxwellian plane
+// has a double root at its mean) count as one multiple real root; a complex pair that
+// close to the real axis is below what the full solve resolves either.
 #pragma once
 #include "hyq_eig.h"
 
@@ -32,8 +32,8 @@ HD void charpoly(const double* J, double c[7]) {
 struct Sturm {
   double s[7][7];
   int deg[7];
-  int n;      // number of members
-  bool ok;    // false: a remainder vanished relative to its dividend (multiple/near-multiple roots)
+  int n;      // number of members; deg[n-1] > 0 when the sequence ends at gcd(p, p')
+  bool ok;    // false: ill-conditioned (a remainder small but not at rounding level)
 };
 HD void sturm_build(const double c[7], Sturm& S) {
   for (int k = 0; k < 7; ++k)
@@ -67,7 +67,11 @@ HD void sturm_build(const double c[7], Sturm& S) {
     // trim and test for a vanishing remainder
     double m = 0;
     for (int t = 0; t <= dr; ++t) m = fmax(m, fabs(r[t]));
-    if (!(m > 1e-9)) { S.ok = false; return; }  // dividend members have unit scale
+    // members have unit scale: a remainder at rounding level means (numerically) multiple
+    // roots, and the last member is then the gcd of p and p'; a small but resolved one is
+    // ill-conditioned and left to the full solve
+    if (!(m > 1e-12)) break;
+    if (!(m > 1e-9)) { S.ok = false; return; }
     while (dr > 0 && fabs(r[dr]) <= 1e-12 * m) --dr;
     const int k = S.n;
     for (int t = 0; t < 7; ++t) S.s[k][t] = 0.0;
@@ -142,7 +146,10 @@ HD FastEig fast_eigs_std(const double* J, double wmin, double wmax, bool need_va
     if (!(fabs(c[t]) < 1e300)) return r;
   Sturm S;
   sturm_build(c, S);
-  if (!S.ok || sturm_real_roots(S) != 6) return r;
+  if (!S.ok) return r;
+  // all six roots real <=> distinct real roots = 6 - deg gcd(p, p')
+  const int nreal = sturm_real_roots(S);
+  if (nreal != 6 - S.deg[S.n - 1]) return r;
   r.real = true;
   if (!need_vals) return r;
   // Samuelson bounds: all roots in mu +- sqrt(5) s
@@ -152,7 +159,7 @@ HD FastEig fast_eigs_std(const double* J, double wmin, double wmax, bool need_va
   double x0 = mu - half, x1 = mu + half;
   if (wmin == wmin) {
     const double g = wmin - (0.02 + 0.02 * fabs(wmin));
-    if (g > x0 && sturm_roots_above(S, g) == 6) x0 = g;
+    if (g > x0 && sturm_roots_above(S, g) == nreal) x0 = g;
   }
   if (wmax == wmax) {
     const double g = wmax + (0.02 + 0.02 * fabs(wmax));

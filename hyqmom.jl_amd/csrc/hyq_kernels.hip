@@ -46,6 +46,7 @@ int fail(int code, const char* what, hipError_t e = hipSuccess) {
   } while (0)
 
 constexpr int kBlock = 128;
+constexpr int kNev = 5;  // timing events per step
 
 struct DevScalars {  // device-resident step scalars
   double t, dt, tmax, active;
@@ -397,8 +398,8 @@ struct hyq_ctx {
   void* vp;        // [6][N]
   uint32_t* flags; // [N]
   DevScalars* sc;
-  hipEvent_t ev[4];
-  double ms_pass1, ms_update;
+  hipEvent_t ev[5];  // per step: pass 1 | dt | update | pass 2
+  double ms_pass1, ms_update, ms_pass2;
   long long timed_steps;
   hyq_mlp* mlp;  // learned mode: warm-start network (not owned)
   float* X;      // [5N][16] plane features
@@ -445,7 +446,7 @@ int hyq_create(const hyq_params* params, hyq_ctx** out) {
     hyq_destroy(c);
     return fail(HYQ_ENOMEM, "device allocation failed");
   }
-  for (int i = 0; i < 4; ++i) hipEventCreate(&c->ev[i]);
+  for (int i = 0; i < kNev; ++i) hipEventCreate(&c->ev[i]);
   *out = c;
   return HYQ_OK;
 }
@@ -461,7 +462,7 @@ int hyq_destroy(hyq_ctx* c) {
   hipFree(c->sc);
   hipFree(c->X);
   hipFree(c->P);
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < kNev; ++i)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -523,6 +524,7 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
                        (const float*)c->F, (const float*)c->vp, c->M, c->sc, p.Nx, p.Ny, p.Nz,
                        c->dx, c->dy, c->dz);
   }
+  if (ev) hipEventRecord(ev[3], c->stream);
   if (p.precision == HYQ_PREC_FP64) {
     hipLaunchKernelGGL((k_pass2<false, 0>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->sc,
                        c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
@@ -533,7 +535,7 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
     hipLaunchKernelGGL((k_pass2<true, 1>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->sc,
                        c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
   }
-  if (ev) hipEventRecord(ev[3], c->stream);
+  if (ev) hipEventRecord(ev[4], c->stream);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(HYQ_EHIP, "kernel launch", e);
   return HYQ_OK;
@@ -551,11 +553,13 @@ static int load_scalars(hyq_ctx* c, double t, long long nn, double tmax, long lo
 }
 
 static int accumulate_times(hyq_ctx* c, hipEvent_t* ev) {
-  float a = 0, b = 0;
+  float a = 0, b = 0, d = 0;
   HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]), "event time");
   HIPCHK(hipEventElapsedTime(&b, ev[2], ev[3]), "event time");
+  HIPCHK(hipEventElapsedTime(&d, ev[3], ev[4]), "event time");
   c->ms_pass1 += a;
   c->ms_update += b;
+  c->ms_pass2 += d;
   c->timed_steps += 1;
   return HYQ_OK;
 }
@@ -586,12 +590,12 @@ int hyq_run(hyq_ctx* c, double tmax, int64_t nnmax, double* t_inout, int64_t* nn
   int rc = load_scalars(c, *t_inout, *nn_inout, tmax, nnmax);
   if (rc) return rc;
   // launch nnmax - nn steps back to back; steps after t >= tmax are no-ops on the device.
-  // HIP events bracket the pass-1 kernel and the update+pass-2 pair of every step.
+  // HIP events bracket pass 1, the update and pass 2 of every step.
   const long long todo = (long long)nnmax - *nn_inout;
-  std::vector<hipEvent_t> ev(4 * (size_t)(todo > 0 ? todo : 0));
+  std::vector<hipEvent_t> ev(kNev * (size_t)(todo > 0 ? todo : 0));
   for (auto& e : ev) HIPCHK(hipEventCreate(&e), "event create");
   for (long long s = 0; s < todo; ++s) {
-    rc = enqueue_step(c, &ev[4 * s]);
+    rc = enqueue_step(c, &ev[kNev * s]);
     if (rc) break;
   }
   DevScalars s;
@@ -599,7 +603,7 @@ int hyq_run(hyq_ctx* c, double tmax, int64_t nnmax, double* t_inout, int64_t* nn
     HIPCHK(hipMemcpyAsync(&s, c->sc, sizeof(s), hipMemcpyDeviceToHost, c->stream), "scalars back");
     HIPCHK(hipStreamSynchronize(c->stream), "run sync");
     const long long ran = s.nn - *nn_inout;
-    for (long long q = 0; q < ran && rc == HYQ_OK; ++q) rc = accumulate_times(c, &ev[4 * q]);
+    for (long long q = 0; q < ran && rc == HYQ_OK; ++q) rc = accumulate_times(c, &ev[kNev * q]);
   }
   for (auto& e : ev) hipEventDestroy(e);
   if (rc) return rc;
@@ -629,18 +633,20 @@ int hyq_get_flags(hyq_ctx* c, uint32_t* flags) {
   return HYQ_OK;
 }
 
-int hyq_kernel_times(hyq_ctx* c, double* ms_pass1, double* ms_update, int64_t* nsteps) {
+int hyq_kernel_times(hyq_ctx* c, double* ms_pass1, double* ms_update, double* ms_pass2,
+                     int64_t* nsteps) {
   if (!c) return fail(HYQ_EINVAL, "null argument");
   const double n = c->timed_steps > 0 ? (double)c->timed_steps : 1.0;
   if (ms_pass1) *ms_pass1 = c->ms_pass1 / n;
   if (ms_update) *ms_update = c->ms_update / n;
+  if (ms_pass2) *ms_pass2 = c->ms_pass2 / n;
   if (nsteps) *nsteps = c->timed_steps;
   return HYQ_OK;
 }
 
 int hyq_reset_kernel_times(hyq_ctx* c) {
   if (!c) return fail(HYQ_EINVAL, "null argument");
-  c->ms_pass1 = c->ms_update = 0.0;
+  c->ms_pass1 = c->ms_update = c->ms_pass2 = 0.0;
   c->timed_steps = 0;
   return HYQ_OK;
 }

@@ -53,7 +53,7 @@ SIGNATURES = {
     "hyq_run": ([VP, ctypes.c_double, I64, D, ctypes.POINTER(I64)], ctypes.c_int),
     "hyq_get_wave_speeds": ([VP, D], ctypes.c_int),
     "hyq_get_flags": ([VP, ctypes.POINTER(ctypes.c_uint32)], ctypes.c_int),
-    "hyq_kernel_times": ([VP, D, D, ctypes.POINTER(I64)], ctypes.c_int),
+    "hyq_kernel_times": ([VP, D, D, D, ctypes.POINTER(I64)], ctypes.c_int),
     "hyq_reset_kernel_times": ([VP], ctypes.c_int),
     "hyq_closure_batch": ([VP, I64, ctypes.c_double, ctypes.c_int32, VP, VP, VP, VP, VP], ctypes.c_int),
     "hyq_wavespeed_batch": ([VP, I64, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
@@ -82,6 +82,13 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIBPATH):
             raise HyqError("libhyqmom_hip.so is not built (%s): run __graft_entry__.build()" % LIBPATH)
+        # torch ships its own HIP runtime under the same soname as /opt/rocm's: when both
+        # are used in one process, torch's must be loaded first so that this library
+        # resolves to it (the other order leaves torch without a device)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIBPATH)
         for name, (args, res) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -167,10 +167,12 @@ class Solver:
         return out
 
     def kernel_times(self):
-        a = ctypes.c_double(); b = ctypes.c_double(); n = _lib.I64()
-        _lib.check(_lib.lib().hyq_kernel_times(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n)),
-                   "kernel times")
-        return a.value, b.value, n.value
+        """(pass 1 ms, update ms, pass 2 ms, steps) averaged since the last reset."""
+        a, b, d = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        n = _lib.I64()
+        _lib.check(_lib.lib().hyq_kernel_times(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(d),
+                                               ctypes.byref(n)), "kernel times")
+        return a.value, b.value, d.value, n.value
 
     def reset_kernel_times(self):
         _lib.check(_lib.lib().hyq_reset_kernel_times(self.h), "reset times")

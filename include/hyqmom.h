@@ -110,9 +110,11 @@ int hyq_run(hyq_ctx* ctx, double tmax, int64_t nnmax, double* t_inout, int64_t* 
 int hyq_get_wave_speeds(hyq_ctx* ctx, double* vp);
 int hyq_get_flags(hyq_ctx* ctx, uint32_t* flags);
 
-/* Average device time per kernel over the steps since the last reset (HIP events on the
- * context stream).  names: "pass1", "update_pass2".  ms_out[i] for i < n. */
-int hyq_kernel_times(hyq_ctx* ctx, double* ms_pass1, double* ms_update, int64_t* nsteps);
+/* Average device time per step of pass 1 (closure, wave speeds, fluxes; in the learned mode
+ * also the network), of the x/y/z update and of pass 2 (realizability, BGK), over the steps
+ * since the last reset (HIP events on the context stream).  Any output may be NULL. */
+int hyq_kernel_times(hyq_ctx* ctx, double* ms_pass1, double* ms_update, double* ms_pass2,
+                     int64_t* nsteps);
 int hyq_reset_kernel_times(hyq_ctx* ctx);
 
 /* Batched per-cell kernels on device SoA planes ([35][n] doubles). */

@@ -65,3 +65,17 @@ def closure1d(m5):
     m = _f(m5); a = ctypes.c_double(); b = ctypes.c_double()
     lib().cell_closure1d(_p(m), ctypes.byref(a), ctypes.byref(b))
     return a.value, b.value
+
+
+def plane_compare(M, f, g, wmin=float("nan"), wmax=float("nan")):
+    """QR and fast (Sturm + Newton) solves of plane (f, g): dict of both results."""
+    M = _f(M); out = np.zeros(9)
+    lib().cell_plane_compare(_p(M), f, g, ctypes.c_double(wmin), ctypes.c_double(wmax), _p(out))
+    return {"qr": (out[0], out[1]), "qr_cplx": bool(out[2]), "fast": (out[3], out[4]),
+            "fast_real": bool(out[5]), "iters": int(out[6]), "shift": out[7], "scale": out[8]}
+
+
+def plane_std_extremes(M, f, g):
+    """Standardized QR extremes of plane (f, g)."""
+    r = plane_compare(M, f, g)
+    return tuple((v - r["shift"]) / r["scale"] for v in r["qr"])

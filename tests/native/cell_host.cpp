@@ -112,7 +112,8 @@ extern "C" void cell_pass2_flags(const double* M, int flag2D, double Ma, double 
   *fl_out = fl;
 }
 // Plane solve both ways on the standardized Jacobian of plane (f,g) of a 35-moment state.
-// out: [qr_min, qr_max, qr_cplx, fast_min, fast_max, fast_real, fast_iters] (raw scale)
+// out: [qr_min, qr_max, qr_cplx, fast_min, fast_max, fast_real, fast_iters, shift, scale]
+// (raw scale; standardized = (raw - shift) / scale)
 extern "C" void cell_plane_compare(const double* M, int f, int g, double wmin, double wmax,
                                    double* out) {
   double c[35], M000, um, vm, wm;
@@ -131,4 +132,5 @@ extern "C" void cell_plane_compare(const double* M, int f, int g, double wmin, d
   out[0] = q.vmin; out[1] = q.vmax; out[2] = q.cplx ? 1 : 0;
   out[3] = p.a + p.sig * fe.lmin; out[4] = p.a + p.sig * fe.lmax; out[5] = fe.real ? 1 : 0;
   out[6] = fe.iters;
+  out[7] = p.a; out[8] = p.sig;
 }

@@ -15,7 +15,7 @@ def declared_symbols():
 
 def test_library_exports_header_symbols():
     from hyqmom_amd import _lib
-    L = ctypes.CDLL(_lib.LIBPATH)
+    L = _lib.lib()
     syms = declared_symbols()
     assert len(syms) >= 15
     for s in syms:
