@@ -112,31 +112,8 @@ __global__ void k_dt(DevScalars* sc, double CFL, double hmin, double dtmax, doub
   sc->t += dt;
 }
 
-// ---- HLL update kernels (the line logic is in hyq_update.h) ----------------------
-__global__ __launch_bounds__(kBlock) void k_update_mixed(const double* __restrict__ Mr,
-                                                         const float* __restrict__ F,
-                                                         const float* __restrict__ vp,
-                                                         double* __restrict__ Mout,
-                                                         const DevScalars* sc, int Nx, int Ny,
-                                                         int Nz, double dx, double dy, double dz) {
-  if (sc->active == 0.0) return;
-  const long long N = (long long)Nx * Ny * Nz;
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= N) return;
-  const int i = (int)(c % Nx), j = (int)((c / Nx) % Ny), k = (int)(c / ((long long)Nx * Ny));
-  const double dt = sc->dt;
-  const long long bx = c - i, by = c - (long long)j * Nx, bz = c - (long long)k * Nx * Ny;
-  for (int n = 0; n < 35; ++n) {
-    const double* Mn = Mr + (size_t)n * N;
-    LineF lx{Mn + bx, F + (size_t)n * N + bx, vp + bx, vp + (size_t)N + bx, 1, Nx};
-    LineF ly{Mn + by, F + (size_t)(35 + n) * N + by, vp + 2 * (size_t)N + by,
-             vp + 3 * (size_t)N + by, Nx, Ny};
-    LineF lz{Mn + bz, F + (size_t)(70 + n) * N + bz, vp + 4 * (size_t)N + bz,
-             vp + 5 * (size_t)N + bz, (long long)Nx * Ny, Nz};
-    Mout[(size_t)n * N + c] = update_cell(lx, ly, lz, Mn[c], i, j, k, Nz, dt / dx, dt / dy, dt / dz);
-  }
-}
-
+// ---- HLL update (the line logic is in hyq_update.h) ----------------------------------
+// FT = double: FP64 fluxes and arithmetic; FT = float: the mixed-precision difference form.
 template <typename FT>
 __global__ __launch_bounds__(kBlock) void k_update(const double* __restrict__ Mr,
                                                    const FT* __restrict__ F,
@@ -150,16 +127,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const double* __restrict__ Mr
   if (c >= N) return;
   const int i = (int)(c % Nx), j = (int)((c / Nx) % Ny), k = (int)(c / ((long long)Nx * Ny));
   const double dt = sc->dt;
-  const long long bx = c - i, by = c - (long long)j * Nx, bz = c - (long long)k * Nx * Ny;
-  for (int n = 0; n < 35; ++n) {
-    const double* Mn = Mr + (size_t)n * N;
-    Line<FT> lx{Mn + bx, F + (size_t)n * N + bx, vp + 0 * N + bx, vp + 1 * (size_t)N + bx, 1, Nx};
-    Line<FT> ly{Mn + by, F + (size_t)(35 + n) * N + by, vp + 2 * (size_t)N + by,
-                vp + 3 * (size_t)N + by, Nx, Ny};
-    Line<FT> lz{Mn + bz, F + (size_t)(70 + n) * N + bz, vp + 4 * (size_t)N + bz,
-                vp + 5 * (size_t)N + bz, (long long)Nx * Ny, Nz};
-    Mout[(size_t)n * N + c] = update_cell(lx, ly, lz, Mn[c], i, j, k, Nz, dt / dx, dt / dy, dt / dz);
-  }
+  update_cell<FT>(Mr, F, vp, Mout, c, i, j, k, Nx, Ny, Nz, dt / dx, dt / dy, dt / dz);
 }
 
 // ---- pass 2 + BGK ----------------------------------------------------------------------
@@ -520,7 +488,7 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
                        (const double*)c->F, (const double*)c->vp, c->M, c->sc, p.Nx, p.Ny, p.Nz,
                        c->dx, c->dy, c->dz);
   } else {
-    hipLaunchKernelGGL(k_update_mixed, dim3(g), dim3(kBlock), 0, c->stream, c->Mr,
+    hipLaunchKernelGGL(k_update<float>, dim3(g), dim3(kBlock), 0, c->stream, c->Mr,
                        (const float*)c->F, (const float*)c->vp, c->M, c->sc, p.Nx, p.Ny, p.Nz,
                        c->dx, c->dy, c->dz);
   }

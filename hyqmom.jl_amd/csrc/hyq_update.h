@@ -1,107 +1,149 @@
 // HLL line update with the reference's physical-BC semantics (pas_HLL.jl:24-84,
-// flux_HLL.jl:20-32, apply_flux_update_3d.jl:30-213), on strided lines of the SoA state.
+// flux_HLL.jl:20-32, apply_flux_update_3d.jl:30-213) on the SoA state.
+//
+// Cell s of a line of n cells takes the value of cell j = clamp(s, 1, n-2) updated with the
+// fluxes of faces j-1 and j (pas_HLL.jl:72-81: Mp[1] = Mp[2], Mp[Np] = Mp[Np-1]), so one
+// cell reads cells j-1, j, j+1 of each of its three lines.  The HLL wave-speed bounds of
+// those faces do not depend on the moment: they are formed once per cell (UpdDir) and the
+// 35 moments then reuse them.  W* = (ll M_q - lr M_{q+1} - (F_q - F_{q+1})) / (ll - lr)
+// where |ll - lr| > 1e-10, else 0; face 0 has zero wave speeds (lleft[1] = lright[1] = 0)
+// and the flux of cell 1; the last face uses F_{n-2} twice (F[Np] = F[Np-1]).
 #pragma once
 #include "hyq_moments.h"
 
 namespace hyq {
+
+// HLL data of one face: bounds, their magnitudes and 1/(ll - lr) (sep: |ll - lr| > 1e-10).
+template <typename T>
+struct FaceHLL {
+  T ll, lr, all, alr, inv;
+  bool sep;
+};
+HD FaceHLL<double> face_hll(double vl0, double vl1, double vh0, double vh1) {
+  FaceHLL<double> f;
+  f.ll = jmin(vl0, vl1);  // Julia min/max: NaN propagates
+  f.lr = jmax(vh0, vh1);
+  f.all = fabs(f.ll);
+  f.alr = fabs(f.lr);
+  f.sep = fabs(f.ll - f.lr) > 1e-10;
+  f.inv = f.sep ? 1.0 / (f.ll - f.lr) : 0.0;
+  return f;
+}
+HD FaceHLL<float> face_hll(float vl0, float vl1, float vh0, float vh1) {
+  FaceHLL<float> f;
+  f.ll = fminf(vl0, vl1);
+  f.lr = fmaxf(vh0, vh1);
+  f.all = fabsf(f.ll);
+  f.alr = fabsf(f.lr);
+  f.sep = fabsf(f.ll - f.lr) > 1e-10f;
+  f.inv = f.sep ? 1.0f / (f.ll - f.lr) : 0.0f;
+  return f;
+}
+
+// One direction of one cell: offsets of cells j-1, j, j+1 (elements, relative to the cell)
+// and the HLL data of faces j-1 (A) and j (B).  mode 0: line of one cell (unchanged);
+// 1: two cells (both take cell 1); 2: general.
+template <typename T>
+struct UpdDir {
+  long long o0, o1, o2;
+  int mode;
+  bool first, last;
+  FaceHLL<T> A, B;
+};
+// vl/vh: the line's lower/upper wave-speed planes at this cell's position.
+template <typename T>
+HD UpdDir<T> upd_dir(const T* vl, const T* vh, int s, int n, long long st) {
+  UpdDir<T> d;
+  d.mode = n == 1 ? 0 : (n == 2 ? 1 : 2);
+  d.first = d.last = false;
+  if (d.mode == 0) {
+    d.o0 = d.o1 = d.o2 = 0;
+    return d;
+  }
+  if (d.mode == 1) {
+    d.o0 = d.o1 = d.o2 = (long long)(1 - s) * st;
+    return d;
+  }
+  const int j = s < 1 ? 1 : (s > n - 2 ? n - 2 : s);
+  d.o0 = (long long)(j - 1 - s) * st;
+  d.o1 = (long long)(j - s) * st;
+  d.o2 = (long long)(j + 1 - s) * st;
+  d.first = j == 1;
+  d.last = j == n - 2;
+  const T l0 = vl[d.o0], l1 = vl[d.o1], l2 = vl[d.o2];
+  const T h0 = vh[d.o0], h1 = vh[d.o1], h2 = vh[d.o2];
+  d.A = face_hll(l0, l1, h0, h1);
+  d.B = face_hll(l1, l2, h1, h2);
+  return d;
+}
+
+// Updated value of one moment along one direction (FP64 fluxes and arithmetic).
+// m, f: the moment's state and flux planes at this cell's position.
 template <typename FT>
-struct Line {
-  const double* M;
-  const FT* F;
-  const FT* lo;
-  const FT* hi;
-  long long st;  // stride between consecutive cells of the line
-  int n;
-  HD double m(int q) const { return M[(long long)q * st]; }
-  HD double f(int q) const { return (double)F[(long long)q * st]; }
-  HD double vl(int q) const { return (double)lo[(long long)q * st]; }
-  HD double vh(int q) const { return (double)hi[(long long)q * st]; }
-  // intermediate HLL state of face q (between cells q and q+1), F unmodified
-  HD double wstar(int q, double ll, double lr) const {
-    if (fabs(ll - lr) > 1e-10)
-      return (ll * m(q) - lr * m(q + 1)) / (ll - lr) - (f(q) - f(q + 1)) / (ll - lr);
-    return 0.0;
+HD double upd_value(const UpdDir<double>& d, const double* m, const FT* f, double dtdx) {
+  if (d.mode == 0) return m[0];
+  if (d.mode == 1) return m[d.o1];
+  const double m0 = m[d.o0], m1 = m[d.o1], m2 = m[d.o2];
+  const double f1 = (double)f[d.o1], f2 = (double)f[d.o2];
+  const double wsB = d.B.sep ? (d.B.ll * m1 - d.B.lr * m2) * d.B.inv - (f1 - f2) * d.B.inv : 0.0;
+  const double fB = 0.5 * (f1 + (d.last ? f1 : f2)) - 0.5 * (d.B.all * (wsB - m1) - d.B.alr * (wsB - m2));
+  double fA;
+  if (d.first) {
+    // face 0: zero wave speeds, W* of face 1, F[1] := F[2]
+    fA = 0.5 * (f1 + f1) - 0.5 * (0.0 * (wsB - m0) - 0.0 * (wsB - m1));
+  } else {
+    const double f0 = (double)f[d.o0];
+    const double wsA = d.A.sep ? (d.A.ll * m0 - d.A.lr * m1) * d.A.inv - (f0 - f1) * d.A.inv : 0.0;
+    fA = 0.5 * (f0 + f1) - 0.5 * (d.A.all * (wsA - m0) - d.A.alr * (wsA - m1));
   }
-  // numerical flux of face q, 0 <= q <= n-2, with the physical-BC copies at both ends
-  HD double flux(int q) const {
-    if (q == 0) {
-      // lleft[1] = lright[1] = 0; F[1] := F[2]; Wstar[1] := Wstar[2]
-      double ws = 0.0;
-      if (n >= 3) {
-        const double ll = jmin(vl(1), vl(2)), lr = jmax(vh(1), vh(2));
-        ws = wstar(1, ll, lr);
-      }
-      const double f1 = f(1);
-      return 0.5 * (f1 + f1) - 0.5 * (0.0 * (ws - m(0)) - 0.0 * (ws - m(1)));
-    }
-    const double ll = jmin(vl(q), vl(q + 1)), lr = jmax(vh(q), vh(q + 1));
-    const double ws = wstar(q, ll, lr);
-    const double fa = f(q), fb = (q == n - 2) ? f(q) : f(q + 1);  // F[Np] := F[Np-1]
-    return 0.5 * (fa + fb) - 0.5 * (fabs(ll) * (ws - m(q)) - fabs(lr) * (ws - m(q + 1)));
-  }
-  // Mp at cell s of the line
-  HD double updated(int s, double dtdx) const {
-    if (n == 1) return m(s);
-    if (n == 2) return m(1);
-    const int j = s < 1 ? 1 : (s > n - 2 ? n - 2 : s);
-    return m(j) - dtdx * (flux(j) - flux(j - 1));
-  }
-};
+  return m1 - dtdx * (fB - fA);
+}
 
-// Mixed precision: the HLL fluxes and increments in FP32 on FP32 fluxes and wave speeds,
-// written in difference form (W* - M_q and W* - M_{q+1} from M_q - M_{q+1}) so that FP32
-// rounding applies to differences, not to the FP64 state; the state stays FP64.
-struct LineF {
-  const double* M;
-  const float* F;
-  const float* lo;
-  const float* hi;
-  long long st;
-  int n;
-  HD double m(int q) const { return M[(long long)q * st]; }
-  HD float f(int q) const { return F[(long long)q * st]; }
-  HD float vl(int q) const { return lo[(long long)q * st]; }
-  HD float vh(int q) const { return hi[(long long)q * st]; }
-  // (W* - M_q, W* - M_{q+1}) of face q
-  HD void wdiff(int q, float ll, float lr, float& w0, float& w1) const {
-    const float dm = (float)(m(q) - m(q + 1));
-    if (fabsf(ll - lr) > 1e-10f) {
-      w0 = (lr * dm - (f(q) - f(q + 1))) / (ll - lr);
-    } else {
-      w0 = -(float)m(q);
-    }
-    w1 = w0 + dm;
+// Mixed precision: FP32 fluxes, wave speeds and HLL arithmetic in difference form (W* - M_q
+// and W* - M_{q+1} from M_q - M_{q+1}), so FP32 rounding applies to differences and to the
+// increment, never to the FP64 state.
+HD double upd_value(const UpdDir<float>& d, const double* m, const float* f, double dtdx) {
+  if (d.mode == 0) return m[0];
+  if (d.mode == 1) return m[d.o1];
+  const double m0 = m[d.o0], m1 = m[d.o1], m2 = m[d.o2];
+  const float f1 = f[d.o1], f2 = f[d.o2];
+  const float dmB = (float)(m1 - m2);
+  const float w0B = d.B.sep ? (d.B.lr * dmB - (f1 - f2)) * d.B.inv : -(float)m1;
+  const float w1B = w0B + dmB;
+  const float fB = 0.5f * (f1 + (d.last ? f1 : f2)) - 0.5f * (d.B.all * w0B - d.B.alr * w1B);
+  float fA;
+  if (d.first) {
+    fA = f1;  // zero wave speeds on face 0: the flux of cell 1
+  } else {
+    const float f0 = f[d.o0];
+    const float dmA = (float)(m0 - m1);
+    const float w0A = d.A.sep ? (d.A.lr * dmA - (f0 - f1)) * d.A.inv : -(float)m0;
+    const float w1A = w0A + dmA;
+    fA = 0.5f * (f0 + f1) - 0.5f * (d.A.all * w0A - d.A.alr * w1A);
   }
-  HD float flux(int q) const {
-    if (q == 0) {
-      const float f1 = f(1);
-      return f1;  // zero wave speeds on the first face: the flux of cell 1
-    }
-    const float ll = fminf(vl(q), vl(q + 1)), lr = fmaxf(vh(q), vh(q + 1));
-    float w0, w1;
-    wdiff(q, ll, lr, w0, w1);
-    const float fa = f(q), fb = (q == n - 2) ? f(q) : f(q + 1);
-    return 0.5f * (fa + fb) - 0.5f * (fabsf(ll) * w0 - fabsf(lr) * w1);
-  }
-  // (updated value at cell s) - M[s]: the increment plus the boundary copy offset
-  HD double updated(int s, double dtdx) const {
-    if (n == 1) return m(s);
-    if (n == 2) return m(1);
-    const int j = s < 1 ? 1 : (s > n - 2 ? n - 2 : s);
-    const float inc = -(float)dtdx * (flux(j) - flux(j - 1));
-    return m(j) + (double)inc;
-  }
-};
+  const float inc = -(float)dtdx * (fB - fA);
+  return m1 + (double)inc;
+}
 
-
-// Combined x/y/z update of moment plane Mn at cell c (i,j,k): Mx + My + Mz - 2M.
-template <typename L>
-HD double update_cell(const L& lx, const L& ly, const L& lz, double m0, int i, int j, int k,
-                      int Nz, double dtdx, double dtdy, double dtdz) {
-  const double mx = lx.updated(i, dtdx);
-  const double my = ly.updated(j, dtdy);
-  const double mz = (Nz == 1) ? m0 : lz.updated(k, dtdz);  // apply_flux_update_3d.jl:183
-  return mx + my + mz - 2.0 * m0;
+// Combined x/y/z update of cell c (i, j, k) for all 35 moments: Mx + My + Mz - 2M
+// (simulation_runner.jl:516-520); the z sweep is skipped when Nz == 1
+// (apply_flux_update_3d.jl:183).  Mr: state planes [35][N]; F: flux planes [105][N]
+// (x, y, z blocks); vp: wave-speed planes [6][N]; out: [35][N].
+template <typename FT>
+HD void update_cell(const double* Mr, const FT* F, const FT* vp, double* out, long long c, int i,
+                    int j, int k, int Nx, int Ny, int Nz, double dtdx, double dtdy, double dtdz) {
+  const long long N = (long long)Nx * Ny * Nz, sy = Nx, sz = (long long)Nx * Ny;
+  const UpdDir<FT> dx = upd_dir(vp + c, vp + N + c, i, Nx, 1);
+  const UpdDir<FT> dy = upd_dir(vp + 2 * N + c, vp + 3 * N + c, j, Ny, sy);
+  const UpdDir<FT> dz = upd_dir(vp + 4 * N + c, vp + 5 * N + c, k, Nz, sz);
+  for (int n = 0; n < 35; ++n) {
+    const double* m = Mr + (size_t)n * N + c;
+    const double m0 = m[0];
+    const double mx = upd_value(dx, m, F + (size_t)n * N + c, dtdx);
+    const double my = upd_value(dy, m, F + (size_t)(35 + n) * N + c, dtdy);
+    const double mz = (Nz == 1) ? m0 : upd_value(dz, m, F + (size_t)(70 + n) * N + c, dtdz);
+    out[(size_t)n * N + c] = mx + my + mz - 2.0 * m0;
+  }
 }
 
 }  // namespace hyq

@@ -79,3 +79,17 @@ def plane_std_extremes(M, f, g):
     """Standardized QR extremes of plane (f, g)."""
     r = plane_compare(M, f, g)
     return tuple((v - r["shift"]) / r["scale"] for v in r["qr"])
+
+
+def host_step(M, Kn, Ma, CFL=0.7, dtmax=1e-2, flag2D=0, t=0.0, tmax=1e9):
+    """One device step emulated on the host: M Fortran (Nx,Ny,Nz,35) -> (M', dt)."""
+    Nx, Ny, Nz = M.shape[:3]
+    N = Nx * Ny * Nz
+    soa = np.array(np.asarray(M, np.float64).ravel(order="F").reshape(35, N))  # a copy
+    L = lib()
+    L.host_step.restype = ctypes.c_double
+    dt = L.host_step(_p(soa), Nx, Ny, Nz, ctypes.c_double(Kn), ctypes.c_double(Ma), flag2D,
+                     ctypes.c_double(CFL), ctypes.c_double(dtmax), ctypes.c_double(1.0 / Nx),
+                     ctypes.c_double(1.0 / Ny), ctypes.c_double(1.0 / Nz), ctypes.c_double(tmax),
+                     ctypes.c_double(t), None)
+    return soa.reshape(-1, order="C").reshape((35, N)).T.reshape((Nx, Ny, Nz, 35), order="F"), dt

@@ -86,15 +86,8 @@ extern "C" double host_step(double* M, int Nx, int Ny, int Nz, double Kn, double
   dt = jmin(dt, tmax - t);
   for (long long c = 0; c < N; ++c) {
     const int i = (int)(c % Nx), j = (int)((c / Nx) % Ny), k = (int)(c / ((long long)Nx * Ny));
-    const long long bx = c - i, by = c - (long long)j * Nx, bz = c - (long long)k * Nx * Ny;
-    for (int n = 0; n < 35; ++n) {
-      const double* Mn = &Mr[n * N];
-      Line<double> lx{Mn + bx, &F[n * N] + bx, &vp[0] + bx, &vp[N] + bx, 1, Nx};
-      Line<double> ly{Mn + by, &F[(35 + n) * N] + by, &vp[2 * N] + by, &vp[3 * N] + by, Nx, Ny};
-      Line<double> lz{Mn + bz, &F[(70 + n) * N] + bz, &vp[4 * N] + bz, &vp[5 * N] + bz,
-                      (long long)Nx * Ny, Nz};
-      M[n * N + c] = update_cell(lx, ly, lz, Mn[c], i, j, k, Nz, dt / dx, dt / dy, dt / dz);
-    }
+    update_cell<double>(Mr.data(), F.data(), vp.data(), M, c, i, j, k, Nx, Ny, Nz, dt / dx, dt / dy,
+                        dt / dz);
   }
   for (long long c = 0; c < N; ++c) {
     double m[35], o[35];

@@ -63,3 +63,36 @@ def test_maxwellian_eig_closed_form():
     vmin, vmax, _, _ = C.eigenvalues6(to35(d), 1)
     assert abs(vmin - (0.3 - np.sqrt(3 * 1.3))) < 1e-6
     assert abs(vmax - (0.3 + np.sqrt(3 * 1.3))) < 1e-6
+
+
+def _field(shape, seed, mixtures=True):
+    """A field of Gaussian cells, optionally with a few non-Gaussian mixture cells."""
+    rng = np.random.default_rng(seed)
+    Nx, Ny, Nz = shape
+    M = np.empty(shape + (35,), order="F")
+    for i in range(Nx):
+        for j in range(Ny):
+            for k in range(Nz):
+                u = (0.3 * np.sin(i), 0.2 * np.cos(j), 0.1 * k)
+                M[i, j, k] = to35(gaussian_raw(0.5 + 0.1 * ((i + j + k) % 3), u, np.eye(3) * (1 + 0.1 * i), 4))
+    for _ in range(max(1, Nx * Ny * Nz // 10) if mixtures else 0):
+        i, j, k = rng.integers(0, Nx), rng.integers(0, Ny), rng.integers(0, Nz)
+        M[i, j, k] = to35(mixture_raw(rng, spread=1.0, lam_lo=0.1))
+    return M
+
+
+@pytest.mark.parametrize("shape", [(6, 5, 4), (7, 3, 1), (2, 5, 3), (1, 4, 3)])
+def test_host_step_matches_oracle(shape):
+    """The device step's update (per-cell HLL face data, BC copies, Nz == 1 and 1- and
+    2-cell lines) and passes, emulated on the host, against the oracle's step.  Gaussian
+    cells only: no cell sits on a realizability or eigenvalue knife edge, so every cell
+    must agree."""
+    M = _field(shape, 3, mixtures=False)
+    Nx, Ny, Nz = shape
+    got, dt = C.host_step(M, 0.05, 1.0)
+    ref, t, nn, dts, rc = O.run(M, Nx, Ny, Nz, 0.05, 1.0, 0, 0.7, 1e-2, 1.0 / Nx, 1.0 / Ny, 1.0 / Nz,
+                                True, 1e9, 1)
+    assert rc == 0 and abs(dt - t) <= 1e-15 * t
+    scale = np.abs(ref).max(axis=(0, 1, 2))
+    err = (np.abs(got - ref) / np.maximum(scale, 1e-300)).max()
+    assert err < 1e-12, err
