@@ -127,7 +127,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const double* __restrict__ Mr
   if (c >= N) return;
   const int i = (int)(c % Nx), j = (int)((c / Nx) % Ny), k = (int)(c / ((long long)Nx * Ny));
   const double dt = sc->dt;
-  update_cell<FT>(Mr, F, vp, Mout, c, i, j, k, Nx, Ny, Nz, dt / dx, dt / dy, dt / dz);
+  update_cell<FT>(Mr, F, vp, Mout, c, i, j, k, Nx, Ny, Nz, N, dt / dx, dt / dy, dt / dz);
 }
 
 // ---- pass 2 + BGK ----------------------------------------------------------------------

@@ -128,11 +128,13 @@ HD double upd_value(const UpdDir<float>& d, const double* m, const float* f, dou
 // Combined x/y/z update of cell c (i, j, k) for all 35 moments: Mx + My + Mz - 2M
 // (simulation_runner.jl:516-520); the z sweep is skipped when Nz == 1
 // (apply_flux_update_3d.jl:183).  Mr: state planes [35][N]; F: flux planes [105][N]
-// (x, y, z blocks); vp: wave-speed planes [6][N]; out: [35][N].
+// (x, y, z blocks); vp: wave-speed planes [6][N]; out: [35][N]; N is the plane stride
+// (a grid's cells may be a contiguous part of longer planes), c the cell's plane index.
 template <typename FT>
 HD void update_cell(const double* Mr, const FT* F, const FT* vp, double* out, long long c, int i,
-                    int j, int k, int Nx, int Ny, int Nz, double dtdx, double dtdy, double dtdz) {
-  const long long N = (long long)Nx * Ny * Nz, sy = Nx, sz = (long long)Nx * Ny;
+                    int j, int k, int Nx, int Ny, int Nz, long long N, double dtdx, double dtdy,
+                    double dtdz) {
+  const long long sy = Nx, sz = (long long)Nx * Ny;
   const UpdDir<FT> dx = upd_dir(vp + c, vp + N + c, i, Nx, 1);
   const UpdDir<FT> dy = upd_dir(vp + 2 * N + c, vp + 3 * N + c, j, Ny, sy);
   const UpdDir<FT> dz = upd_dir(vp + 4 * N + c, vp + 5 * N + c, k, Nz, sz);

@@ -72,6 +72,16 @@ SIGNATURES = {
     "hyq_mlp_last_error": ([], ctypes.c_char_p),
     "hyq_set_mlp": ([VP, VP], ctypes.c_int),
     "hyq_plane_dataset": ([VP, I64, VP, VP, VP], ctypes.c_int),
+    "hyq_ensemble_create": ([ctypes.POINTER(HyqParams), ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)],
+                            ctypes.c_int),
+    "hyq_ensemble_destroy": ([VP], ctypes.c_int),
+    "hyq_ensemble_set_state": ([VP, ctypes.c_int32, D], ctypes.c_int),
+    "hyq_ensemble_get_state": ([VP, ctypes.c_int32, D], ctypes.c_int),
+    "hyq_ensemble_status": ([VP, D, D, ctypes.POINTER(I64)], ctypes.c_int),
+    "hyq_ensemble_run": ([VP, I64, ctypes.c_int32], ctypes.c_int),
+    "hyq_ensemble_set_blocks": ([VP, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+    "hyq_ensemble_tune": ([VP, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), D], ctypes.c_int),
+    "hyq_ensemble_last_error": ([], ctypes.c_char_p),
     "hyq_last_error": ([], ctypes.c_char_p),
     "hyq_version": ([], ctypes.c_char_p),
 }

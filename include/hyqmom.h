@@ -170,6 +170,29 @@ int hyq_set_mlp(hyq_ctx* ctx, hyq_mlp* mlp);
  * extreme eigenvalues from the full FP64 solve) with valid [5n] (1: real and finite). */
 int hyq_plane_dataset(const double* M, int64_t n, float* X, float* L, uint8_t* valid);
 
+/* ---- ensemble: independent runs advanced together --------------------------------------
+ * Runs with their own grid, Ma, Kn, CFL and tmax (one precision for all: FP64, MIXED or
+ * MIXED_NEWTON) share SoA planes and one HIP-graph-captured step; each run keeps its own
+ * CFL dt and loop condition t < tmax.  No counterpart in the reference, which runs one
+ * simulation per process (simulation_runner.jl:63). */
+typedef struct hyq_ensemble hyq_ensemble;
+int hyq_ensemble_create(const hyq_params* runs, int32_t nruns, hyq_ensemble** out);
+int hyq_ensemble_destroy(hyq_ensemble* e);
+/* state of one run, host Julia layout (Nx,Ny,Nz,35); set_state resets its t and nn */
+int hyq_ensemble_set_state(hyq_ensemble* e, int32_t run, const double* M);
+int hyq_ensemble_get_state(hyq_ensemble* e, int32_t run, double* M);
+/* per-run t, last dt, steps taken (arrays of nruns; any may be NULL) */
+int hyq_ensemble_status(hyq_ensemble* e, double* t, double* dt, int64_t* nn);
+/* nsteps steps (graph replays if use_graph, else direct launches); synchronizes */
+int hyq_ensemble_run(hyq_ensemble* e, int64_t nsteps, int32_t use_graph);
+/* block sizes of pass 1, update, pass 2 (each 64, 128 or 256) */
+int hyq_ensemble_set_blocks(hyq_ensemble* e, const int32_t* blocks);
+/* autotuner: times each block-size candidate over trial_steps steps from the current
+ * state (restored afterwards) and keeps the fastest per kernel; blocks_out [3],
+ * ms_out [3 candidates][3 kernels] (either may be NULL) */
+int hyq_ensemble_tune(hyq_ensemble* e, int32_t trial_steps, int32_t* blocks_out, double* ms_out);
+const char* hyq_ensemble_last_error(void);
+
 /* Text of the last error on the calling thread ("" if none). */
 const char* hyq_last_error(void);
 

@@ -86,8 +86,8 @@ extern "C" double host_step(double* M, int Nx, int Ny, int Nz, double Kn, double
   dt = jmin(dt, tmax - t);
   for (long long c = 0; c < N; ++c) {
     const int i = (int)(c % Nx), j = (int)((c / Nx) % Ny), k = (int)(c / ((long long)Nx * Ny));
-    update_cell<double>(Mr.data(), F.data(), vp.data(), M, c, i, j, k, Nx, Ny, Nz, dt / dx, dt / dy,
-                        dt / dz);
+    update_cell<double>(Mr.data(), F.data(), vp.data(), M, c, i, j, k, Nx, Ny, Nz, N, dt / dx,
+                        dt / dy, dt / dz);
   }
   for (long long c = 0; c < N; ++c) {
     double m[35], o[35];
