@@ -47,11 +47,14 @@ int fail(int code, const char* what, hipError_t e = hipSuccess) {
 
 constexpr int kBlock = 128;
 constexpr int kNev = 5;  // timing events per step
+constexpr const char* kNonfiniteMsg =
+    "non-finite Jacobian in the z wave-speed solve (eigenvalues6z_hyperbolic_3D.jl:108,121 throws)";
 
 struct DevScalars {  // device-resident step scalars
   double t, dt, tmax, active;
   unsigned long long vmax_bits;
-  long long nn, nnmax, pad;
+  long long nn, nnmax;
+  unsigned long long err;  // bit 0: a non-finite z Jacobian (where the reference throws)
 };
 
 // ---- pass 1 -------------------------------------------------------------------------
@@ -89,6 +92,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1(const double* __restrict__ M,
       vb = b > vb ? b : vb;  // |vp| >= 0: bit order = value order; NaN sorts above +Inf
     }
     if (flags) flags[c] = fl;
+    if (sc != nullptr && (fl & F_NONFIN)) atomicOr(&sc->err, 1ull);
   }
   if (sc != nullptr) {
     vb = wave_max_u64(vb);
@@ -344,6 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1b(double* __restrict__ AMr, flo
       vb = b > vb ? b : vb;
     }
     flags[c] = fl;
+    if (fl & F_NONFIN) atomicOr(&sc->err, 1ull);
   }
   vb = wave_max_u64(vb);
   if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
@@ -550,6 +555,7 @@ int hyq_step(hyq_ctx* c, double* t_inout, double* dt_out, int64_t* nn_inout) {
   *t_inout = s.t;
   if (dt_out) *dt_out = s.active != 0.0 ? s.dt : 0.0;
   *nn_inout = s.nn;
+  if (s.err) return fail(HYQ_ENONFINITE, kNonfiniteMsg);
   return HYQ_OK;
 }
 
@@ -577,6 +583,7 @@ int hyq_run(hyq_ctx* c, double tmax, int64_t nnmax, double* t_inout, int64_t* nn
   if (rc) return rc;
   *t_inout = s.t;
   *nn_inout = s.nn;
+  if (s.err) return fail(HYQ_ENONFINITE, kNonfiniteMsg);
   return HYQ_OK;
 }
 

@@ -142,16 +142,19 @@ class Solver:
 
     def step(self):
         t = ctypes.c_double(self.t); dt = ctypes.c_double(); nn = _lib.I64(self.nn)
-        _lib.check(_lib.lib().hyq_step(self.h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(nn)),
-                   "hyq_step")
-        self.t, self.nn = t.value, nn.value
+        rc = _lib.lib().hyq_step(self.h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(nn))
+        if rc in (0, -4):  # HYQ_ENONFINITE still completes the device step
+            self.t, self.nn = t.value, nn.value
+        _lib.check(rc, "hyq_step")
         return dt.value
 
     def run(self, tmax, nnmax):
         t = ctypes.c_double(self.t); nn = _lib.I64(self.nn)
-        _lib.check(_lib.lib().hyq_run(self.h, ctypes.c_double(tmax), _lib.I64(int(nnmax)), ctypes.byref(t),
-                                      ctypes.byref(nn)), "hyq_run")
-        self.t, self.nn = t.value, nn.value
+        rc = _lib.lib().hyq_run(self.h, ctypes.c_double(tmax), _lib.I64(int(nnmax)), ctypes.byref(t),
+                                ctypes.byref(nn))
+        if rc in (0, -4):  # HYQ_ENONFINITE still completes the device steps
+            self.t, self.nn = t.value, nn.value
+        _lib.check(rc, "hyq_run")
 
     def wave_speeds(self):
         n = self.p.Nx * self.p.Ny * self.p.Nz

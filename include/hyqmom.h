@@ -98,7 +98,8 @@ int hyq_get_state(hyq_ctx* ctx, double* M);
 /* One step of the reference loop body (simulation_runner.jl:346-560): pass 1 (closure,
  * wave speeds), dt = min(CFL*min(dx,dy,dz)/vmax, min(dtmax,Kn), tmax - t), x/y/z HLL
  * updates combined as Mx+My+Mz-2M, pass 2 realizability and BGK.  t and nn advance as
- * in the reference. */
+ * in the reference.  Returns HYQ_ENONFINITE (with t, nn and the state of the completed
+ * device step, which then carries NaNs) where the reference's z eigen solve throws. */
 int hyq_step(hyq_ctx* ctx, double* t_inout, double* dt_out, int64_t* nn_inout);
 
 /* while t < tmax && nn < nnmax: hyq_step.  The loop condition of

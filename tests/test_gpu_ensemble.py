@@ -20,11 +20,15 @@ def _runs(precision):
 
 def _single(p, steps):
     import hyqmom_amd as H
+    from hyqmom_amd._lib import HyqError
     s = H.Solver(p)
     s.set_state(H.initial_state(p))
     for _ in range(steps):
         if s.t < float(p["tmax"]):
-            s.step()
+            try:
+                s.step()
+            except HyqError as e:  # HYQ_ENONFINITE: the reference throws; t is NaN
+                assert "HYQ_ENONFINITE" in str(e)
     out = (s.get_state(), s.t, s.nn)
     s.close()
     return out
@@ -41,8 +45,10 @@ def test_ensemble_matches_single_runs(precision):
     t, dt, nn = e.status()
     for r, p in enumerate(runs):
         M, ts, ns = _single(p, steps)
-        assert nn[r] == ns and t[r] == ts, (r, nn[r], ns, t[r], ts)
-        assert np.array_equal(e.get_state(r), M), r
+        # the thin (10, 2, 5) run goes non-finite on its second step, as the oracle's does
+        # (its z Jacobian, where the reference throws): NaNs must match too
+        assert nn[r] == ns and (t[r] == ts or (np.isnan(t[r]) and np.isnan(ts))), (r, nn[r], ns, t[r], ts)
+        assert np.array_equal(e.get_state(r), M, equal_nan=True), r
     e.close()
 
 
@@ -65,7 +71,7 @@ def test_graph_blocks_and_tuner_do_not_change_results():
         assert list(n0) == [1] * len(runs)
         e.run(2, graph=True)
         for r in range(len(runs)):
-            assert np.array_equal(e.get_state(r), want[r]), (blocks, r)
+            assert np.array_equal(e.get_state(r), want[r], equal_nan=True), (blocks, r)
         e.close()
 
 
