@@ -114,7 +114,9 @@ class Ensemble:
 
 def compare_sequential(runs, steps=10, tune=True, log=print):
     """Aggregate cell-steps/s of the graph-captured ensemble against the same runs stepped
-    one after another, each through its own single-run context."""
+    one after another, each through its own single-run context.  Only steps a run actually
+    takes count (a run that has reached tmax, or gone non-finite where the reference
+    throws, stops in both)."""
     import torch
     e = Ensemble(runs)
     e.set_initial_states()
@@ -123,29 +125,46 @@ def compare_sequential(runs, steps=10, tune=True, log=print):
         blocks, ms = e.tune(2)
         log("autotuned blocks (pass1, update, pass2): %s; trial ms per candidate 64/128/256:\n%s" % (blocks, ms))
     e.run(1)  # graph build and first replay
+    _, _, nn0 = e.status()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e.run(steps)
     el_e = time.perf_counter() - t0
-    cells = e.cells
+    _, _, nn1 = e.status()
+    sizes = np.array([s[0] * s[1] * s[2] for s in e.shapes], dtype=np.int64)
+    work_e = int((sizes * (nn1 - nn0)).sum())
     e.close()
     solvers = []
     for p in runs:
         s = Solver(p)
         s.set_state(initial_state(p))
-        s.step()
-        solvers.append(s)
+        live = True
+        try:
+            s.step()
+        except _lib.HyqError:
+            live = False
+        solvers.append([s, live, int(p["Nx"]) * int(p["Ny"]) * int(p["Nz"])])
     torch.cuda.synchronize()
+    work_s = 0
     t0 = time.perf_counter()
     for _ in range(steps):
-        for s in solvers:
-            s.step()
+        for ent in solvers:
+            s, live, n = ent
+            if not live or not (s.t < float(s.p.tmax)):
+                continue
+            try:
+                s.step()
+                work_s += n
+            except _lib.HyqError:
+                ent[1] = False
+                work_s += n
     el_s = time.perf_counter() - t0
-    for s in solvers:
-        s.close()
-    out = {"runs": len(runs), "cells": cells, "steps": steps, "blocks": blocks,
-           "ensemble_cell_steps_per_s": cells * steps / el_e,
-           "sequential_cell_steps_per_s": cells * steps / el_s,
-           "speedup": el_s / el_e}
+    for ent in solvers:
+        ent[0].close()
+    out = {"runs": len(runs), "cells": int(sizes.sum()), "steps": steps, "blocks": blocks,
+           "ensemble_cell_steps": work_e, "sequential_cell_steps": work_s,
+           "ensemble_cell_steps_per_s": work_e / el_e,
+           "sequential_cell_steps_per_s": work_s / el_s,
+           "speedup": (work_e / el_e) / (work_s / el_s)}
     log(str(out))
     return out
