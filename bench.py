@@ -83,8 +83,12 @@ def timed_run(N, precision, M0, steps, warmup, model=None):
     p1, up, p2, nst = s.kernel_times()
     assert nst == steps and s.nn == steps, (nst, s.nn)
     attached = s._mlp is not None
+    fl = s.flags()  # pass-1 flags of the last step
     s.close()
-    return {"seconds": el, "pass1_ms": p1, "update_ms": up, "pass2_ms": p2, "network": attached}
+    from hyqmom_amd._lib import newton_iterations
+    return {"seconds": el, "pass1_ms": p1, "update_ms": up, "pass2_ms": p2, "network": attached,
+            "last_step": {"fallback_cells_frac": float(((fl >> 12) & 1).mean()),
+                          "iterations_per_cell": float(newton_iterations(fl).mean())}}
 
 
 def replay_states(N, precision, M0, steps, nsample, model=None):
@@ -157,24 +161,25 @@ def accuracy(N, precision, M, model=None):
     return out
 
 
-def cpu_baseline(M0, seconds=10.0):
+def cpu_baseline(M, step, seconds=10.0):
     """The oracle (FP64 CPU restatement of the reference step) on this host's cores, on a
-    bounded sample of the same workload: steps of a 16x16x16 block cut from the 120x120x60
-    state."""
+    bounded sample of the same workload: steps of the 16x16x16 block around a jet's centre
+    (cells 22..37 in x, y and z), cut from the FP64 state at the given step."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     nth = os.cpu_count() or 1
-    sub = np.asfortranarray(M0[52:68, 52:68, 22:38, :])
+    sub = np.asfortranarray(M[22:38, 22:38, 22:38, :])
     t0 = time.time()
     steps = 0
     while time.time() - t0 < seconds and steps < 50:
-        O.run(sub, 16, 16, 16, 0.01, 1.5, 0, 0.7, 1e-2, 1 / 120, 1 / 120, 1 / 60, True, 1e9, 1, nthreads=nth)
+        sub, _, _, _, _ = O.run(sub, 16, 16, 16, 0.01, 1.5, 0, 0.7, 1e-2, 1 / 120, 1 / 120, 1 / 60, True,
+                                1e9, 1, nthreads=nth)
         steps += 1
     el = time.time() - t0
     return {"value": 16 ** 3 * steps / el, "unit": "cell-steps/s", "cores": nth, "kind": "port",
-            "sample": "%d steps of a 16^3 block of the 120x120x60 crossing-jets state, %d OpenMP threads"
-                      % (steps, nth)}
+            "sample": "%d consecutive steps of the 16^3 block around a jet centre of the 120x120x60 "
+                      "crossing-jets FP64 state at step %d, %d OpenMP threads" % (steps, step, nth)}
 
 
 def kernel_report(run, flops, precision, N):
@@ -238,24 +243,28 @@ def main():
                                % (N3 + (a.steps, a.precision)),
                    "cells": N, "precision": a.precision,
                    "warmstart_network": run["network"] if a.precision == "mixed_learned" else None},
+        "fast_wave_speeds_last_step": run["last_step"] if a.precision in ("mixed_newton", "mixed_learned") else None,
         "kernels": ks,
         "roofline": roof,
     }
+    # the FP64 trajectory, sampled: FLOPs of the FP64 kernels, the accuracy check's and the
+    # CPU baseline's starting state (mid-run, when the jets have crossed)
+    fstates = states if a.precision == "fp64" else None
+    if fstates is None and ((a.compare_fp64 and a.count_flops) or a.accuracy or a.cpu_baseline):
+        fstates = replay_states(N3, "fp64", M0, a.steps, 6)
+    mid = sorted(fstates)[len(fstates) // 2] if fstates else 0
+    Mmid = fstates[mid] if fstates else M0
     if a.compare_fp64 and a.precision != "fp64":
         f = timed_run(N3, "fp64", M0, a.steps, a.warmup)
         fv = N * a.steps / f["seconds"]
-        fflops = None
-        if a.count_flops:
-            fflops = count_flops(replay_states(N3, "fp64", M0, a.steps, 6), "fp64")
+        fflops = count_flops(fstates, "fp64") if a.count_flops else None
         out["fp64"] = {"value": fv, "ms_per_step": f["seconds"] * 1e3 / a.steps,
                        "kernels": kernel_report(f, fflops, "fp64", N)}
         out["speedup_vs_fp64"] = value / fv
     if a.accuracy and a.precision != "fp64":
-        mid = sorted(states)[len(states) // 2]
-        Mfp = replay_states(N3, "fp64", M0, mid + 1, 2)[mid] if mid > 0 else M0
-        out["accuracy"] = dict(accuracy(N3, a.precision, Mfp, a.model), from_step=mid)
+        out["accuracy"] = dict(accuracy(N3, a.precision, Mmid, a.model), from_step=mid)
     if a.cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(M0)
+        out["cpu_baseline"] = cpu_baseline(Mmid, mid)
     print(json.dumps(out), flush=True)
 
 

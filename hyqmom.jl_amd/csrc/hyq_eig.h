@@ -160,75 +160,130 @@ HD void house(double x, double y, double z, bool three, double& v1, double& v2, 
   v1 = y / v0;
   v2 = three ? z / v0 : 0.0;
 }
-// Francis double-shift QR on an upper Hessenberg matrix; returns false if it stalled.
-template <int N>
-HD bool hqr_n(double* H, double* wr, double* wi) {
+// Francis double-shift QR on an upper Hessenberg matrix, on any 2D accessor H(i, j); each
+// deflated eigenvalue (or conjugate pair) goes to sink(index, re, im).  Returns false if it
+// stalled.
+template <int N, typename Mat, typename Sink>
+HD bool hqr_core(Mat& H, Sink& sink) {
   int hi = N - 1, its = 0;
   while (hi >= 0) {
     int lo = hi;
     while (lo > 0) {
-      double s = fabs(H[(lo - 1) * N + lo - 1]) + fabs(H[lo * N + lo]);
+      double s = fabs(H(lo - 1, lo - 1)) + fabs(H(lo, lo));
       if (s == 0.0) s = 1.0;
-      if (fabs(H[lo * N + lo - 1]) <= EPS * s) { H[lo * N + lo - 1] = 0.0; break; }
+      if (fabs(H(lo, lo - 1)) <= EPS * s) { H(lo, lo - 1) = 0.0; break; }
       --lo;
     }
-    if (lo == hi) { wr[hi] = H[hi * N + hi]; wi[hi] = 0.0; --hi; its = 0; continue; }
+    if (lo == hi) { sink(hi, H(hi, hi), 0.0); --hi; its = 0; continue; }
     if (lo == hi - 1) {
-      eig2x2(H[(hi - 1) * N + hi - 1], H[(hi - 1) * N + hi], H[hi * N + hi - 1], H[hi * N + hi],
-             wr[hi - 1], wi[hi - 1], wr[hi], wi[hi]);
+      double r1, i1, r2, i2;
+      eig2x2(H(hi - 1, hi - 1), H(hi - 1, hi), H(hi, hi - 1), H(hi, hi), r1, i1, r2, i2);
+      sink(hi - 1, r1, i1);
+      sink(hi, r2, i2);
       hi -= 2; its = 0; continue;
     }
     if (its >= 100) return false;
     ++its;
     double s, t;
     if (its % 10 == 0) {
-      const double w = fabs(H[hi * N + hi - 1]) + fabs(H[(hi - 1) * N + hi - 2]);
+      const double w = fabs(H(hi, hi - 1)) + fabs(H(hi - 1, hi - 2));
       s = 1.5 * w;
       t = w * w;
     } else {
-      s = H[(hi - 1) * N + hi - 1] + H[hi * N + hi];
-      t = H[(hi - 1) * N + hi - 1] * H[hi * N + hi] - H[(hi - 1) * N + hi] * H[hi * N + hi - 1];
+      s = H(hi - 1, hi - 1) + H(hi, hi);
+      t = H(hi - 1, hi - 1) * H(hi, hi) - H(hi - 1, hi) * H(hi, hi - 1);
     }
-    double x = H[lo * N + lo] * H[lo * N + lo] + H[lo * N + lo + 1] * H[(lo + 1) * N + lo] -
-               s * H[lo * N + lo] + t;
-    double y = H[(lo + 1) * N + lo] * (H[lo * N + lo] + H[(lo + 1) * N + lo + 1] - s);
-    double z = H[(lo + 1) * N + lo] * H[(lo + 2) * N + lo + 1];
+    double x = H(lo, lo) * H(lo, lo) + H(lo, lo + 1) * H(lo + 1, lo) - s * H(lo, lo) + t;
+    double y = H(lo + 1, lo) * (H(lo, lo) + H(lo + 1, lo + 1) - s);
+    double z = H(lo + 1, lo) * H(lo + 2, lo + 1);
     for (int k = lo - 1; k <= hi - 3; ++k) {
       double v1, v2, beta;
       house(x, y, z, true, v1, v2, beta);
       const int q = lo > k ? lo : k;
       for (int j = q; j <= hi; ++j) {
-        const double d = beta * (H[(k + 1) * N + j] + v1 * H[(k + 2) * N + j] + v2 * H[(k + 3) * N + j]);
-        H[(k + 1) * N + j] -= d;
-        H[(k + 2) * N + j] -= d * v1;
-        H[(k + 3) * N + j] -= d * v2;
+        const double d = beta * (H(k + 1, j) + v1 * H(k + 2, j) + v2 * H(k + 3, j));
+        H(k + 1, j) -= d;
+        H(k + 2, j) -= d * v1;
+        H(k + 3, j) -= d * v2;
       }
       const int r = k + 4 < hi ? k + 4 : hi;
       for (int i = lo; i <= r; ++i) {
-        const double d = beta * (H[i * N + k + 1] + v1 * H[i * N + k + 2] + v2 * H[i * N + k + 3]);
-        H[i * N + k + 1] -= d;
-        H[i * N + k + 2] -= d * v1;
-        H[i * N + k + 3] -= d * v2;
+        const double d = beta * (H(i, k + 1) + v1 * H(i, k + 2) + v2 * H(i, k + 3));
+        H(i, k + 1) -= d;
+        H(i, k + 2) -= d * v1;
+        H(i, k + 3) -= d * v2;
       }
-      x = H[(k + 2) * N + k + 1];
-      y = H[(k + 3) * N + k + 1];
-      if (k < hi - 3) z = H[(k + 4) * N + k + 1];
+      x = H(k + 2, k + 1);
+      y = H(k + 3, k + 1);
+      if (k < hi - 3) z = H(k + 4, k + 1);
     }
     double v1, v2, beta;
     house(x, y, 0.0, false, v1, v2, beta);
     for (int j = hi - 2; j <= hi; ++j) {
-      const double d = beta * (H[(hi - 1) * N + j] + v1 * H[hi * N + j]);
-      H[(hi - 1) * N + j] -= d;
-      H[hi * N + j] -= d * v1;
+      const double d = beta * (H(hi - 1, j) + v1 * H(hi, j));
+      H(hi - 1, j) -= d;
+      H(hi, j) -= d * v1;
     }
     for (int i = lo; i <= hi; ++i) {
-      const double d = beta * (H[i * N + hi - 1] + v1 * H[i * N + hi]);
-      H[i * N + hi - 1] -= d;
-      H[i * N + hi] -= d * v1;
+      const double d = beta * (H(i, hi - 1) + v1 * H(i, hi));
+      H(i, hi - 1) -= d;
+      H(i, hi) -= d * v1;
     }
   }
   return true;
 }
+
+// Row-major N x N array accessor.
+template <int N>
+struct ArrMat {
+  double* a;
+  HD double& operator()(int i, int j) { return a[i * N + j]; }
+};
+// Sinks: all eigenvalues into arrays, or only the extreme real parts and max |imag|.
+struct ArraySink {
+  double *wr, *wi;
+  HD void operator()(int k, double re, double im) { wr[k] = re; wi[k] = im; }
+};
+struct ExtremeSink {
+  double mn, mx, mi;
+  bool first;
+  HD void operator()(int, double re, double im) {
+    mn = first ? re : fmin(mn, re);
+    mx = first ? re : fmax(mx, re);
+    mi = first ? fabs(im) : fmax(mi, fabs(im));
+    first = false;
+  }
+};
+
+template <int N>
+HD bool hqr_n(double* H, double* wr, double* wi) {
+  ArrMat<N> m{H};
+  ArraySink s{wr, wi};
+  return hqr_core<N>(m, s);
+}
+
+// Storage of the 6x6 matrix during the QR iteration, whose data-dependent indices would
+// otherwise put it in per-lane scratch memory: on the device a slot of LDS per lane
+// (element-major, lane-minor: conflict-free), provided by the launch as dynamic shared memory
+// of qr_lds_bytes(block) bytes; on the host a local array.
+HD constexpr unsigned qr_lds_bytes(unsigned block) { return 36u * 8u * block; }
+#if defined(__HIP_DEVICE_COMPILE__)
+struct QrMat {
+  double* b;
+  int s;
+  __device__ double& operator()(int i, int j) { return b[(i * 6 + j) * s]; }
+};
+__device__ __forceinline__ QrMat qr_mat() {
+  extern __shared__ double hyq_qr_lds[];
+  return QrMat{hyq_qr_lds + threadIdx.x, (int)blockDim.x};
+}
+#else
+struct QrMat {
+  double a[36];
+  double& operator()(int i, int j) { return a[i * 6 + j]; }
+};
+inline QrMat qr_mat() { return QrMat{}; }
+#endif
 
 struct EigRes {
   double vmin, vmax;  // extreme real parts (raw scale), NaN if not finite
@@ -251,24 +306,22 @@ HD EigRes plane_eigs(const Plane& p, uint32_t& fl) {
   }
   balance_n<6>(H);
   hessenberg_n<6>(H);
-  double wr[6], wi[6];
-  if (!hqr_n<6>(H, wr, wi)) {
+  QrMat Q = qr_mat();
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) Q(i, j) = H[i * 6 + j];
+  ExtremeSink ex{0.0, 0.0, 0.0, true};
+  if (!hqr_core<6>(Q, ex)) {
     fl |= F_NOCONV;
     r.vmin = r.vmax = NAN;
     r.cplx = false;
     r.ok = false;
     return r;
   }
-  double mn = wr[0], mx = wr[0], mi = fabs(wi[0]);
-#pragma unroll
-  for (int t = 1; t < 6; ++t) {
-    mn = fmin(mn, wr[t]);
-    mx = fmax(mx, wr[t]);
-    mi = fmax(mi, fabs(wi[t]));
-  }
-  r.vmin = p.a + p.sig * mn;
-  r.vmax = p.a + p.sig * mx;
-  r.cplx = p.sig * mi > 1000 * EPS;
+  r.vmin = p.a + p.sig * ex.mn;
+  r.vmax = p.a + p.sig * ex.mx;
+  r.cplx = p.sig * ex.mi > 1000 * EPS;
   r.ok = true;
   return r;
 }

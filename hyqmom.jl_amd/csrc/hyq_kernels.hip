@@ -47,6 +47,8 @@ int fail(int code, const char* what, hipError_t e = hipSuccess) {
 
 constexpr int kBlock = 128;
 constexpr int kNev = 5;  // timing events per step
+// dynamic LDS of the kernels that run the QR eigen solve (hyq_eig.h: qr_mat)
+constexpr unsigned kQrLds = qr_lds_bytes(kBlock);
 constexpr const char* kNonfiniteMsg =
     "non-finite Jacobian in the z wave-speed solve (eigenvalues6z_hyperbolic_3D.jl:108,121 throws)";
 
@@ -465,15 +467,15 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
   hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, c->stream, c->sc);
   if (ev) hipEventRecord(ev[0], c->stream);
   if (p.precision == HYQ_PREC_FP64) {
-    hipLaunchKernelGGL((k_pass1<double, 0>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
+    hipLaunchKernelGGL((k_pass1<double, 0>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->Mr,
                        (double*)c->F, (double*)c->vp, c->flags, c->sc, c->N, p.flag2D, p.Ma,
                        p.positivity);
   } else if (p.precision == HYQ_PREC_MIXED) {
-    hipLaunchKernelGGL((k_pass1<float, 0>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
+    hipLaunchKernelGGL((k_pass1<float, 0>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->Mr,
                        (float*)c->F, (float*)c->vp, c->flags, c->sc, c->N, p.flag2D, p.Ma,
                        p.positivity);
   } else if (p.precision == HYQ_PREC_MIXED_NEWTON || c->mlp == nullptr) {
-    hipLaunchKernelGGL((k_pass1<float, 1>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
+    hipLaunchKernelGGL((k_pass1<float, 1>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->Mr,
                        (float*)c->F, (float*)c->vp, c->flags, c->sc, c->N, p.flag2D, p.Ma,
                        p.positivity);
   } else {
@@ -482,7 +484,7 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
                        c->N, p.flag2D, p.Ma);
     const int rc = hyq_mlp_infer_async(c->mlp, c->X, 5 * c->N, c->P, c->stream);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_pass1b, dim3(g), dim3(kBlock), 0, c->stream, c->Mr, (float*)c->F,
+    hipLaunchKernelGGL(k_pass1b, dim3(g), dim3(kBlock), kQrLds, c->stream, c->Mr, (float*)c->F,
                        (float*)c->vp, c->flags, c->P, c->sc, c->N, p.flag2D, p.Ma, p.positivity);
   }
   if (ev) hipEventRecord(ev[1], c->stream);
@@ -499,13 +501,13 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
   }
   if (ev) hipEventRecord(ev[3], c->stream);
   if (p.precision == HYQ_PREC_FP64) {
-    hipLaunchKernelGGL((k_pass2<false, 0>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->sc,
+    hipLaunchKernelGGL((k_pass2<false, 0>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->sc,
                        c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
   } else if (p.precision == HYQ_PREC_MIXED) {
-    hipLaunchKernelGGL((k_pass2<true, 0>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->sc,
+    hipLaunchKernelGGL((k_pass2<true, 0>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->sc,
                        c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
   } else {
-    hipLaunchKernelGGL((k_pass2<true, 1>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->sc,
+    hipLaunchKernelGGL((k_pass2<true, 1>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->sc,
                        c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
   }
   if (ev) hipEventRecord(ev[4], c->stream);
@@ -646,7 +648,8 @@ int hyq_wavespeed_batch(const double* M, int64_t n, double Ma, int32_t flag2D,
   // the three flux outputs must be one [105][n] block for the pass-1 kernel
   if (Fy != Fx + (size_t)35 * n || Fz != Fx + (size_t)70 * n)
     return fail(HYQ_EINVAL, "Fx, Fy, Fz must be consecutive [35][n] blocks");
-  hipLaunchKernelGGL((k_pass1<double, 0>), dim3(grid_for(n)), dim3(kBlock), 0, 0, M, Mr, Fx, vp, flags,
+  hipLaunchKernelGGL((k_pass1<double, 0>), dim3(grid_for(n)), dim3(kBlock), kQrLds, 0, M, Mr, Fx, vp,
+                     flags,
                      (DevScalars*)nullptr, (long long)n, flag2D, Ma, positivity);
   HIPCHK(hipGetLastError(), "pass1 launch");
   HIPCHK(hipDeviceSynchronize(), "pass1 sync");
@@ -674,7 +677,7 @@ int hyq_plane_dataset(const double* M, int64_t n, float* X, float* L, uint8_t* v
   if (!M || !X || n < 0 || ((L == nullptr) != (valid == nullptr)))
     return fail(HYQ_EINVAL, "bad argument");
   if (n == 0) return HYQ_OK;
-  hipLaunchKernelGGL(k_plane_dataset, dim3(grid_for(n)), dim3(kBlock), 0, 0, M, (long long)n, X, L,
+  hipLaunchKernelGGL(k_plane_dataset, dim3(grid_for(n)), dim3(kBlock), kQrLds, 0, M, (long long)n, X, L,
                      valid);
   HIPCHK(hipGetLastError(), "dataset launch");
   HIPCHK(hipDeviceSynchronize(), "dataset sync");
