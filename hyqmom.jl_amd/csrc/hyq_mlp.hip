@@ -187,6 +187,129 @@ __global__ void k_denorm(const float* __restrict__ Y, const float* __restrict__ 
   P[2 * b + 1] = Y[b * 16 + 1] * ystat[3] + ystat[1];
 }
 
+// ---- fused inference ------------------------------------------------------------------
+// All four layers in one kernel: the bf16 weights, pre-arranged in MFMA B-operand order, sit
+// in LDS for the whole block; each wave takes 16-row tiles, keeps a tile's activations in a
+// per-wave LDS buffer between layers (bias, GELU and bf16 rounding applied on the way), and
+// writes only the two de-normalized outputs.  HBM traffic per sample: 64 B in, 8 B out.
+//
+// Packed weight layout: for k-step ks (16 rows of W) and n-tile t (16 columns), chunk
+// (ks * NT + t) holds, for lane l, the 4 bf16 W[16 ks + 4 (l / 16) + e][16 t + l % 16],
+// e = 0..3 -- one 8-byte LDS read per lane per MFMA.
+constexpr int PK_W1 = 0;                      // 1 k-step x 8 tiles
+constexpr int PK_W2 = PK_W1 + 1 * 8 * 256;    // 8 x 8 chunks of 256 bf16
+constexpr int PK_W3 = PK_W2 + 8 * 8 * 256;
+constexpr int PK_W4 = PK_W3 + 8 * 8 * 256;    // 8 k-steps x 1 tile
+constexpr int PK_N = PK_W4 + 8 * 1 * 256;     // bf16 elements
+constexpr int ACT_LD = HID + 4;               // bf16 row pitch of the activation tile
+
+__global__ void k_pack(const float* __restrict__ p, short* __restrict__ pk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= PK_N) return;
+  int base, K, NT;
+  const float* W;
+  if (i < PK_W2) { base = PK_W1; K = IN; NT = 8; W = p + OFF_W1; }
+  else if (i < PK_W3) { base = PK_W2; K = HID; NT = 8; W = p + OFF_W2; }
+  else if (i < PK_W4) { base = PK_W3; K = HID; NT = 8; W = p + OFF_W3; }
+  else { base = PK_W4; K = HID; NT = 1; W = p + OFF_W4; }
+  const int o = i - base, chunk = o / 256, r = o % 256, lane = r / 4, e = r % 4;
+  const int ks = chunk / NT, t = chunk % NT;
+  const int row = 16 * ks + 4 * (lane / 16) + e, col = 16 * t + lane % 16;
+  const int ncol = NT * 16;
+  (void)K;
+  pk[i] = f2bf(W[row * ncol + col]);
+}
+
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+
+__global__ __launch_bounds__(256) void k_mlp_fused(const float* __restrict__ X, long n,
+                                                   const float* __restrict__ p,
+                                                   const short* __restrict__ pk,
+                                                   float* __restrict__ P) {
+  __shared__ s4 wts[PK_N / 4];
+  __shared__ float bias[3 * HID + 2];
+  __shared__ __align__(16) short act[4][16 * ACT_LD];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const s4* pk4 = reinterpret_cast<const s4*>(pk);
+  for (int i = tid; i < PK_N / 4; i += 256) wts[i] = pk4[i];
+  for (int i = tid; i < HID; i += 256) {
+    bias[i] = p[OFF_B1 + i];
+    bias[HID + i] = p[OFF_B2 + i];
+    bias[2 * HID + i] = p[OFF_B3 + i];
+  }
+  if (tid < 2) bias[3 * HID + tid] = p[OFF_B4 + tid];
+  __syncthreads();
+  const int r = lane & 15, q = lane >> 4;
+  const long ntiles = (n + 15) / 16;
+  short* A = act[w];
+  // block-uniform loop: every wave runs every iteration (rows past n are zeros, not stored)
+  for (long base = (long)blockIdx.x * 4; base < ntiles; base += (long)gridDim.x * 4) {
+    const long row0 = (base + w) * 16;
+    // layer 1: normalized input (k = 4q..4q+3 of 16) times W1
+    s4 a;
+    {
+      const long row = row0 + r;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int k = 4 * q + e;
+        float v = 0.f;
+        if (row < n && k < 15) v = (X[row * 16 + k] - p[OFF_XMU + k]) / p[OFF_XSD + k];
+        a[e] = f2bf(v);
+      }
+    }
+    f4 acc[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wts[(PK_W1 / 4) + t * 64 + lane],
+                                                         f4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        A[(4 * q + e) * ACT_LD + 16 * t + r] = f2bf(gelu(acc[t][e] + bias[16 * t + r]));
+    __syncthreads();
+    // layers 2 and 3
+#pragma unroll
+    for (int L = 0; L < 2; ++L) {
+      const int wb = (L == 0 ? PK_W2 : PK_W3) / 4;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < 8; ++ks) {
+        const s4 av = *reinterpret_cast<const s4*>(&A[r * ACT_LD + 16 * ks + 4 * q]);
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, wts[wb + (ks * 8 + t) * 64 + lane],
+                                                             acc[t], 0, 0, 0);
+      }
+      __syncthreads();
+      const float* bl = bias + HID * (L + 1);
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          A[(4 * q + e) * ACT_LD + 16 * t + r] = f2bf(gelu(acc[t][e] + bl[16 * t + r]));
+      __syncthreads();
+    }
+    // layer 4: one 16-column tile, outputs 0 and 1 kept
+    f4 o = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+      const s4 av = *reinterpret_cast<const s4*>(&A[r * ACT_LD + 16 * ks + 4 * q]);
+      o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, wts[(PK_W4 / 4) + ks * 64 + lane], o, 0, 0, 0);
+    }
+    if (r < 2) {
+      const float mu = p[OFF_YMU + r], sd = p[OFF_YSD + r], b = bias[3 * HID + r];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const long row = row0 + 4 * q + e;
+        if (row < n) P[row * 2 + r] = (o[e] + b) * sd + mu;
+      }
+    }
+    __syncthreads();  // the next tile overwrites A
+  }
+}
+
 unsigned g1(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
 template <int NT>
@@ -211,6 +334,8 @@ struct hyq_mlp {
   float* loss;
   long step;
   hipStream_t stream;
+  short* packed;  // bf16 weights in MFMA operand order (fused inference)
+  long version, packed_version;  // blob changes / version the packed copy was made from
 };
 
 static int ensure_icap(hyq_mlp* m, long n) {
@@ -305,6 +430,9 @@ int hyq_mlp_create(const float* blob_host, hyq_mlp** out) {
   }
   MCHK(hipMemset(m->m1, 0, sizeof(float) * NPARAM), "adam m");
   MCHK(hipMemset(m->m2, 0, sizeof(float) * NPARAM), "adam v");
+  MCHK(hipMalloc(&m->packed, sizeof(short) * PK_N), "mlp packed weights");
+  m->version = 1;
+  m->packed_version = 0;
   *out = m;
   return HYQ_OK;
 }
@@ -315,6 +443,7 @@ int hyq_mlp_destroy(hyq_mlp* m) {
   float* bufs[] = {m->blob, m->grad, m->m1, m->m2, m->loss, m->Xn, m->Z1, m->H1, m->Z2,
                    m->H2, m->Z3, m->H3, m->Y, m->dY, m->dA, m->dB, m->iX, m->iA, m->iB, m->iY};
   for (float* b : bufs) (void)hipFree(b);
+  (void)hipFree(m->packed);
   (void)hipStreamDestroy(m->stream);
   delete m;
   return HYQ_OK;
@@ -331,19 +460,42 @@ int hyq_mlp_set_blob(hyq_mlp* m, const float* blob_host) {
   MCHK(hipStreamSynchronize(m->stream), "mlp sync");
   MCHK(hipMemcpy(m->blob, blob_host, sizeof(float) * NBLOB, hipMemcpyHostToDevice), "mlp blob");
   m->step = 0;
+  m->version += 1;
   MCHK(hipMemset(m->m1, 0, sizeof(float) * NPARAM), "adam m");
   MCHK(hipMemset(m->m2, 0, sizeof(float) * NPARAM), "adam v");
   return HYQ_OK;
 }
 
 // Predicted (lmin, lmax) for n feature rows X [n][16] (device) into P [n][2] (device),
-// enqueued on `stream` without synchronizing.
+// enqueued on `stream` without synchronizing: the fused kernel (the packed weights are
+// refreshed first when the parameters changed).
 int hyq_mlp_infer_async(hyq_mlp* m, const float* X, int64_t n, float* P, hipStream_t stream) {
+  if (!m || !X || !P || n < 0) return mfail(HYQ_EINVAL, "bad argument");
+  if (n == 0) return HYQ_OK;
+  if (m->packed_version != m->version) {
+    // the parameters may have been written on the network's own stream
+    MCHK(hipStreamSynchronize(m->stream), "mlp sync");
+    hipLaunchKernelGGL(k_pack, dim3(g1(PK_N)), dim3(256), 0, stream, m->blob, m->packed);
+    m->packed_version = m->version;
+  }
+  const long tiles = ((long)n + 15) / 16;
+  long blocks = (tiles + 3) / 4;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(k_mlp_fused, dim3((unsigned)blocks), dim3(256), 0, stream, X, (long)n, m->blob,
+                     m->packed, P);
+  MCHK(hipGetLastError(), "mlp infer launch");
+  return HYQ_OK;
+}
+
+// The same prediction through the layer-by-layer GEMM kernels of training (a cross-check
+// of the fused kernel).
+int hyq_mlp_infer_layers(hyq_mlp* m, const float* X, int64_t n, float* P) {
   if (!m || !X || !P || n < 0) return mfail(HYQ_EINVAL, "bad argument");
   if (n == 0) return HYQ_OK;
   int rc = ensure_icap(m, n);
   if (rc) return rc;
   const float* p = m->blob;
+  hipStream_t stream = m->stream;
   hipLaunchKernelGGL(k_normalize, dim3(g1(n * 16)), dim3(256), 0, stream, X, p + OFF_XMU, m->iX, (long)n);
   gemm<8>(stream, m->iX, IN, 1, p + OFF_W1, HID, 1, n, HID, IN, p + OFF_B1, nullptr, m->iA, HID, HID, 1);
   gemm<8>(stream, m->iA, HID, 1, p + OFF_W2, HID, 1, n, HID, HID, p + OFF_B2, nullptr, m->iB, HID, HID, 1);
@@ -351,6 +503,7 @@ int hyq_mlp_infer_async(hyq_mlp* m, const float* X, int64_t n, float* P, hipStre
   gemm<1>(stream, m->iA, HID, 1, p + OFF_W4, OUT, 1, n, OUT, HID, p + OFF_B4, nullptr, m->iY, OUT, OUT, 0);
   hipLaunchKernelGGL(k_denorm, dim3(g1(n)), dim3(256), 0, stream, m->iY, p + OFF_YMU, P, (long)n);
   MCHK(hipGetLastError(), "mlp infer launch");
+  MCHK(hipStreamSynchronize(stream), "mlp infer sync");
   return HYQ_OK;
 }
 int hyq_mlp_infer(hyq_mlp* m, const float* X, int64_t n, float* P) {
@@ -389,6 +542,7 @@ int hyq_mlp_train_step(hyq_mlp* m, const float* X, const float* T, int64_t n, fl
   int rc = hyq_mlp_grad(m, X, T, n, loss_out, nullptr);
   if (rc) return rc;
   m->step += 1;
+  m->version += 1;
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = 1.f - powf(b1, (float)m->step), c2 = 1.f - powf(b2, (float)m->step);
   hipLaunchKernelGGL(k_adam, dim3(g1(NPARAM)), dim3(256), 0, m->stream, m->blob, m->grad, m->m1,

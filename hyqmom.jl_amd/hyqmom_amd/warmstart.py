@@ -85,6 +85,16 @@ class WarmStartMLP:
         self._check(_lib.lib().hyq_mlp_infer(self.handle, _ptr(X), X.shape[0], _ptr(P)), "hyq_mlp_infer")
         return P
 
+    def infer_layers(self, X):
+        """The layer-by-layer path (training kernels) for the same prediction."""
+        torch = _torch()
+        X = X.contiguous()
+        _sync()
+        P = torch.empty((X.shape[0], 2), dtype=torch.float32, device=X.device)
+        self._check(_lib.lib().hyq_mlp_infer_layers(self.handle, _ptr(X), X.shape[0], _ptr(P)),
+                    "hyq_mlp_infer_layers")
+        return P
+
     def grad(self, X, T):
         loss = ctypes.c_float()
         g = np.empty(self.nparam, np.float32)

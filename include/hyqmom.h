@@ -154,8 +154,10 @@ int hyq_mlp_create(const float* blob_host /* NULL: zeros */, hyq_mlp** out);
 int hyq_mlp_destroy(hyq_mlp* m);
 int hyq_mlp_get_blob(hyq_mlp* m, float* blob_host);
 int hyq_mlp_set_blob(hyq_mlp* m, const float* blob_host);
-/* X [n][16] features (device), P [n][2] predictions (device). */
+/* X [n][16] features (device), P [n][2] predictions (device): the fused inference kernel. */
 int hyq_mlp_infer(hyq_mlp* m, const float* X, int64_t n, float* P);
+/* the same through the layer-by-layer training kernels (bitwise equal; a cross-check) */
+int hyq_mlp_infer_layers(hyq_mlp* m, const float* X, int64_t n, float* P);
 /* loss and gradients (grad_host: hyq_mlp_param_count() floats, may be NULL); T [n][2]. */
 int hyq_mlp_grad(hyq_mlp* m, const float* X, const float* T, int64_t n, float* loss_out,
                  float* grad_host);

@@ -35,6 +35,27 @@ def test_forward_matches_reference(n):
     assert np.abs(P - ref64).max() / ysd.max() > 0 or n == 1
 
 
+@pytest.mark.parametrize("n", [1, 63, 4099])
+def test_fused_inference_equals_layers(n):
+    """The fused kernel (weights packed in LDS, activations per wave) rounds and accumulates
+    exactly like the layer-by-layer kernels: bitwise equal predictions."""
+    import torch
+    from hyqmom_amd.warmstart import WarmStartMLP
+    blob = R.random_blob(13)
+    X = np.random.default_rng(14).standard_normal((n, 16)).astype(np.float32)
+    m = WarmStartMLP(blob)
+    Xd = _dev(X)
+    a = m.infer(Xd).cpu().numpy()
+    b = m.infer_layers(Xd).cpu().numpy()
+    assert np.array_equal(a, b), np.abs(a - b).max()
+    # a parameter change reaches the packed copy
+    blob2 = R.random_blob(15)
+    m.set_blob(blob2)
+    c = m.infer(Xd).cpu().numpy()
+    assert np.array_equal(c, m.infer_layers(Xd).cpu().numpy())
+    assert not np.array_equal(a, c)
+
+
 def test_gradient_matches_reference():
     from hyqmom_amd.warmstart import WarmStartMLP
     blob = R.random_blob(21)
