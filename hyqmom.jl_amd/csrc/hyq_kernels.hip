@@ -46,6 +46,12 @@ int fail(int code, const char* what, hipError_t e = hipSuccess) {
   } while (0)
 
 constexpr int kBlock = 128;
+// minimum waves per SIMD the register allocator must allow for the pass-1/pass-2 kernels
+// (1: no spills to scratch, low occupancy; 2: some spills, twice the waves to hide latency)
+#ifndef HYQ_PASS_WAVES
+#define HYQ_PASS_WAVES 1
+#endif
+constexpr int kPassWaves = HYQ_PASS_WAVES;
 constexpr int kNev = 5;  // timing events per step
 // dynamic LDS of the kernels that run the QR eigen solve (hyq_eig.h: qr_mat)
 constexpr unsigned kQrLds = qr_lds_bytes(kBlock);
@@ -70,7 +76,7 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
 }
 
 template <typename FT, int EM>
-__global__ __launch_bounds__(kBlock) void k_pass1(const double* __restrict__ M,
+__global__ __launch_bounds__(kBlock, kPassWaves) void k_pass1(const double* __restrict__ M,
                                                   double* __restrict__ Mr, FT* __restrict__ F,
                                                   FT* __restrict__ vp, uint32_t* __restrict__ flags,
                                                   DevScalars* sc, long long N, int flag2D,
@@ -138,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const double* __restrict__ Mr
 
 // ---- pass 2 + BGK ----------------------------------------------------------------------
 template <bool MIXED, int EM>
-__global__ __launch_bounds__(kBlock) void k_pass2(double* __restrict__ M, const DevScalars* sc,
+__global__ __launch_bounds__(kBlock, kPassWaves) void k_pass2(double* __restrict__ M, const DevScalars* sc,
                                                   long long N, int flag2D, double Ma,
                                                   int positivity, double Kn) {
   if (sc->active == 0.0) return;
@@ -297,7 +303,7 @@ __global__ __launch_bounds__(kBlock) void k_plane_dataset(const double* __restri
   }
 }
 // Pass 1, first half: projection (state written to A) and the plane features.
-__global__ __launch_bounds__(kBlock) void k_pass1a(const double* __restrict__ M, double* __restrict__ A,
+__global__ __launch_bounds__(kBlock, kPassWaves) void k_pass1a(const double* __restrict__ M, double* __restrict__ A,
                                                    float* __restrict__ X, DevScalars* sc, long long N,
                                                    int flag2D, double Ma) {
   if (sc->active == 0.0) return;
@@ -321,7 +327,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1a(const double* __restrict__ M,
   }
 }
 // Pass 1, second half, with the network's predictions P [5N][2] as Newton warm starts.
-__global__ __launch_bounds__(kBlock) void k_pass1b(double* __restrict__ AMr, float* __restrict__ F,
+__global__ __launch_bounds__(kBlock, kPassWaves) void k_pass1b(double* __restrict__ AMr, float* __restrict__ F,
                                                    float* __restrict__ vp, uint32_t* __restrict__ flags,
                                                    const float* __restrict__ P, DevScalars* sc,
                                                    long long N, int flag2D, double Ma, int positivity) {

@@ -222,17 +222,20 @@ __global__ void k_pack(const float* __restrict__ p, short* __restrict__ pk) {
 
 __device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
 
-__global__ __launch_bounds__(256) void k_mlp_fused(const float* __restrict__ X, long n,
-                                                   const float* __restrict__ p,
-                                                   const short* __restrict__ pk,
-                                                   float* __restrict__ P) {
+// WAVES waves per block share the weights; each wave takes RT 16-row tiles at a time, so
+// every B operand read from LDS feeds RT MFMAs.
+template <int WAVES, int RT>
+__global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restrict__ X, long n,
+                                                          const float* __restrict__ p,
+                                                          const short* __restrict__ pk,
+                                                          float* __restrict__ P) {
   __shared__ s4 wts[PK_N / 4];
   __shared__ float bias[3 * HID + 2];
-  __shared__ __align__(16) short act[4][16 * ACT_LD];
+  __shared__ __align__(16) short act[WAVES][RT * 16 * ACT_LD];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const s4* pk4 = reinterpret_cast<const s4*>(pk);
-  for (int i = tid; i < PK_N / 4; i += 256) wts[i] = pk4[i];
-  for (int i = tid; i < HID; i += 256) {
+  for (int i = tid; i < PK_N / 4; i += 64 * WAVES) wts[i] = pk4[i];
+  for (int i = tid; i < HID; i += 64 * WAVES) {
     bias[i] = p[OFF_B1 + i];
     bias[HID + i] = p[OFF_B2 + i];
     bias[2 * HID + i] = p[OFF_B3 + i];
@@ -240,15 +243,17 @@ __global__ __launch_bounds__(256) void k_mlp_fused(const float* __restrict__ X, 
   if (tid < 2) bias[3 * HID + tid] = p[OFF_B4 + tid];
   __syncthreads();
   const int r = lane & 15, q = lane >> 4;
-  const long ntiles = (n + 15) / 16;
+  const long ngroups = (n + 16 * RT - 1) / (16 * RT);  // RT-tile groups
   short* A = act[w];
   // block-uniform loop: every wave runs every iteration (rows past n are zeros, not stored)
-  for (long base = (long)blockIdx.x * 4; base < ntiles; base += (long)gridDim.x * 4) {
-    const long row0 = (base + w) * 16;
+  for (long base = (long)blockIdx.x * WAVES; base < ngroups; base += (long)gridDim.x * WAVES) {
+    const long row0 = (base + w) * 16 * RT;
+    f4 acc[RT][8];
     // layer 1: normalized input (k = 4q..4q+3 of 16) times W1
-    s4 a;
-    {
-      const long row = row0 + r;
+#pragma unroll
+    for (int u = 0; u < RT; ++u) {
+      const long row = row0 + 16 * u + r;
+      s4 a;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int k = 4 * q + e;
@@ -256,58 +261,88 @@ __global__ __launch_bounds__(256) void k_mlp_fused(const float* __restrict__ X, 
         if (row < n && k < 15) v = (X[row * 16 + k] - p[OFF_XMU + k]) / p[OFF_XSD + k];
         a[e] = f2bf(v);
       }
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wts[(PK_W1 / 4) + t * 64 + lane],
+                                                              f4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
     }
-    f4 acc[8];
 #pragma unroll
-    for (int t = 0; t < 8; ++t)
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wts[(PK_W1 / 4) + t * 64 + lane],
-                                                         f4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+    for (int u = 0; u < RT; ++u)
 #pragma unroll
-    for (int t = 0; t < 8; ++t)
+      for (int t = 0; t < 8; ++t)
 #pragma unroll
-      for (int e = 0; e < 4; ++e)
-        A[(4 * q + e) * ACT_LD + 16 * t + r] = f2bf(gelu(acc[t][e] + bias[16 * t + r]));
+        for (int e = 0; e < 4; ++e)
+          A[(16 * u + 4 * q + e) * ACT_LD + 16 * t + r] = f2bf(gelu(acc[u][t][e] + bias[16 * t + r]));
     __syncthreads();
     // layers 2 and 3
 #pragma unroll
     for (int L = 0; L < 2; ++L) {
       const int wb = (L == 0 ? PK_W2 : PK_W3) / 4;
 #pragma unroll
-      for (int t = 0; t < 8; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+      for (int u = 0; u < RT; ++u)
+#pragma unroll
+        for (int t = 0; t < 8; ++t) acc[u][t] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int ks = 0; ks < 8; ++ks) {
-        const s4 av = *reinterpret_cast<const s4*>(&A[r * ACT_LD + 16 * ks + 4 * q]);
+        s4 av[RT];
 #pragma unroll
-        for (int t = 0; t < 8; ++t)
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, wts[wb + (ks * 8 + t) * 64 + lane],
-                                                             acc[t], 0, 0, 0);
+        for (int u = 0; u < RT; ++u)
+          av[u] = *reinterpret_cast<const s4*>(&A[(16 * u + r) * ACT_LD + 16 * ks + 4 * q]);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const s4 bv = wts[wb + (ks * 8 + t) * 64 + lane];
+#pragma unroll
+          for (int u = 0; u < RT; ++u)
+            acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[u], bv, acc[u][t], 0, 0, 0);
+        }
       }
       __syncthreads();
       const float* bl = bias + HID * (L + 1);
 #pragma unroll
-      for (int t = 0; t < 8; ++t)
+      for (int u = 0; u < RT; ++u)
 #pragma unroll
-        for (int e = 0; e < 4; ++e)
-          A[(4 * q + e) * ACT_LD + 16 * t + r] = f2bf(gelu(acc[t][e] + bl[16 * t + r]));
+        for (int t = 0; t < 8; ++t)
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            A[(16 * u + 4 * q + e) * ACT_LD + 16 * t + r] = f2bf(gelu(acc[u][t][e] + bl[16 * t + r]));
       __syncthreads();
     }
     // layer 4: one 16-column tile, outputs 0 and 1 kept
-    f4 o = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int ks = 0; ks < 8; ++ks) {
-      const s4 av = *reinterpret_cast<const s4*>(&A[r * ACT_LD + 16 * ks + 4 * q]);
-      o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, wts[(PK_W4 / 4) + ks * 64 + lane], o, 0, 0, 0);
-    }
-    if (r < 2) {
-      const float mu = p[OFF_YMU + r], sd = p[OFF_YSD + r], b = bias[3 * HID + r];
+    for (int u = 0; u < RT; ++u) {
+      f4 o = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const long row = row0 + 4 * q + e;
-        if (row < n) P[row * 2 + r] = (o[e] + b) * sd + mu;
+      for (int ks = 0; ks < 8; ++ks) {
+        const s4 av = *reinterpret_cast<const s4*>(&A[(16 * u + r) * ACT_LD + 16 * ks + 4 * q]);
+        o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, wts[(PK_W4 / 4) + ks * 64 + lane], o, 0, 0, 0);
+      }
+      if (r < 2) {
+        const float mu = p[OFF_YMU + r], sd = p[OFF_YSD + r], b = bias[3 * HID + r];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const long row = row0 + 16 * u + 4 * q + e;
+          if (row < n) P[row * 2 + r] = (o[e] + b) * sd + mu;
+        }
       }
     }
-    __syncthreads();  // the next tile overwrites A
+    __syncthreads();  // the next group overwrites A
   }
+}
+
+// Launch shapes of the fused inference (waves per block, row tiles per wave), both within
+// the 160 KB of LDS per CU next to the 72 KB of packed weights: 16 x 1 (4 waves per SIMD)
+// and 8 x 2 (2 waves per SIMD, each B operand feeding two MFMAs).  They compute the same
+// values in the same order; the first large inference times both and keeps the faster.
+void launch_fused(int shape, const float* X, long n, const float* p, const short* pk, float* P,
+                  hipStream_t s) {
+  const int waves = shape == 0 ? 16 : 8, rt = shape == 0 ? 1 : 2;
+  const long groups = (n + 16 * rt - 1) / (16 * rt);
+  long blocks = (groups + waves - 1) / waves;
+  if (blocks > 1024) blocks = 1024;
+  if (shape == 0)
+    hipLaunchKernelGGL((k_mlp_fused<16, 1>), dim3((unsigned)blocks), dim3(64 * 16), 0, s, X, n, p, pk, P);
+  else
+    hipLaunchKernelGGL((k_mlp_fused<8, 2>), dim3((unsigned)blocks), dim3(64 * 8), 0, s, X, n, p, pk, P);
 }
 
 unsigned g1(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
@@ -335,6 +370,8 @@ struct hyq_mlp {
   long step;
   hipStream_t stream;
   short* packed;  // bf16 weights in MFMA operand order (fused inference)
+  int shape;      // fused launch shape chosen by timing (-1: not yet)
+  hipEvent_t ev[2];
   long version, packed_version;  // blob changes / version the packed copy was made from
 };
 
@@ -431,6 +468,9 @@ int hyq_mlp_create(const float* blob_host, hyq_mlp** out) {
   MCHK(hipMemset(m->m1, 0, sizeof(float) * NPARAM), "adam m");
   MCHK(hipMemset(m->m2, 0, sizeof(float) * NPARAM), "adam v");
   MCHK(hipMalloc(&m->packed, sizeof(short) * PK_N), "mlp packed weights");
+  m->shape = -1;
+  MCHK(hipEventCreate(&m->ev[0]), "mlp event");
+  MCHK(hipEventCreate(&m->ev[1]), "mlp event");
   m->version = 1;
   m->packed_version = 0;
   *out = m;
@@ -444,6 +484,8 @@ int hyq_mlp_destroy(hyq_mlp* m) {
                    m->H2, m->Z3, m->H3, m->Y, m->dY, m->dA, m->dB, m->iX, m->iA, m->iB, m->iY};
   for (float* b : bufs) (void)hipFree(b);
   (void)hipFree(m->packed);
+  for (hipEvent_t e : m->ev)
+    if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(m->stream);
   delete m;
   return HYQ_OK;
@@ -478,11 +520,19 @@ int hyq_mlp_infer_async(hyq_mlp* m, const float* X, int64_t n, float* P, hipStre
     hipLaunchKernelGGL(k_pack, dim3(g1(PK_N)), dim3(256), 0, stream, m->blob, m->packed);
     m->packed_version = m->version;
   }
-  const long tiles = ((long)n + 15) / 16;
-  long blocks = (tiles + 3) / 4;
-  if (blocks > 2048) blocks = 2048;
-  hipLaunchKernelGGL(k_mlp_fused, dim3((unsigned)blocks), dim3(256), 0, stream, X, (long)n, m->blob,
-                     m->packed, P);
+  if (m->shape < 0 && n >= 65536) {
+    // pick the faster launch shape on this workload (one timed run of each)
+    float ms[2] = {0.f, 0.f};
+    for (int sh = 0; sh < 2; ++sh) {
+      MCHK(hipEventRecord(m->ev[0], stream), "mlp event");
+      launch_fused(sh, X, (long)n, m->blob, m->packed, P, stream);
+      MCHK(hipEventRecord(m->ev[1], stream), "mlp event");
+      MCHK(hipEventSynchronize(m->ev[1]), "mlp event");
+      MCHK(hipEventElapsedTime(&ms[sh], m->ev[0], m->ev[1]), "mlp event");
+    }
+    m->shape = ms[1] < ms[0] ? 1 : 0;
+  }
+  launch_fused(m->shape < 0 ? 0 : m->shape, X, (long)n, m->blob, m->packed, P, stream);
   MCHK(hipGetLastError(), "mlp infer launch");
   return HYQ_OK;
 }

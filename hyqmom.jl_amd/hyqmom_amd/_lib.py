@@ -7,7 +7,7 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(os.path.dirname(HERE), "build", "libhyqmom_hip.so")
+LIBPATH = os.environ.get("HYQMOM_AMD_LIB") or os.path.join(os.path.dirname(HERE), "build", "libhyqmom_hip.so")
 
 HYQ_OK = 0
 ERRORS = {-1: "HYQ_EINVAL", -2: "HYQ_ENOMEM", -3: "HYQ_EHIP", -4: "HYQ_ENONFINITE",

@@ -35,10 +35,11 @@ def test_forward_matches_reference(n):
     assert np.abs(P - ref64).max() / ysd.max() > 0 or n == 1
 
 
-@pytest.mark.parametrize("n", [1, 63, 4099])
+@pytest.mark.parametrize("n", [1, 63, 4099, 70001])
 def test_fused_inference_equals_layers(n):
     """The fused kernel (weights packed in LDS, activations per wave) rounds and accumulates
-    exactly like the layer-by-layer kernels: bitwise equal predictions."""
+    exactly like the layer-by-layer kernels: bitwise equal predictions (n = 70001 also runs
+    the launch-shape timing, which keeps the faster of two equal-valued shapes)."""
     import torch
     from hyqmom_amd.warmstart import WarmStartMLP
     blob = R.random_blob(13)
@@ -168,3 +169,42 @@ def test_newton_and_learned_modes_match_mixed():
         scale = np.abs(Mm).max(axis=(0, 1, 2))
         e = (np.abs(Mx - Mm) / scale).max(axis=3)
         assert e.max() < 1e-6, (prec, e.max())
+
+
+def test_trained_network_reduces_iterations():
+    """What the trained network (hyqmom.jl_amd/models/warmstart_mlp.f32) gains: on a
+    developed crossing-jets state, the Laguerre iterations of the extreme wave speeds with
+    the network's warm starts against the analytic bounds, at identical results."""
+    import os
+    import hyqmom_amd as H
+    from hyqmom_amd import _lib
+    from hyqmom_amd.warmstart import DEFAULT_MODEL, WarmStartMLP
+    assert os.path.exists(DEFAULT_MODEL)
+    N = (60, 60, 30)
+    p = _params(N, 1.5, 0.01, "fp64")
+    s = H.Solver(p)
+    s.set_state(H.initial_state(p))
+    s.run(10.0, 25)
+    M = s.get_state()
+    s.close()
+    out = {}
+    for prec in ("mixed_newton", "mixed_learned"):
+        q = dict(p, precision=prec)
+        if prec == "mixed_learned":
+            q["warmstart_model"] = WarmStartMLP.load(DEFAULT_MODEL)
+        s = H.Solver(q)
+        s.set_state(M)
+        s.step()
+        fl = s.flags()
+        out[prec] = (_lib.newton_iterations(fl).mean(), ((fl >> 12) & 1).mean(), s.wave_speeds(), s.get_state())
+        s.close()
+    it_n, fb_n, v_n, M_n = out["mixed_newton"]
+    it_l, fb_l, v_l, M_l = out["mixed_learned"]
+    print("iterations per cell: analytic start %.2f, network start %.2f (%.0f%% fewer); fallback "
+          "cells %.4f / %.4f" % (it_n, it_l, 100 * (1 - it_l / it_n), fb_n, fb_l))
+    assert it_l < 0.85 * it_n
+    assert fb_l == fb_n  # the hyperbolicity decision does not depend on the start
+    scale = np.abs(v_n).max(axis=1, keepdims=True)
+    assert (np.abs(v_l - v_n) / scale).max() < 2e-7
+    sc = np.abs(M_n).max(axis=(0, 1, 2))
+    assert (np.abs(M_l - M_n) / sc).max() < 1e-6
