@@ -32,8 +32,10 @@ __device__ __forceinline__ unsigned long long ewave_max(unsigned long long v) {
   return v;
 }
 
+// at least two waves per SIMD: as for the single-run kernels, latency hiding pays for the
+// spills this forces
 template <int B, typename FT, int EM>
-__global__ __launch_bounds__(B) void ke_pass1(const double* __restrict__ M, double* __restrict__ Mr,
+__global__ __launch_bounds__(B, 2) void ke_pass1(const double* __restrict__ M, double* __restrict__ Mr,
                                               FT* __restrict__ F, FT* __restrict__ vp,
                                               uint32_t* __restrict__ flags, RunDev* R,
                                               const Blk* __restrict__ T, long long P) {
@@ -80,7 +82,7 @@ __global__ __launch_bounds__(B) void ke_update(const double* __restrict__ Mr, co
 }
 
 template <int B, bool MIXED, int EM>
-__global__ __launch_bounds__(B) void ke_pass2(double* __restrict__ M, const RunDev* R,
+__global__ __launch_bounds__(B, 2) void ke_pass2(double* __restrict__ M, const RunDev* R,
                                               const Blk* __restrict__ T, long long P) {
   const Blk b = T[blockIdx.x];
   const RunDev* q = R + b.run;
@@ -113,13 +115,13 @@ void launch_cells(int which, int b, unsigned nblk, const Blk* T, const EnsPtrs& 
 #define HYQ_EL(B)                                                                                    \
   do {                                                                                               \
     if (which == 0)                                                                                  \
-      hipLaunchKernelGGL((ke_pass1<B, FT, EM>), dim3(nblk), dim3(B), qr_lds_bytes(B), s, p.M, p.Mr, F, vp, p.flags, \
+      hipLaunchKernelGGL((ke_pass1<B, FT, EM>), dim3(nblk), dim3(B), EM == 0 ? qr_lds_bytes(B) : 0u, s, p.M, p.Mr, F, vp, p.flags, \
                          p.R, T, p.P);                                                               \
     else if (which == 1)                                                                             \
       hipLaunchKernelGGL((ke_update<B, FT>), dim3(nblk), dim3(B), 0, s, p.Mr, F, vp, p.M, p.R, T,     \
                          p.P);                                                                       \
     else                                                                                             \
-      hipLaunchKernelGGL((ke_pass2<B, MIXED, EM>), dim3(nblk), dim3(B), qr_lds_bytes(B), s, p.M, p.R, T, p.P);      \
+      hipLaunchKernelGGL((ke_pass2<B, MIXED, EM>), dim3(nblk), dim3(B), EM == 0 ? qr_lds_bytes(B) : 0u, s, p.M, p.R, T, p.P);      \
   } while (0)
   if (b == 64) HYQ_EL(64);
   else if (b == 128) HYQ_EL(128);

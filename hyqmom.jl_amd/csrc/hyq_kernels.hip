@@ -46,12 +46,19 @@ int fail(int code, const char* what, hipError_t e = hipSuccess) {
   } while (0)
 
 constexpr int kBlock = 128;
-// minimum waves per SIMD the register allocator must allow for the pass-1/pass-2 kernels
-// (1: no spills to scratch, low occupancy; 2: some spills, twice the waves to hide latency)
+// Minimum waves per SIMD the register allocator must allow for the pass-1/pass-2 kernels:
+// more waves hide more latency at the price of spills to scratch.  With the QR solve's
+// matrix in LDS (full-solve kernels, EM = 0), 288 B of LDS per lane cap occupancy at 2;
+// the fast kernels (EM = 1) keep their rare fallback QR in lane-private memory and may go
+// higher.
 #ifndef HYQ_PASS_WAVES
-#define HYQ_PASS_WAVES 1
+#define HYQ_PASS_WAVES 2
 #endif
-constexpr int kPassWaves = HYQ_PASS_WAVES;
+#ifndef HYQ_FAST_WAVES
+#define HYQ_FAST_WAVES 2
+#endif
+constexpr int kPassWaves = HYQ_PASS_WAVES, kFastWaves = HYQ_FAST_WAVES;
+
 constexpr int kNev = 5;  // timing events per step
 // dynamic LDS of the kernels that run the QR eigen solve (hyq_eig.h: qr_mat)
 constexpr unsigned kQrLds = qr_lds_bytes(kBlock);
@@ -76,7 +83,7 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
 }
 
 template <typename FT, int EM>
-__global__ __launch_bounds__(kBlock, kPassWaves) void k_pass1(const double* __restrict__ M,
+__global__ __launch_bounds__(kBlock, EM == 0 ? kPassWaves : kFastWaves) void k_pass1(const double* __restrict__ M,
                                                   double* __restrict__ Mr, FT* __restrict__ F,
                                                   FT* __restrict__ vp, uint32_t* __restrict__ flags,
                                                   DevScalars* sc, long long N, int flag2D,
@@ -144,7 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_update(const double* __restrict__ Mr
 
 // ---- pass 2 + BGK ----------------------------------------------------------------------
 template <bool MIXED, int EM>
-__global__ __launch_bounds__(kBlock, kPassWaves) void k_pass2(double* __restrict__ M, const DevScalars* sc,
+__global__ __launch_bounds__(kBlock, EM == 0 ? kPassWaves : kFastWaves) void k_pass2(double* __restrict__ M, const DevScalars* sc,
                                                   long long N, int flag2D, double Ma,
                                                   int positivity, double Kn) {
   if (sc->active == 0.0) return;
@@ -327,7 +334,7 @@ __global__ __launch_bounds__(kBlock, kPassWaves) void k_pass1a(const double* __r
   }
 }
 // Pass 1, second half, with the network's predictions P [5N][2] as Newton warm starts.
-__global__ __launch_bounds__(kBlock, kPassWaves) void k_pass1b(double* __restrict__ AMr, float* __restrict__ F,
+__global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1b(double* __restrict__ AMr, float* __restrict__ F,
                                                    float* __restrict__ vp, uint32_t* __restrict__ flags,
                                                    const float* __restrict__ P, DevScalars* sc,
                                                    long long N, int flag2D, double Ma, int positivity) {
@@ -481,7 +488,7 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
                        (float*)c->F, (float*)c->vp, c->flags, c->sc, c->N, p.flag2D, p.Ma,
                        p.positivity);
   } else if (p.precision == HYQ_PREC_MIXED_NEWTON || c->mlp == nullptr) {
-    hipLaunchKernelGGL((k_pass1<float, 1>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->Mr,
+    hipLaunchKernelGGL((k_pass1<float, 1>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
                        (float*)c->F, (float*)c->vp, c->flags, c->sc, c->N, p.flag2D, p.Ma,
                        p.positivity);
   } else {
@@ -490,7 +497,7 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
                        c->N, p.flag2D, p.Ma);
     const int rc = hyq_mlp_infer_async(c->mlp, c->X, 5 * c->N, c->P, c->stream);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_pass1b, dim3(g), dim3(kBlock), kQrLds, c->stream, c->Mr, (float*)c->F,
+    hipLaunchKernelGGL(k_pass1b, dim3(g), dim3(kBlock), 0, c->stream, c->Mr, (float*)c->F,
                        (float*)c->vp, c->flags, c->P, c->sc, c->N, p.flag2D, p.Ma, p.positivity);
   }
   if (ev) hipEventRecord(ev[1], c->stream);
@@ -513,7 +520,7 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
     hipLaunchKernelGGL((k_pass2<true, 0>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->sc,
                        c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
   } else {
-    hipLaunchKernelGGL((k_pass2<true, 1>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->sc,
+    hipLaunchKernelGGL((k_pass2<true, 1>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->sc,
                        c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
   }
   if (ev) hipEventRecord(ev[4], c->stream);

@@ -59,17 +59,18 @@ extern "C" {
 #define HYQ_FLAG_RHOCLAMP (1u << 11)    /* density clamp in the z eigen path */
 #define HYQ_FLAG_EIG_FALLBACK (1u << 12) /* learned warm start fell back to the full solve */
 #define HYQ_FLAG_QR_NOCONV (1u << 13)   /* QR iteration limit reached */
-/* bits 16..27: Newton iterations summed over the cell's fast plane solves (precision
-   HYQ_PREC_MIXED_NEWTON / HYQ_PREC_MIXED_LEARNED; saturates at 4095) */
+/* bits 16..27: root-polishing (Laguerre) iterations summed over the cell's fast plane
+   solves (precision HYQ_PREC_MIXED_NEWTON / HYQ_PREC_MIXED_LEARNED; saturates at 4095) */
 #define HYQ_FLAG_NEWTON_SHIFT 16
 #define HYQ_FLAG_NEWTON_MASK 0xFFFu
 
 /* precision modes */
 #define HYQ_PREC_FP64 0           /* everything FP64 */
 #define HYQ_PREC_MIXED 1          /* FP32 fluxes, HLL update and BGK relaxation */
-#define HYQ_PREC_MIXED_LEARNED 2  /* MIXED + MLP warm-started Newton wave speeds */
-#define HYQ_PREC_MIXED_NEWTON 3   /* MIXED + Newton wave speeds from analytic bounds (the
-                                     learned mode without its network, for comparison) */
+#define HYQ_PREC_MIXED_LEARNED 2  /* MIXED + extreme wave speeds by a Sturm test and Laguerre
+                                     iteration seeded by the warm-start network */
+#define HYQ_PREC_MIXED_NEWTON 3   /* the same seeded by the analytic root bounds (the learned
+                                     mode without its network) */
 
 /* Mirrors the params NamedTuple of simulation_runner (src/simulation_runner.jl:20-37,
  * 74-113).  Fields the device step does not need (snapshot, symmetry, debug, MPI) stay
