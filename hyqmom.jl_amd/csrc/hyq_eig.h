@@ -284,31 +284,14 @@ struct QrMat {
 };
 inline QrMat qr_mat() { return QrMat{}; }
 #endif
-// Lane-private storage (scratch memory on the device): for kernels where the QR solve is a
-// rare fallback and LDS would only cap occupancy.
-struct LocalMat {
-  double a[36];
-  HD double& operator()(int i, int j) { return a[i * 6 + j]; }
-};
-template <bool LDS>
-struct QrStore;
-template <>
-struct QrStore<true> {
-  HD static QrMat get() { return qr_mat(); }
-};
-template <>
-struct QrStore<false> {
-  HD static LocalMat get() { return LocalMat{}; }
-};
 
 struct EigRes {
   double vmin, vmax;  // extreme real parts (raw scale), NaN if not finite
   bool cplx;          // some |imag| > 1000 eps (raw scale)
   bool ok;
 };
-// Eigenvalues of the plane Jacobian mapped back to the raw scale.  LDS: the QR matrix in the
-// launch's dynamic LDS (qr_lds_bytes) rather than in lane-private scratch.
-template <bool LDS = true>
+// Eigenvalues of the plane Jacobian mapped back to the raw scale (the QR matrix in the
+// launch's dynamic LDS, qr_lds_bytes).
 HD EigRes plane_eigs(const Plane& p, uint32_t& fl) {
   EigRes r;
   double H[36];
@@ -324,7 +307,7 @@ HD EigRes plane_eigs(const Plane& p, uint32_t& fl) {
   }
   balance_n<6>(H);
   hessenberg_n<6>(H);
-  auto Q = QrStore<LDS>::get();
+  QrMat Q = qr_mat();
 #pragma unroll
   for (int i = 0; i < 6; ++i)
 #pragma unroll

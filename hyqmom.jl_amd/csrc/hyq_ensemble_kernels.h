@@ -115,13 +115,13 @@ void launch_cells(int which, int b, unsigned nblk, const Blk* T, const EnsPtrs& 
 #define HYQ_EL(B)                                                                                    \
   do {                                                                                               \
     if (which == 0)                                                                                  \
-      hipLaunchKernelGGL((ke_pass1<B, FT, EM>), dim3(nblk), dim3(B), EM == 0 ? qr_lds_bytes(B) : 0u, s, p.M, p.Mr, F, vp, p.flags, \
+      hipLaunchKernelGGL((ke_pass1<B, FT, EM>), dim3(nblk), dim3(B), qr_lds_bytes(B), s, p.M, p.Mr, F, vp, p.flags, \
                          p.R, T, p.P);                                                               \
     else if (which == 1)                                                                             \
       hipLaunchKernelGGL((ke_update<B, FT>), dim3(nblk), dim3(B), 0, s, p.Mr, F, vp, p.M, p.R, T,     \
                          p.P);                                                                       \
     else                                                                                             \
-      hipLaunchKernelGGL((ke_pass2<B, MIXED, EM>), dim3(nblk), dim3(B), EM == 0 ? qr_lds_bytes(B) : 0u, s, p.M, p.R, T, p.P);      \
+      hipLaunchKernelGGL((ke_pass2<B, MIXED, EM>), dim3(nblk), dim3(B), qr_lds_bytes(B), s, p.M, p.R, T, p.P);      \
   } while (0)
   if (b == 64) HYQ_EL(64);
   else if (b == 128) HYQ_EL(128);

@@ -47,10 +47,10 @@ int fail(int code, const char* what, hipError_t e = hipSuccess) {
 
 constexpr int kBlock = 128;
 // Minimum waves per SIMD the register allocator must allow for the pass-1/pass-2 kernels:
-// more waves hide more latency at the price of spills to scratch.  With the QR solve's
-// matrix in LDS (full-solve kernels, EM = 0), 288 B of LDS per lane cap occupancy at 2;
-// the fast kernels (EM = 1) keep their rare fallback QR in lane-private memory and may go
-// higher.
+// more waves hide more latency at the price of spills to scratch.  The QR solve's matrix
+// in LDS (288 B per lane) caps occupancy at 2 anyway; measured on the 120x120x60 workload,
+// 2 beats 1 for every pass kernel, and the fast kernels (EM = 1) are slower at 3 or 4 with
+// their QR fallback in lane-private memory.
 #ifndef HYQ_PASS_WAVES
 #define HYQ_PASS_WAVES 2
 #endif
@@ -488,7 +488,7 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
                        (float*)c->F, (float*)c->vp, c->flags, c->sc, c->N, p.flag2D, p.Ma,
                        p.positivity);
   } else if (p.precision == HYQ_PREC_MIXED_NEWTON || c->mlp == nullptr) {
-    hipLaunchKernelGGL((k_pass1<float, 1>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
+    hipLaunchKernelGGL((k_pass1<float, 1>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->Mr,
                        (float*)c->F, (float*)c->vp, c->flags, c->sc, c->N, p.flag2D, p.Ma,
                        p.positivity);
   } else {
@@ -497,7 +497,7 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
                        c->N, p.flag2D, p.Ma);
     const int rc = hyq_mlp_infer_async(c->mlp, c->X, 5 * c->N, c->P, c->stream);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_pass1b, dim3(g), dim3(kBlock), 0, c->stream, c->Mr, (float*)c->F,
+    hipLaunchKernelGGL(k_pass1b, dim3(g), dim3(kBlock), kQrLds, c->stream, c->Mr, (float*)c->F,
                        (float*)c->vp, c->flags, c->P, c->sc, c->N, p.flag2D, p.Ma, p.positivity);
   }
   if (ev) hipEventRecord(ev[1], c->stream);
@@ -520,7 +520,7 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
     hipLaunchKernelGGL((k_pass2<true, 0>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->sc,
                        c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
   } else {
-    hipLaunchKernelGGL((k_pass2<true, 1>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->sc,
+    hipLaunchKernelGGL((k_pass2<true, 1>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->sc,
                        c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
   }
   if (ev) hipEventRecord(ev[4], c->stream);

@@ -1,4 +1,4 @@
-// Learned warm start: a 15 -> 128 -> 128 -> 128 -> 2 MLP (GELU) that predicts the extreme
+// Learned warm start: a 15 -> 128 -> 128 -> 128 -> 2 MLP (tanh-form GELU) that predicts the extreme
 // eigenvalues of the standardized flux Jacobian from the 15 standardized moments of a plane.
 // Matrix products run on MFMA (v_mfma_f32_16x16x16_bf16: bf16 operands, FP32 accumulate);
 // weights are kept as FP32 master copies and rounded to bf16 as they are loaded.  Training
@@ -48,6 +48,18 @@ __device__ __forceinline__ short f2bf(float f) {  // round to nearest even
   unsigned u = __float_as_uint(f);
   u += 0x7fffu + ((u >> 16) & 1u);
   return (short)(u >> 16);
+}
+
+// Activation: the tanh form of GELU, z * sigmoid(2 sqrt(2/pi) (z + 0.044715 z^3)), with the
+// hardware exp and reciprocal (its inputs and outputs are rounded to bf16 around the MFMAs,
+// so the few-ulp error of the fast exp and reciprocal is invisible); used identically by
+// training, the layer kernels and the fused inference.
+__device__ __forceinline__ float act_u(float z) { return 1.5957691216057308f * (z + 0.044715f * z * z * z); }
+__device__ __forceinline__ float act_sig(float u) { return __builtin_amdgcn_rcpf(1.0f + __expf(-u)); }
+__device__ __forceinline__ float gelu(float z) { return z * act_sig(act_u(z)); }
+__device__ __forceinline__ float dgelu(float z) {
+  const float s = act_sig(act_u(z));
+  return s + z * s * (1.0f - s) * 1.5957691216057308f * (1.0f + 0.134145f * z * z);
 }
 
 // C[M x N] = A[M x K] . B[K x N] (+ bias), A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn].
@@ -102,7 +114,7 @@ __global__ __launch_bounds__(256) void k_gemm(const float* __restrict__ A, long 
       if (orow >= M) continue;
       const float z = acc[t][e] + bv;
       if (Z) Z[orow * ldc + col] = z;
-      if (H) H[orow * ldc + col] = act == 1 ? 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)) : z;
+      if (H) H[orow * ldc + col] = act == 1 ? gelu(z) : z;
     }
   }
 }
@@ -111,10 +123,7 @@ __global__ __launch_bounds__(256) void k_gemm(const float* __restrict__ A, long 
 __global__ void k_gelu_bwd(const float* __restrict__ Z, float* __restrict__ dH, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float z = Z[i];
-  const float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
-  const float pdf = 0.3989422804014327f * expf(-0.5f * z * z);
-  dH[i] *= cdf + z * pdf;
+  dH[i] *= dgelu(Z[i]);
 }
 // normalized features: Xn[b][j] = (X[b][j] - mu[j]) / sd[j]  (j < 15), 0 for j = 15
 __global__ void k_normalize(const float* __restrict__ X, const float* __restrict__ stat,
@@ -220,7 +229,6 @@ __global__ void k_pack(const float* __restrict__ p, short* __restrict__ pk) {
   pk[i] = f2bf(W[row * ncol + col]);
 }
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
 
 // WAVES waves per block share the weights; each wave takes RT 16-row tiles at a time, so
 // every B operand read from LDS feeds RT MFMAs.

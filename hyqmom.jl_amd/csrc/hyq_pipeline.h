@@ -35,13 +35,12 @@ HD void pack35(const double* c, double* M) {
   for (int n = 0; n < 35; ++n) M[n] = c[T35(n)];
 }
 
-// Eigen solve of one plane in mode EM: 0 = full FP64 solve (QR, its matrix in LDS); 1 =
-// Sturm test + Laguerre from warm starts (wmin/wmax in standardized units, NaN = analytic
-// bounds), falling back to the full solve (matrix in lane-private memory) when hyperbolicity
-// is not proven or the iteration does not converge.
+// Eigen solve of one plane in mode EM: 0 = full FP64 solve (QR); 1 = Sturm test + Laguerre
+// from warm starts (wmin/wmax in standardized units, NaN = analytic bounds), falling back
+// to the full solve when hyperbolicity is not proven or the iteration does not converge.
 template <int EM>
 HD EigRes plane_solve(const Plane& p, float wmin, float wmax, bool need_vals, uint32_t& fl) {
-  if (EM == 0) return plane_eigs<true>(p, fl);
+  if (EM == 0) return plane_eigs(p, fl);
   if (p.ok) {
     double J[36];
     jacobian_std(p, J);
@@ -64,7 +63,7 @@ HD EigRes plane_solve(const Plane& p, float wmin, float wmax, bool need_vals, ui
     }
   }
   fl |= F_FALLBACK;
-  return plane_eigs<false>(p, fl);  // rare: lane-private storage, no LDS held by the kernel
+  return plane_eigs(p, fl);
 }
 // Warm starts of the five distinct plane Jacobians of a cell (UV, UW, VU, VW, WU), each
 // (min, max); NaN when absent.

@@ -1,18 +1,19 @@
 """CPU reference of the warm-start network (test infrastructure only).
 
-The network has no counterpart in HyQMOM.jl: it only seeds the Newton solve of the wave
-speeds, whose result the FP64 step checks and, failing that, replaces by the full eigen
-solve.  This module restates, in numpy, what the device code computes:
+The network has no counterpart in HyQMOM.jl: it only seeds the root-polishing iteration of
+the wave speeds, whose starting point the FP64 step verifies (Sturm count) and otherwise
+replaces by the analytic bound.  This module restates, in numpy, what the device code computes:
 
   Xn = (X[:, :15] - xmu) / xsd, padded with a zero 16th column
-  H1 = gelu(Xn W1 + b1), H2 = gelu(H1 W2 + b2), H3 = gelu(H2 W3 + b3), Y = H3 W4 + b4
+  H1 = g(Xn W1 + b1), H2 = g(H1 W2 + b2), H3 = g(H2 W3 + b3), Y = H3 W4 + b4
   prediction (lmin, lmax) = Y[:, :2] * ysd + ymu
   loss = 1/(2n) sum_b sum_{o<2} (Y[b, o] - (T[b, o] - ymu[o]) / ysd[o])^2
 
 with every matrix-product operand rounded to bfloat16 (round to nearest even) and products
 accumulated in high precision, as the MFMA v_mfma_f32_16x16x16_bf16 instructions do (FP32
-accumulate).  gelu is the exact erf form.  With bf16=False everything stays in FP64, which
-the finite-difference check of the backward pass uses.
+accumulate).  g is the tanh form of GELU, z sigmoid(2 sqrt(2/pi) (z + 0.044715 z^3)).
+With bf16=False everything stays in FP64, which the finite-difference check of the backward
+pass uses.
 
 Parameter blob (FP32): W1 [16][128], b1 [128], W2 [128][128], b2, W3 [128][128], b3,
 W4 [128][16], b4 [16], then xmu [16], xsd [16], ymu [2], ysd [2].
@@ -26,12 +27,6 @@ SHAPES = [("W1", (IN, HID)), ("b1", (HID,)), ("W2", (HID, HID)), ("b2", (HID,)),
           ("W3", (HID, HID)), ("b3", (HID,)), ("W4", (HID, OUT)), ("b4", (OUT,))]
 NPARAM = sum(int(np.prod(s)) for _, s in SHAPES)
 NBLOB = NPARAM + 16 + 16 + 2 + 2
-
-try:
-    from scipy.special import erf as _erf
-except ImportError:  # pragma: no cover
-    _erf = np.vectorize(math.erf)
-
 
 def bf16(a):
     """Round float32 values to bfloat16 (nearest even), returned as float32."""
@@ -82,12 +77,18 @@ def _mm(A, B, use_bf16):
     return np.asarray(A, np.float64) @ np.asarray(B, np.float64)
 
 
+def _sig(u):
+    with np.errstate(over="ignore"):
+        return 1.0 / (1.0 + np.exp(-u))
+
+
 def _gelu(z):
-    return 0.5 * z * (1.0 + _erf(z / math.sqrt(2.0)))
+    return z * _sig(1.5957691216057308 * (z + 0.044715 * z ** 3))
 
 
 def _dgelu(z):
-    return 0.5 * (1.0 + _erf(z / math.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / math.sqrt(2.0 * math.pi)
+    s = _sig(1.5957691216057308 * (z + 0.044715 * z ** 3))
+    return s + z * s * (1.0 - s) * 1.5957691216057308 * (1.0 + 0.134145 * z * z)
 
 
 def _f(x, use_bf16):
