@@ -100,8 +100,7 @@ def initialize_moment_field(Nx, Ny, Nz, background, regions, r110=0.0, r101=0.0,
         if any(math.isinf(w) for w in reg.width):
             idx[...] = ri
             continue
-        mx = np.array([cell_overlaps_cube((x, 0, 0), size, CubicRegion(reg.center, (reg.width[0], math.inf, math.inf))) if False else
-                       (max(x - dx / 2, reg.center[0] - reg.width[0] / 2) < min(x + dx / 2, reg.center[0] + reg.width[0] / 2)) for x in xm])
+        mx = np.array([(max(x - dx / 2, reg.center[0] - reg.width[0] / 2) < min(x + dx / 2, reg.center[0] + reg.width[0] / 2)) for x in xm])
         my = np.array([(max(y - dy / 2, reg.center[1] - reg.width[1] / 2) < min(y + dy / 2, reg.center[1] + reg.width[1] / 2)) for y in ym])
         mz = np.array([(max(z - dz / 2, reg.center[2] - reg.width[2] / 2) < min(z + dz / 2, reg.center[2] + reg.width[2] / 2)) for z in zm])
         sel = mx[:, None, None] & my[None, :, None] & mz[None, None, :]
