@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--model", default=None, help="warm-start network blob (mixed_learned)")
     ap.add_argument("--grid", default="%d,%d,%d" % GRID)
     ap.add_argument("--compare-fp64", type=int, default=1)
+    ap.add_argument("--compare-newton", type=int, default=1,
+                    help="with mixed_learned: also time the same step without the network")
     ap.add_argument("--count-flops", type=int, default=1)
     ap.add_argument("--accuracy", type=int, default=1)
     ap.add_argument("--cpu-baseline", type=int, default=1)
@@ -261,6 +263,14 @@ def main():
         out["fp64"] = {"value": fv, "ms_per_step": f["seconds"] * 1e3 / a.steps,
                        "kernels": kernel_report(f, fflops, "fp64", N)}
         out["speedup_vs_fp64"] = value / fv
+    if a.compare_newton and a.precision == "mixed_learned":
+        w = timed_run(N3, "mixed_newton", M0, a.steps, a.warmup)
+        wv = N * a.steps / w["seconds"]
+        out["mixed_newton"] = {"value": wv, "ms_per_step": w["seconds"] * 1e3 / a.steps,
+                               "kernels": {k: w[k] for k in ("pass1_ms", "update_ms", "pass2_ms")},
+                               "fast_wave_speeds_last_step": w["last_step"]}
+        if "fp64" in out:
+            out["mixed_newton"]["speedup_vs_fp64"] = wv / out["fp64"]["value"]
     if a.accuracy and a.precision != "fp64":
         out["accuracy"] = dict(accuracy(N3, a.precision, Mmid, a.model), from_step=mid)
     if a.cpu_baseline:

@@ -26,6 +26,8 @@ using namespace hyq;
 
 extern "C" int hyq_mlp_infer_async(hyq_mlp* m, const float* X, int64_t n, float* P,
                                    hipStream_t stream);
+extern "C" int hyq_mlp_infer_masked_async(hyq_mlp* m, const float* X, int64_t n, float* P,
+                                          hipStream_t stream);
 
 namespace {
 
@@ -268,7 +270,10 @@ __device__ void plane_feature(const Plane& p, float* x) {
   x[0] = 1.f; x[1] = 0.f; x[2] = 1.f; x[3] = (float)p.S03; x[4] = (float)p.S04;
   x[5] = 0.f; x[6] = (float)p.S11; x[7] = (float)p.S12; x[8] = (float)p.S13;
   x[9] = 1.f; x[10] = (float)p.S21; x[11] = (float)p.S22; x[12] = (float)p.S30;
-  x[13] = (float)p.S31; x[14] = (float)p.S40; x[15] = 0.f;
+  x[13] = (float)p.S31; x[14] = (float)p.S40;
+  // column 15 is not a network input (normalization zeroes it): it marks the planes the
+  // step's masked inference must evaluate (the others take the Gaussian seed)
+  x[15] = plane_near_gaussian(p) ? 0.f : 1.f;
 }
 __device__ void cell_planes(const double* M, Plane pl[5]) {
   double c[35], M000, um, vm, wm;
@@ -294,6 +299,7 @@ __global__ __launch_bounds__(kBlock) void k_plane_dataset(const double* __restri
   for (int q = 0; q < 5; ++q) {
     float x[16];
     plane_feature(pl[q], x);
+    x[15] = 0.f;
     float4* dst = reinterpret_cast<float4*>(X + ((size_t)q * n + c) * 16);
 #pragma unroll
     for (int t = 0; t < 4; ++t) dst[t] = make_float4(x[4 * t], x[4 * t + 1], x[4 * t + 2], x[4 * t + 3]);
@@ -495,7 +501,7 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
     // learned: projection + features, network warm starts, wave speeds + fluxes
     hipLaunchKernelGGL(k_pass1a, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr, c->X, c->sc,
                        c->N, p.flag2D, p.Ma);
-    const int rc = hyq_mlp_infer_async(c->mlp, c->X, 5 * c->N, c->P, c->stream);
+    const int rc = hyq_mlp_infer_masked_async(c->mlp, c->X, 5 * c->N, c->P, c->stream);
     if (rc) return rc;
     hipLaunchKernelGGL(k_pass1b, dim3(g), dim3(kBlock), kQrLds, c->stream, c->Mr, (float*)c->F,
                        (float*)c->vp, c->flags, c->P, c->sc, c->N, p.flag2D, p.Ma, p.positivity);

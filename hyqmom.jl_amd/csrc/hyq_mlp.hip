@@ -44,10 +44,8 @@ constexpr size_t OFF_XMU = NPARAM, OFF_XSD = OFF_XMU + 16, OFF_YMU = OFF_XSD + 1
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ short f2bf(float f) {  // round to nearest even
-  unsigned u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (short)(u >> 16);
+__device__ __forceinline__ short f2bf(float f) {  // round to nearest even (v_cvt_pk_bf16_f32)
+  return __builtin_bit_cast(short, (__bf16)f);
 }
 
 // Activation: the tanh form of GELU, z * sigmoid(2 sqrt(2/pi) (z + 0.044715 z^3)), with the
@@ -231,8 +229,10 @@ __global__ void k_pack(const float* __restrict__ p, short* __restrict__ pk) {
 
 
 // WAVES waves per block share the weights; each wave takes RT 16-row tiles at a time, so
-// every B operand read from LDS feeds RT MFMAs.
-template <int WAVES, int RT>
+// every B operand read from LDS feeds RT MFMAs.  MASK: rows whose column 15 is 0 need no
+// prediction; a wave whose rows all are such writes NaN predictions and skips the work
+// (it still takes part in the block's barriers).
+template <int WAVES, int RT, bool MASK>
 __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restrict__ X, long n,
                                                           const float* __restrict__ p,
                                                           const short* __restrict__ pk,
@@ -256,80 +256,98 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
   // block-uniform loop: every wave runs every iteration (rows past n are zeros, not stored)
   for (long base = (long)blockIdx.x * WAVES; base < ngroups; base += (long)gridDim.x * WAVES) {
     const long row0 = (base + w) * 16 * RT;
-    f4 acc[RT][8];
-    // layer 1: normalized input (k = 4q..4q+3 of 16) times W1
-#pragma unroll
-    for (int u = 0; u < RT; ++u) {
-      const long row = row0 + 16 * u + r;
-      s4 a;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int k = 4 * q + e;
-        float v = 0.f;
-        if (row < n && k < 15) v = (X[row * 16 + k] - p[OFF_XMU + k]) / p[OFF_XSD + k];
-        a[e] = f2bf(v);
+    bool work = true;
+    if (MASK) {
+      const long row = row0 + lane;
+      const bool need = lane < 16 * RT && row < n && X[row * 16 + 15] != 0.f;
+      work = __ballot(need) != 0ull;
+      if (!work && lane < 16 * RT && row < n) {
+        P[row * 2] = NAN;
+        P[row * 2 + 1] = NAN;
       }
-#pragma unroll
-      for (int t = 0; t < 8; ++t)
-        acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wts[(PK_W1 / 4) + t * 64 + lane],
-                                                              f4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
     }
+    f4 acc[RT][8];
+    if (work) {
+      // layer 1: normalized input (k = 4q..4q+3 of 16) times W1
 #pragma unroll
-    for (int u = 0; u < RT; ++u)
+      for (int u = 0; u < RT; ++u) {
+        const long row = row0 + 16 * u + r;
+        s4 a;
 #pragma unroll
-      for (int t = 0; t < 8; ++t)
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-          A[(16 * u + 4 * q + e) * ACT_LD + 16 * t + r] = f2bf(gelu(acc[u][t][e] + bias[16 * t + r]));
-    __syncthreads();
-    // layers 2 and 3
-#pragma unroll
-    for (int L = 0; L < 2; ++L) {
-      const int wb = (L == 0 ? PK_W2 : PK_W3) / 4;
-#pragma unroll
-      for (int u = 0; u < RT; ++u)
-#pragma unroll
-        for (int t = 0; t < 8; ++t) acc[u][t] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int ks = 0; ks < 8; ++ks) {
-        s4 av[RT];
-#pragma unroll
-        for (int u = 0; u < RT; ++u)
-          av[u] = *reinterpret_cast<const s4*>(&A[(16 * u + r) * ACT_LD + 16 * ks + 4 * q]);
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          const s4 bv = wts[wb + (ks * 8 + t) * 64 + lane];
-#pragma unroll
-          for (int u = 0; u < RT; ++u)
-            acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[u], bv, acc[u][t], 0, 0, 0);
+        for (int e = 0; e < 4; ++e) {
+          const int k = 4 * q + e;
+          float v = 0.f;
+          if (row < n && k < 15) v = (X[row * 16 + k] - p[OFF_XMU + k]) / p[OFF_XSD + k];
+          a[e] = f2bf(v);
         }
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+          acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wts[(PK_W1 / 4) + t * 64 + lane],
+                                                                f4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
       }
-      __syncthreads();
-      const float* bl = bias + HID * (L + 1);
 #pragma unroll
       for (int u = 0; u < RT; ++u)
 #pragma unroll
         for (int t = 0; t < 8; ++t)
 #pragma unroll
           for (int e = 0; e < 4; ++e)
-            A[(16 * u + 4 * q + e) * ACT_LD + 16 * t + r] = f2bf(gelu(acc[u][t][e] + bl[16 * t + r]));
+            A[(16 * u + 4 * q + e) * ACT_LD + 16 * t + r] = f2bf(gelu(acc[u][t][e] + bias[16 * t + r]));
+    }
+    __syncthreads();
+    // layers 2 and 3
+#pragma unroll
+    for (int L = 0; L < 2; ++L) {
+      const int wb = (L == 0 ? PK_W2 : PK_W3) / 4;
+      if (work) {
+#pragma unroll
+        for (int u = 0; u < RT; ++u)
+#pragma unroll
+          for (int t = 0; t < 8; ++t) acc[u][t] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) {
+          s4 av[RT];
+#pragma unroll
+          for (int u = 0; u < RT; ++u)
+            av[u] = *reinterpret_cast<const s4*>(&A[(16 * u + r) * ACT_LD + 16 * ks + 4 * q]);
+#pragma unroll
+          for (int t = 0; t < 8; ++t) {
+            const s4 bv = wts[wb + (ks * 8 + t) * 64 + lane];
+#pragma unroll
+            for (int u = 0; u < RT; ++u)
+              acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[u], bv, acc[u][t], 0, 0, 0);
+          }
+        }
+      }
+      __syncthreads();
+      if (work) {
+        const float* bl = bias + HID * (L + 1);
+#pragma unroll
+        for (int u = 0; u < RT; ++u)
+#pragma unroll
+          for (int t = 0; t < 8; ++t)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              A[(16 * u + 4 * q + e) * ACT_LD + 16 * t + r] = f2bf(gelu(acc[u][t][e] + bl[16 * t + r]));
+      }
       __syncthreads();
     }
     // layer 4: one 16-column tile, outputs 0 and 1 kept
+    if (work) {
 #pragma unroll
-    for (int u = 0; u < RT; ++u) {
-      f4 o = f4{0.f, 0.f, 0.f, 0.f};
+      for (int u = 0; u < RT; ++u) {
+        f4 o = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ks = 0; ks < 8; ++ks) {
-        const s4 av = *reinterpret_cast<const s4*>(&A[(16 * u + r) * ACT_LD + 16 * ks + 4 * q]);
-        o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, wts[(PK_W4 / 4) + ks * 64 + lane], o, 0, 0, 0);
-      }
-      if (r < 2) {
-        const float mu = p[OFF_YMU + r], sd = p[OFF_YSD + r], b = bias[3 * HID + r];
+        for (int ks = 0; ks < 8; ++ks) {
+          const s4 av = *reinterpret_cast<const s4*>(&A[(16 * u + r) * ACT_LD + 16 * ks + 4 * q]);
+          o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, wts[(PK_W4 / 4) + ks * 64 + lane], o, 0, 0, 0);
+        }
+        if (r < 2) {
+          const float mu = p[OFF_YMU + r], sd = p[OFF_YSD + r], b = bias[3 * HID + r];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const long row = row0 + 16 * u + 4 * q + e;
-          if (row < n) P[row * 2 + r] = (o[e] + b) * sd + mu;
+          for (int e = 0; e < 4; ++e) {
+            const long row = row0 + 16 * u + 4 * q + e;
+            if (row < n) P[row * 2 + r] = (o[e] + b) * sd + mu;
+          }
         }
       }
     }
@@ -341,16 +359,21 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
 // the 160 KB of LDS per CU next to the 72 KB of packed weights: 16 x 1 (4 waves per SIMD)
 // and 8 x 2 (2 waves per SIMD, each B operand feeding two MFMAs).  They compute the same
 // values in the same order; the first large inference times both and keeps the faster.
-void launch_fused(int shape, const float* X, long n, const float* p, const short* pk, float* P,
-                  hipStream_t s) {
+void launch_fused(int shape, bool mask, const float* X, long n, const float* p, const short* pk,
+                  float* P, hipStream_t s) {
   const int waves = shape == 0 ? 16 : 8, rt = shape == 0 ? 1 : 2;
   const long groups = (n + 16 * rt - 1) / (16 * rt);
   long blocks = (groups + waves - 1) / waves;
   if (blocks > 1024) blocks = 1024;
-  if (shape == 0)
-    hipLaunchKernelGGL((k_mlp_fused<16, 1>), dim3((unsigned)blocks), dim3(64 * 16), 0, s, X, n, p, pk, P);
+  const dim3 g((unsigned)blocks);
+  if (shape == 0 && !mask)
+    hipLaunchKernelGGL((k_mlp_fused<16, 1, false>), g, dim3(64 * 16), 0, s, X, n, p, pk, P);
+  else if (shape == 0)
+    hipLaunchKernelGGL((k_mlp_fused<16, 1, true>), g, dim3(64 * 16), 0, s, X, n, p, pk, P);
+  else if (!mask)
+    hipLaunchKernelGGL((k_mlp_fused<8, 2, false>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
   else
-    hipLaunchKernelGGL((k_mlp_fused<8, 2>), dim3((unsigned)blocks), dim3(64 * 8), 0, s, X, n, p, pk, P);
+    hipLaunchKernelGGL((k_mlp_fused<8, 2, true>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
 }
 
 unsigned g1(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
@@ -516,10 +539,8 @@ int hyq_mlp_set_blob(hyq_mlp* m, const float* blob_host) {
   return HYQ_OK;
 }
 
-// Predicted (lmin, lmax) for n feature rows X [n][16] (device) into P [n][2] (device),
-// enqueued on `stream` without synchronizing: the fused kernel (the packed weights are
-// refreshed first when the parameters changed).
-int hyq_mlp_infer_async(hyq_mlp* m, const float* X, int64_t n, float* P, hipStream_t stream) {
+static int infer_fused(hyq_mlp* m, const float* X, int64_t n, float* P, hipStream_t stream,
+                       bool mask) {
   if (!m || !X || !P || n < 0) return mfail(HYQ_EINVAL, "bad argument");
   if (n == 0) return HYQ_OK;
   if (m->packed_version != m->version) {
@@ -533,16 +554,29 @@ int hyq_mlp_infer_async(hyq_mlp* m, const float* X, int64_t n, float* P, hipStre
     float ms[2] = {0.f, 0.f};
     for (int sh = 0; sh < 2; ++sh) {
       MCHK(hipEventRecord(m->ev[0], stream), "mlp event");
-      launch_fused(sh, X, (long)n, m->blob, m->packed, P, stream);
+      launch_fused(sh, mask, X, (long)n, m->blob, m->packed, P, stream);
       MCHK(hipEventRecord(m->ev[1], stream), "mlp event");
       MCHK(hipEventSynchronize(m->ev[1]), "mlp event");
       MCHK(hipEventElapsedTime(&ms[sh], m->ev[0], m->ev[1]), "mlp event");
     }
     m->shape = ms[1] < ms[0] ? 1 : 0;
   }
-  launch_fused(m->shape < 0 ? 0 : m->shape, X, (long)n, m->blob, m->packed, P, stream);
+  launch_fused(m->shape < 0 ? 0 : m->shape, mask, X, (long)n, m->blob, m->packed, P, stream);
   MCHK(hipGetLastError(), "mlp infer launch");
   return HYQ_OK;
+}
+
+// Predicted (lmin, lmax) for n feature rows X [n][16] (device) into P [n][2] (device),
+// enqueued on `stream` without synchronizing: the fused kernel (the packed weights are
+// refreshed first when the parameters changed).
+int hyq_mlp_infer_async(hyq_mlp* m, const float* X, int64_t n, float* P, hipStream_t stream) {
+  return infer_fused(m, X, n, P, stream, false);
+}
+// The step's inference: rows whose column 15 is 0 (near-Gaussian planes) need no
+// prediction; 16-row tiles made only of such rows are skipped and get NaN.
+int hyq_mlp_infer_masked_async(hyq_mlp* m, const float* X, int64_t n, float* P,
+                               hipStream_t stream) {
+  return infer_fused(m, X, n, P, stream, true);
 }
 
 // The same prediction through the layer-by-layer GEMM kernels of training (a cross-check

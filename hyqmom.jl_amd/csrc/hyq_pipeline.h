@@ -35,12 +35,30 @@ HD void pack35(const double* c, double* M) {
   for (int n = 0; n < 35; ++n) M[n] = c[T35(n)];
 }
 
+// Largest deviation of a plane's standardized moments from a Gaussian's (S30 = S03 = S21 =
+// S12 = 0, S40 = S04 = 3, S31 = S13 = 3 S11, S22 = 1 + 2 S11^2).  Below kGaussTol the
+// Gaussian's extreme eigenvalues seed the fast solve (the seed is still verified by the
+// Sturm count); in the learned mode such planes skip the network.
+constexpr double kGaussTol = 1e-3;
+HD double plane_gauss_dev(const Plane& p) {
+  double d = fmax(fmax(fabs(p.S30), fabs(p.S03)), fmax(fabs(p.S21), fabs(p.S12)));
+  d = fmax(d, fmax(fabs(p.S40 - 3.0), fabs(p.S04 - 3.0)));
+  d = fmax(d, fmax(fabs(p.S31 - 3.0 * p.S11), fabs(p.S13 - 3.0 * p.S11)));
+  return fmax(d, fabs(p.S22 - 1.0 - 2.0 * p.S11 * p.S11));
+}
+HD bool plane_near_gaussian(const Plane& p) { return p.ok && plane_gauss_dev(p) < kGaussTol; }
+
 // Eigen solve of one plane in mode EM: 0 = full FP64 solve (QR); 1 = Sturm test + Laguerre
 // from warm starts (wmin/wmax in standardized units, NaN = analytic bounds), falling back
 // to the full solve when hyperbolicity is not proven or the iteration does not converge.
 template <int EM>
 HD EigRes plane_solve(const Plane& p, float wmin, float wmax, bool need_vals, uint32_t& fl) {
   if (EM == 0) return plane_eigs(p, fl);
+  if (wmin != wmin && plane_near_gaussian(p)) {
+    // a Gaussian plane's standardized extreme eigenvalues are -sqrt(3), +sqrt(3)
+    wmin = -1.7320508075688772f;
+    wmax = 1.7320508075688772f;
+  }
   if (p.ok) {
     double J[36];
     jacobian_std(p, J);
