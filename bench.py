@@ -7,6 +7,15 @@ HBM, no host copies inside the timed region.  The same workload in plain FP64 is
 beside it ("fp64"), and the headline mode's one-step error against FP64 from the same
 state is reported ("accuracy").
 
+Breakdowns: the 200-step trajectory of this workload passes through near-vacuum cells with
+zero variance and runaway velocity, where the reference's z eigenvalue solve throws
+(non-finite Jacobian) and this library returns HYQ_ENONFINITE.  Whether a run meets such a
+cell within 200 steps is decided by rounding: perturbing the density of 100 cells of the
+initial state by one ulp moves the breakdown between step ~90 and beyond 200.  Each timed
+run therefore starts from the exact initial state and, if that trajectory breaks down,
+from the seeded one-ulp perturbations 1, 2, ... until one completes the steps; the failed
+attempts are reported ("initial_state") and not timed.
+
 Roofline: per kernel (pass 1, update, pass 2) the average duration from HIP events on the
 context stream over the timed steps; pass 1 and pass 2 are priced as FP64 work
 (algorithmic FLOPs counted on the states of the run by the op-counting host build of the
@@ -23,6 +32,11 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "hyqmom.jl_amd"))
 
+# HBM bytes per step of pass 1 in the mixed+learned mode (k_pass1a + k_mlp_fused + k_pass1b),
+# measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE in separate runs, each scaled by
+# what the counter reads for a 1 GiB device copy (bench.py --copy-calibration 1): 1.120 +
+# 0.313 + 2.778 GB on the first steps' states
+MEASURED_PASS1_TRAFFIC = {"mixed_learned": 4.21e9}
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 GRID = (120, 120, 60)
@@ -67,10 +81,31 @@ def jets_params(N, precision, model=None):
     return p
 
 
+def perturbed_state(M0, seed):
+    """The initial state (seed 0) or the seeded one-ulp perturbation of the density of 100
+    cells (seed > 0)."""
+    import numpy as np
+    if seed == 0:
+        return M0
+    M = np.array(M0, order="F")
+    rng = np.random.default_rng(seed)
+    idx = tuple(rng.integers(0, n, 100) for n in M.shape[:3])
+    M[idx + (0,)] = np.nextafter(M[idx + (0,)], np.inf)
+    return M
+
+
+class Breakdown(Exception):
+    def __init__(self, step):
+        super().__init__("non-finite at step %d" % step)
+        self.step = step
+
+
 def timed_run(N, precision, M0, steps, warmup, model=None):
-    """Wall time and per-kernel event times of `steps` steps from M0."""
+    """Wall time and per-kernel event times of `steps` steps from M0; Breakdown if the
+    trajectory goes non-finite."""
     import torch
     import hyqmom_amd as H
+    from hyqmom_amd._lib import HyqError
     s = H.Solver(jets_params(N, precision, model))
     s.set_state(M0)
     s.run(1e9, warmup)
@@ -79,18 +114,41 @@ def timed_run(N, precision, M0, steps, warmup, model=None):
     s.reset_kernel_times()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    s.run(1e9, steps)
+    try:
+        s.run(1e9, steps)
+    except HyqError:
+        nn = s.nn
+        s.close()
+        raise Breakdown(nn)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     p1, up, p2, nst = s.kernel_times()
     assert nst == steps and s.nn == steps, (nst, s.nn)
     attached = s._mlp is not None
     fl = s.flags()  # pass-1 flags of the last step
+    d1, d2 = s.deferred_cells()
     s.close()
     from hyqmom_amd._lib import newton_iterations
     return {"seconds": el, "pass1_ms": p1, "update_ms": up, "pass2_ms": p2, "network": attached,
             "last_step": {"fallback_cells_frac": float(((fl >> 12) & 1).mean()),
-                          "iterations_per_cell": float(newton_iterations(fl).mean())}}
+                          "iterations_per_cell": float(newton_iterations(fl).mean()),
+                          "deferred_cells_frac": [d1 / fl.size, d2 / fl.size]}}
+
+
+def surviving_run(N, precision, M0, steps, warmup, model=None, tries=24):
+    """timed_run from the initial state, else from the first seeded perturbation whose
+    trajectory completes the steps.  Returns (run, start state, record of the attempts)."""
+    failed = []
+    for seed in range(tries):
+        M = perturbed_state(M0, seed)
+        try:
+            run = timed_run(N, precision, M, steps, warmup, model)
+        except Breakdown as b:
+            failed.append({"seed": seed, "breakdown_step": b.step})
+            continue
+        return run, M, {"seed": seed, "failed_attempts": failed}
+    raise RuntimeError("%s: no trajectory completed %d steps in %d attempts: %s"
+                       % (precision, steps, tries, failed))
 
 
 def replay_states(N, precision, M0, steps, nsample, model=None):
@@ -220,19 +278,20 @@ def main():
         del x, y
     M0 = np.asfortranarray(H.initial_state(jets_params(N3, "fp64")))
 
-    run = timed_run(N3, a.precision, M0, a.steps, a.warmup, a.model)
+    run, Mrun, start = surviving_run(N3, a.precision, M0, a.steps, a.warmup, a.model)
     value = N * a.steps / run["seconds"]
     flops = None
     states = None
     if a.count_flops or a.accuracy:
-        states = replay_states(N3, "fp64" if a.precision == "fp64" else a.precision, M0, a.steps, 6, a.model)
+        states = replay_states(N3, a.precision, Mrun, a.steps, 6, a.model)
     if a.count_flops:
         flops = count_flops(states, a.precision)
     ks = kernel_report(run, flops, a.precision, N)
     dom = max(ks, key=lambda k: ks[k]["ms"])
     d = ks[dom]
+    traffic = MEASURED_PASS1_TRAFFIC.get(a.precision) if dom == "pass1" and N3 == GRID else None
     roof = {"kernel": dom, "bound": d["bound"], "achieved": d.get("achieved"), "peak": d["peak"],
-            "unit": d["unit"], "frac": d.get("frac"), "traffic": None}
+            "unit": d["unit"], "frac": d.get("frac"), "traffic": traffic}
     out = {
         "metric": "cell-steps/s at 120x120x60 (crossing jets, Ma=1.5, Kn=0.01, 200 steps); "
                   "mixed+learned vs plain FP64",
@@ -241,6 +300,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if a.precision == "fp64" else "f64+f32" + ("+bf16" if a.precision == "mixed_learned" else ""),
         "data": "synthetic crossing-jets initial condition",
+        "initial_state": start,
         "config": {"workload": "crossing jets %dx%dx%d, Ma=1.5, Kn=0.01, CFL=0.7, %d steps, %s"
                                % (N3 + (a.steps, a.precision)),
                    "cells": N, "precision": a.precision,
@@ -251,24 +311,29 @@ def main():
     }
     # the FP64 trajectory, sampled: FLOPs of the FP64 kernels, the accuracy check's and the
     # CPU baseline's starting state (mid-run, when the jets have crossed)
-    fstates = states if a.precision == "fp64" else None
+    fstates, fstart = (states, start) if a.precision == "fp64" else (None, None)
+    f = None
+    if a.compare_fp64 and a.precision != "fp64":
+        f, Mf, fstart = surviving_run(N3, "fp64", M0, a.steps, a.warmup)
     if fstates is None and ((a.compare_fp64 and a.count_flops) or a.accuracy or a.cpu_baseline):
-        fstates = replay_states(N3, "fp64", M0, a.steps, 6)
+        if fstart is None:
+            _, Mf, fstart = surviving_run(N3, "fp64", M0, a.steps, 0)
+        fstates = replay_states(N3, "fp64", Mf, a.steps, 6)
     mid = sorted(fstates)[len(fstates) // 2] if fstates else 0
     Mmid = fstates[mid] if fstates else M0
-    if a.compare_fp64 and a.precision != "fp64":
-        f = timed_run(N3, "fp64", M0, a.steps, a.warmup)
+    if f is not None:
         fv = N * a.steps / f["seconds"]
         fflops = count_flops(fstates, "fp64") if a.count_flops else None
         out["fp64"] = {"value": fv, "ms_per_step": f["seconds"] * 1e3 / a.steps,
-                       "kernels": kernel_report(f, fflops, "fp64", N)}
+                       "kernels": kernel_report(f, fflops, "fp64", N), "initial_state": fstart}
         out["speedup_vs_fp64"] = value / fv
     if a.compare_newton and a.precision == "mixed_learned":
-        w = timed_run(N3, "mixed_newton", M0, a.steps, a.warmup)
+        w, _, wstart = surviving_run(N3, "mixed_newton", M0, a.steps, a.warmup)
         wv = N * a.steps / w["seconds"]
         out["mixed_newton"] = {"value": wv, "ms_per_step": w["seconds"] * 1e3 / a.steps,
                                "kernels": {k: w[k] for k in ("pass1_ms", "update_ms", "pass2_ms")},
-                               "fast_wave_speeds_last_step": w["last_step"]}
+                               "fast_wave_speeds_last_step": w["last_step"],
+                               "initial_state": wstart}
         if "fp64" in out:
             out["mixed_newton"]["speedup_vs_fp64"] = wv / out["fp64"]["value"]
     if a.accuracy and a.precision != "fp64":

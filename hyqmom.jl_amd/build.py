@@ -14,7 +14,11 @@ CXX = os.environ.get("CXX", "g++")
 FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC",
          # no packed FP32 instructions: some of their forms have produced wrong results on
          # this hardware without any error
-         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         # multiply-add fusion within one source expression only: the same cell code then
+         # rounds the same way in every kernel it is inlined into (single run, deferred
+         # cells, ensemble), which keeps those paths bitwise identical
+         "-ffp-contract=on"]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-Wno-unknown-pragmas"]
 SOURCES = ["hyq_kernels.hip", "hyq_ensemble.hip", "hyq_ens_fp64.hip", "hyq_ens_mixed.hip",
            "hyq_ens_newton.hip", "hyq_mlp.hip"]

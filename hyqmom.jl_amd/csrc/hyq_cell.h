@@ -42,13 +42,40 @@ HD void s28_to_packed(const S28& s, double* c) {
   c[tpos(0, 2, 2)] = s.S022; c[tpos(0, 1, 3)] = s.S013; c[tpos(0, 0, 4)] = s.S004;
 }
 
-// Flux_closure35_and_realizable_3D (src/numerics/Flux_closure35_and_realizable_3D.jl:33-234).
-// With FLUX, Fx/Fy/Fz (35 each) are written through the pointers with stride `st`.
-template <bool FLUX, typename FT = double>
-HD void flux_closure35(const double* M4, int flag2D, double Ma, double* Mr, FT* Fx, FT* Fy,
-                       FT* Fz, size_t st, uint32_t& fl) {
+// The closure's state between its two halves: the realized standardized 4th-order set, the
+// floored variances and the density and means (35 values, storable as a moment vector).
+struct ClosureState {
+  S28 s;
+  double C200, C020, C002, M000, um, vm, wm;
+};
+HD void closure_state_store(const ClosureState& q, double* v) {
+  v[0] = q.s.S300; v[1] = q.s.S400; v[2] = q.s.S110; v[3] = q.s.S210;
+  v[4] = q.s.S310; v[5] = q.s.S120; v[6] = q.s.S220; v[7] = q.s.S030;
+  v[8] = q.s.S130; v[9] = q.s.S040; v[10] = q.s.S101; v[11] = q.s.S201;
+  v[12] = q.s.S301; v[13] = q.s.S102; v[14] = q.s.S202; v[15] = q.s.S003;
+  v[16] = q.s.S103; v[17] = q.s.S004; v[18] = q.s.S011; v[19] = q.s.S111;
+  v[20] = q.s.S211; v[21] = q.s.S021; v[22] = q.s.S121; v[23] = q.s.S031;
+  v[24] = q.s.S012; v[25] = q.s.S112; v[26] = q.s.S013; v[27] = q.s.S022;
+  v[28] = q.C200; v[29] = q.C020; v[30] = q.C002;
+  v[31] = q.M000; v[32] = q.um; v[33] = q.vm; v[34] = q.wm;
+}
+HD void closure_state_load(const double* v, ClosureState& q) {
+  q.s.S300 = v[0]; q.s.S400 = v[1]; q.s.S110 = v[2]; q.s.S210 = v[3];
+  q.s.S310 = v[4]; q.s.S120 = v[5]; q.s.S220 = v[6]; q.s.S030 = v[7];
+  q.s.S130 = v[8]; q.s.S040 = v[9]; q.s.S101 = v[10]; q.s.S201 = v[11];
+  q.s.S301 = v[12]; q.s.S102 = v[13]; q.s.S202 = v[14]; q.s.S003 = v[15];
+  q.s.S103 = v[16]; q.s.S004 = v[17]; q.s.S011 = v[18]; q.s.S111 = v[19];
+  q.s.S211 = v[20]; q.s.S021 = v[21]; q.s.S121 = v[22]; q.s.S031 = v[23];
+  q.s.S012 = v[24]; q.s.S112 = v[25]; q.s.S013 = v[26]; q.s.S022 = v[27];
+  q.C200 = v[28]; q.C020 = v[29]; q.C002 = v[30];
+  q.M000 = v[31]; q.um = v[32]; q.vm = v[33]; q.wm = v[34];
+}
+
+// Flux_closure35_and_realizable_3D (src/numerics/Flux_closure35_and_realizable_3D.jl:33-234),
+// first half: standardization and the realizability projection (:33-176).
+HD void closure_project(const double* M4, int flag2D, double Ma, ClosureState& q, uint32_t& fl) {
   const double s3max = 4.0 + fabs(Ma) / 2.0, h2min = 1.0e-8;
-  double c[56];
+  double c[35];
   double M000, um, vm, wm;
   raw_to_central4(M4, c, M000, um, vm, wm);
   const double C200 = jmax(EPS, c[tpos(2, 0, 0)]), C020 = jmax(EPS, c[tpos(0, 2, 0)]),
@@ -107,7 +134,19 @@ HD void flux_closure35(const double* M4, int flag2D, double Ma, double* Mr, FT* 
     s.S111 = 0.0; s.S202 = 1.0; s.S022 = 1.0; s.S112 = s.S110; s.S121 = 0.0; s.S211 = 0.0;
     s.S103 = 0.0; s.S013 = 0.0; s.S301 = 0.0; s.S031 = 0.0;
   }
-  // 5th-order closure, S -> C (S_to_C_batch, moment_array_utils.jl:137-204), C -> M
+  q.s = s;
+  q.C200 = C200; q.C020 = C020; q.C002 = C002;
+  q.M000 = M000; q.um = um; q.vm = vm; q.wm = wm;
+}
+
+// Second half: the 5th-order closure, S -> C (S_to_C_batch, moment_array_utils.jl:137-204),
+// C -> M.  With FLUX, Fx/Fy/Fz (35 each) are written through the pointers with stride `st`.
+template <bool FLUX, typename FT = double>
+HD void closure_emit(const ClosureState& q, double* Mr, FT* Fx, FT* Fy, FT* Fz, size_t st) {
+  const S28& s = q.s;
+  const double C200 = q.C200, C020 = q.C020, C002 = q.C002;
+  double M000 = q.M000, um = q.um, vm = q.vm, wm = q.wm;
+  double c[56];
   s28_to_packed(s, c);
   c[tpos(5, 0, 0)] = f500(s.S300, s.S400);
   c[tpos(0, 5, 0)] = f500(s.S030, s.S040);
@@ -160,6 +199,14 @@ HD void flux_closure35(const double* M4, int flag2D, double Ma, double* Mr, FT* 
 #pragma unroll
     for (int n = 0; n < 35; ++n) Mr[n] = c[T35(n)];
   }
+}
+
+template <bool FLUX, typename FT = double>
+HD void flux_closure35(const double* M4, int flag2D, double Ma, double* Mr, FT* Fx, FT* Fy,
+                       FT* Fz, size_t st, uint32_t& fl) {
+  ClosureState q;
+  closure_project(M4, flag2D, Ma, q, fl);
+  closure_emit<FLUX, FT>(q, Mr, Fx, Fy, Fz, st);
 }
 
 }  // namespace hyq

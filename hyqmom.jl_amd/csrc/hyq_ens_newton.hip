@@ -1,4 +1,4 @@
-// Ensemble cell kernels, mixed-precision Newton instantiation (see hyq_ensemble_kernels.h).
+// Ensemble cell kernels, mixed_newton instantiation (see hyq_ensemble_kernels.h).
 #include "hyq_ensemble_kernels.h"
 
 namespace hyq_ens {

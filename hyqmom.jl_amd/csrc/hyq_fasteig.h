@@ -1,4 +1,4 @@
-// Fast wave speeds for the learned / Newton modes: the characteristic polynomial of the
+// Fast wave speeds for the mixed_newton / mixed_learned modes: the characteristic polynomial of the
 // standardized flux Jacobian, a Sturm-sequence hyperbolicity test, and Laguerre refinement
 // of the extreme eigenvalues from outside the root interval (analytic bound or a verified
 // warm start).  Cells where the test is not conclusive, or where the iteration does not

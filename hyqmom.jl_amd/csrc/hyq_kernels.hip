@@ -60,6 +60,16 @@ constexpr int kBlock = 128;
 #define HYQ_FAST_WAVES 2
 #endif
 constexpr int kPassWaves = HYQ_PASS_WAVES, kFastWaves = HYQ_FAST_WAVES;
+// The fast modes' pass 1 and pass 2 as two kernels each (wave speeds | closure and fluxes;
+// checks | BGK) instead of one: less live state per kernel, fewer spills.
+#ifndef HYQ_SPLIT1
+#define HYQ_SPLIT1 0
+#endif
+#ifndef HYQ_SPLIT2
+#define HYQ_SPLIT2 1
+#endif
+constexpr bool kSplit1 = HYQ_SPLIT1 != 0, kSplit2 = HYQ_SPLIT2 != 0;
+constexpr bool kSplitClosure = HYQ_SPLIT1 == 2;  // the closure's two halves in two kernels
 
 constexpr int kNev = 5;  // timing events per step
 // dynamic LDS of the kernels that run the QR eigen solve (hyq_eig.h: qr_mat)
@@ -72,6 +82,7 @@ struct DevScalars {  // device-resident step scalars
   unsigned long long vmax_bits;
   long long nn, nnmax;
   unsigned long long err;  // bit 0: a non-finite z Jacobian (where the reference throws)
+  unsigned int nhard1, nhard2;  // cells deferred to the full pass 1 / pass 2 this step
 };
 
 // ---- pass 1 -------------------------------------------------------------------------
@@ -123,6 +134,8 @@ __global__ void k_step_begin(DevScalars* sc) {
   sc->active = act ? 1.0 : 0.0;
   if (act) sc->nn += 1;
   sc->vmax_bits = 0ull;
+  sc->nhard1 = 0u;
+  sc->nhard2 = 0u;
 }
 __global__ void k_dt(DevScalars* sc, double CFL, double hmin, double dtmax, double Kn) {
   if (sc->active == 0.0) return;
@@ -315,10 +328,11 @@ __global__ __launch_bounds__(kBlock) void k_plane_dataset(const double* __restri
     }
   }
 }
-// Pass 1, first half: projection (state written to A) and the plane features.
+// Pass 1, first half: projection (state written to A) and, with FEAT, the plane features.
+template <bool FEAT>
 __global__ __launch_bounds__(kBlock, kPassWaves) void k_pass1a(const double* __restrict__ M, double* __restrict__ A,
-                                                   float* __restrict__ X, DevScalars* sc, long long N,
-                                                   int flag2D, double Ma) {
+                                                   float* __restrict__ X, uint32_t* __restrict__ flags,
+                                                   DevScalars* sc, long long N, int flag2D, double Ma) {
   if (sc->active == 0.0) return;
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
@@ -329,6 +343,8 @@ __global__ __launch_bounds__(kBlock, kPassWaves) void k_pass1a(const double* __r
   pass1a_cell(m, flag2D, Ma, a, fl);
 #pragma unroll
   for (int t = 0; t < 35; ++t) A[(size_t)t * N + c] = a[t];
+  flags[c] = fl;
+  if (!FEAT) return;
   Plane pl[5];
   cell_planes(a, pl);
   for (int q = 0; q < 5; ++q) {
@@ -339,25 +355,234 @@ __global__ __launch_bounds__(kBlock, kPassWaves) void k_pass1a(const double* __r
     for (int t = 0; t < 4; ++t) dst[t] = make_float4(x[4 * t], x[4 * t + 1], x[4 * t + 2], x[4 * t + 3]);
   }
 }
-// Pass 1, second half, with the network's predictions P [5N][2] as Newton warm starts.
-__global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1b(double* __restrict__ AMr, float* __restrict__ F,
-                                                   float* __restrict__ vp, uint32_t* __restrict__ flags,
-                                                   const float* __restrict__ P, DevScalars* sc,
-                                                   long long N, int flag2D, double Ma, int positivity) {
+
+// Spread of a deferred-cell list of n entries over every wave of the grid: each wave takes
+// `per` consecutive entries (per = ceil(n / waves), at most 64) on its first `per` lanes, and
+// the grid strides by waves * per.  Few cells per wave keep the union of their branch paths,
+// which a wave executes, short; the deferred cells are the rare, branch-heavy ones.
+struct ListSpread {
+  unsigned n, per, wave, lane, stride;
+  __device__ explicit ListSpread(unsigned n_) : n(n_) {
+    const unsigned waves = gridDim.x * (blockDim.x / 64);
+    per = (n + waves - 1) / waves;
+    per = per < 1 ? 1 : (per > 64 ? 64 : per);
+    wave = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    lane = threadIdx.x & 63;
+    stride = waves * per;
+  }
+  __device__ bool idle() const { return wave * per >= n; }  // the same for every lane of a wave
+  __device__ unsigned first() const { return wave * per + lane; }
+  __device__ bool mine(unsigned i) const { return lane < per && i < n; }
+};
+// Grid of the list kernels: two waves per SIMD on every CU (their occupancy), 128-thread blocks.
+constexpr unsigned kListBlocks = 256 * 4 * kPassWaves / 2;
+
+// ---- the fast step with deferred cells (mixed_newton / mixed_learned modes) ----------------
+// Pass 1 of the cells that take no rare branch; the others keep their projected state in Mr
+// and go to the list for k_pass1_full.  PROJECT: the projection runs here (mixed_newton);
+// otherwise k_pass1a already left it in Mr and the network's seeds are in P (learned).
+template <bool PROJECT>
+__global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_fast(
+    const double* __restrict__ M, double* __restrict__ AMr, float* __restrict__ F, float* __restrict__ vp,
+    uint32_t* __restrict__ flags, const float* __restrict__ P, int* __restrict__ list, DevScalars* sc,
+    long long N, int flag2D, double Ma, int positivity) {
   if (sc->active == 0.0) return;
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long vb = 0;
   if (c < N) {
-    double a[35], mr[35], v[6];
+    double a[35];
+    uint32_t fl = 0;
+    if (PROJECT) {
+      double m[35];
 #pragma unroll
-    for (int t = 0; t < 35; ++t) a[t] = AMr[(size_t)t * N + c];
-    Warm w;
+      for (int t = 0; t < 35; ++t) m[t] = M[(size_t)t * N + c];
+      pass1a_cell(m, flag2D, Ma, a, fl);
+    } else {
+#pragma unroll
+      for (int t = 0; t < 35; ++t) a[t] = AMr[(size_t)t * N + c];
+      fl = flags[c];
+    }
+    Warm w = no_warm();
+    if (!PROJECT) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        w.w[2 * q] = P[((size_t)q * N + c) * 2];
+        w.w[2 * q + 1] = P[((size_t)q * N + c) * 2 + 1];
+      }
+    }
+    double v6[6];
+    if (fast_cell_speeds(a, positivity != 0, true, w, v6, fl)) {
+      double mr[35], v[6];
+      pass1_finish<float>(a, flag2D, Ma, mr, F + c, F + (size_t)35 * N + c, F + (size_t)70 * N + c,
+                          (size_t)N, v6, v, fl);
+#pragma unroll
+      for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = mr[t];
+#pragma unroll
+      for (int d = 0; d < 6; ++d) {
+        vp[(size_t)d * N + c] = (float)v[d];
+        const unsigned long long b = __double_as_longlong(fabs(v[d]));
+        vb = b > vb ? b : vb;
+      }
+      flags[c] = fl;
+    } else {
+      if (PROJECT) {
+#pragma unroll
+        for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = a[t];
+      }
+      flags[c] = fl & 0xFFFFu;  // the full path counts its own iterations
+      list[atomicAdd(&sc->nhard1, 1u)] = (int)c;
+    }
+  }
+  vb = wave_max_u64(vb);
+  if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
+}
+
+// Split form of k_pass1_fast: the wave speeds of the 6x6 Jacobians in one kernel
+// (V6 [6][N], FP64; V6[0] = NaN marks a deferred cell), closure and fluxes in the next.
+// Each half holds fewer live values than the fused kernel, so it spills less.
+template <bool LEARNED>
+__global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_speeds(
+    const double* __restrict__ A, uint32_t* __restrict__ flags, const float* __restrict__ P,
+    double* __restrict__ V6, int* __restrict__ list, DevScalars* sc, long long N, int positivity) {
+  if (sc->active == 0.0) return;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  double a[35];
+#pragma unroll
+  for (int t = 0; t < 35; ++t) a[t] = A[(size_t)t * N + c];
+  Warm w = no_warm();
+  if (LEARNED) {
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
       w.w[2 * q] = P[((size_t)q * N + c) * 2];
       w.w[2 * q + 1] = P[((size_t)q * N + c) * 2 + 1];
     }
-    uint32_t fl = 0;
+  }
+  uint32_t fl = flags[c];
+  double v6[6];
+  if (fast_cell_speeds(a, positivity != 0, true, w, v6, fl)) {
+#pragma unroll
+    for (int d = 0; d < 6; ++d) V6[(size_t)d * N + c] = v6[d];
+    flags[c] = fl;
+  } else {
+    V6[c] = NAN;
+    flags[c] = fl & 0xFFFFu;  // the full path counts its own iterations
+    list[atomicAdd(&sc->nhard1, 1u)] = (int)c;
+  }
+}
+
+__global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_finish(
+    double* __restrict__ AMr, float* __restrict__ F, float* __restrict__ vp,
+    uint32_t* __restrict__ flags, const double* __restrict__ V6, DevScalars* sc, long long N,
+    int flag2D, double Ma) {
+  if (sc->active == 0.0) return;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long vb = 0;
+  const double v0 = c < N ? V6[c] : NAN;
+  if (v0 == v0) {
+    double a[35], mr[35], v6[6], v[6];
+#pragma unroll
+    for (int t = 0; t < 35; ++t) a[t] = AMr[(size_t)t * N + c];
+    v6[0] = v0;
+#pragma unroll
+    for (int d = 1; d < 6; ++d) v6[d] = V6[(size_t)d * N + c];
+    uint32_t fl = flags[c];
+    pass1_finish<float>(a, flag2D, Ma, mr, F + c, F + (size_t)35 * N + c, F + (size_t)70 * N + c,
+                        (size_t)N, v6, v, fl);
+#pragma unroll
+    for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = mr[t];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+      vp[(size_t)d * N + c] = (float)v[d];
+      const unsigned long long b = __double_as_longlong(fabs(v[d]));
+      vb = b > vb ? b : vb;
+    }
+    flags[c] = fl;
+  }
+  vb = wave_max_u64(vb);
+  if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
+}
+
+// The finishing half of the split pass 1 in two kernels (HYQ_SPLIT1 = 2): the closure's
+// projection, whose state (ClosureState, 35 values) overwrites A in Mr, then the 5th-order
+// closure, fluxes and the combined wave speeds.  Deferred cells (V6[0] = NaN) are skipped.
+__global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_project(
+    double* __restrict__ AMr, uint32_t* __restrict__ flags, const double* __restrict__ V6,
+    DevScalars* sc, long long N, int flag2D, double Ma) {
+  if (sc->active == 0.0) return;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const double v0 = V6[c];
+  if (v0 != v0) return;
+  double a[35], v[35];
+#pragma unroll
+  for (int t = 0; t < 35; ++t) a[t] = AMr[(size_t)t * N + c];
+  uint32_t fl = flags[c];
+  ClosureState q;
+  closure_project(a, flag2D, Ma, q, fl);
+  closure_state_store(q, v);
+#pragma unroll
+  for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = v[t];
+  flags[c] = fl;
+}
+
+__global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_emit(
+    double* __restrict__ AMr, float* __restrict__ F, float* __restrict__ vp,
+    const double* __restrict__ V6, DevScalars* sc, long long N) {
+  if (sc->active == 0.0) return;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long vb = 0;
+  const double v0 = c < N ? V6[c] : NAN;
+  if (v0 == v0) {
+    double w[35], mr[35], v6[6], v[6];
+#pragma unroll
+    for (int t = 0; t < 35; ++t) w[t] = AMr[(size_t)t * N + c];
+    ClosureState q;
+    closure_state_load(w, q);
+    closure_emit<true, float>(q, mr, F + c, F + (size_t)35 * N + c, F + (size_t)70 * N + c,
+                              (size_t)N);
+    v6[0] = v0;
+#pragma unroll
+    for (int d = 1; d < 6; ++d) v6[d] = V6[(size_t)d * N + c];
+    combine_speeds(mr, v6, v);
+#pragma unroll
+    for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = mr[t];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+      vp[(size_t)d * N + c] = (float)v[d];
+      const unsigned long long b = __double_as_longlong(fabs(v[d]));
+      vb = b > vb ? b : vb;
+    }
+  }
+  vb = wave_max_u64(vb);
+  if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
+}
+
+// The full pass 1 (QR fallbacks, corrections, z clamp / re-projection) from the projected
+// state in Mr, for the listed cells.  P: the network's seeds (learned mode) or null.
+__global__ __launch_bounds__(kBlock, kPassWaves) void k_pass1_full(
+    double* __restrict__ AMr, float* __restrict__ F, float* __restrict__ vp, uint32_t* __restrict__ flags,
+    const float* __restrict__ P, const int* __restrict__ list, DevScalars* sc, long long N, int flag2D,
+    double Ma, int positivity) {
+  if (sc->active == 0.0) return;
+  const ListSpread ls(sc->nhard1);
+  if (ls.idle()) return;  // wave-uniform
+  unsigned long long vb = 0;
+  for (unsigned i = ls.first(); i < ls.n; i += ls.stride) {
+    if (!ls.mine(i)) continue;
+    const long long c = list[i];
+    double a[35], mr[35], v[6];
+#pragma unroll
+    for (int t = 0; t < 35; ++t) a[t] = AMr[(size_t)t * N + c];
+    Warm w = no_warm();
+    if (P != nullptr) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        w.w[2 * q] = P[((size_t)q * N + c) * 2];
+        w.w[2 * q + 1] = P[((size_t)q * N + c) * 2 + 1];
+      }
+    }
+    uint32_t fl = flags[c];
     pass1b_cell<float, 1>(a, flag2D, Ma, positivity != 0, mr, F + c, F + (size_t)35 * N + c,
                           F + (size_t)70 * N + c, (size_t)N, v, fl, w);
 #pragma unroll
@@ -373,6 +598,92 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1b(double* __restric
   }
   vb = wave_max_u64(vb);
   if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
+}
+
+// Pass 2 of the cells whose x/y planes need no correction; the others go to the list.
+__global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_fast(double* __restrict__ M,
+                                                                  int* __restrict__ list,
+                                                                  DevScalars* sc, long long N,
+                                                                  int flag2D, double Ma,
+                                                                  int positivity, double Kn) {
+  if (sc->active == 0.0) return;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  double m[35], o[35];
+#pragma unroll
+  for (int n = 0; n < 35; ++n) m[n] = M[(size_t)n * N + c];
+  uint32_t fl = 0;
+  if (pass2_fast<true>(m, flag2D, Ma, positivity != 0, sc->dt, Kn, o, fl)) {
+#pragma unroll
+    for (int n = 0; n < 35; ++n) M[(size_t)n * N + c] = o[n];
+  } else {
+    list[atomicAdd(&sc->nhard2, 1u)] = (int)c;
+  }
+}
+
+// Split form of k_pass2_fast: projection and the x/y hyperbolicity checks (projected state
+// to A; A[0] = NaN marks a deferred cell), then projection and BGK.
+__global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_check(const double* __restrict__ M,
+                                                                   double* __restrict__ A,
+                                                                   int* __restrict__ list,
+                                                                   DevScalars* sc, long long N,
+                                                                   int flag2D, double Ma) {
+  if (sc->active == 0.0) return;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  double m[35], a[35];
+#pragma unroll
+  for (int n = 0; n < 35; ++n) m[n] = M[(size_t)n * N + c];
+  uint32_t fl = 0;
+  if (pass2_check(m, flag2D, Ma, a, fl)) {
+#pragma unroll
+    for (int n = 0; n < 35; ++n) A[(size_t)n * N + c] = a[n];
+  } else {
+    A[c] = NAN;
+    list[atomicAdd(&sc->nhard2, 1u)] = (int)c;
+  }
+}
+
+__global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_relax(double* __restrict__ M,
+                                                                   const double* __restrict__ A,
+                                                                   DevScalars* sc, long long N,
+                                                                   int flag2D, double Ma,
+                                                                   int positivity, double Kn) {
+  if (sc->active == 0.0) return;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const double a0 = A[c];
+  if (a0 != a0) return;
+  double a[35], o[35];
+  a[0] = a0;
+#pragma unroll
+  for (int n = 1; n < 35; ++n) a[n] = A[(size_t)n * N + c];
+  uint32_t fl = 0;
+  pass2_relax<true>(a, flag2D, Ma, positivity != 0, sc->dt, Kn, o, fl);
+#pragma unroll
+  for (int n = 0; n < 35; ++n) M[(size_t)n * N + c] = o[n];
+}
+
+// The full pass 2 for the listed cells.
+__global__ __launch_bounds__(kBlock, kPassWaves) void k_pass2_full(double* __restrict__ M,
+                                                                  const int* __restrict__ list,
+                                                                  DevScalars* sc, long long N,
+                                                                  int flag2D, double Ma,
+                                                                  int positivity, double Kn) {
+  if (sc->active == 0.0) return;
+  const ListSpread ls(sc->nhard2);
+  if (ls.idle()) return;
+  for (unsigned i = ls.first(); i < ls.n; i += ls.stride) {
+    if (!ls.mine(i)) continue;
+    const long long c = list[i];
+    double m[35], o[35];
+#pragma unroll
+    for (int t = 0; t < 35; ++t) m[t] = M[(size_t)t * N + c];
+    uint32_t fl = 0;
+    pass2_cell<true, 1>(m, flag2D, Ma, positivity != 0, sc->dt, Kn, o, fl);
+#pragma unroll
+    for (int t = 0; t < 35; ++t) M[(size_t)t * N + c] = o[t];
+  }
 }
 
 unsigned grid_for(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -398,6 +709,8 @@ struct hyq_ctx {
   hyq_mlp* mlp;  // learned mode: warm-start network (not owned)
   float* X;      // [5N][16] plane features
   float* P;      // [5N][2] predictions
+  int *list1, *list2;  // cells deferred to the full pass 1 / pass 2 (fast modes)
+  double* V6;          // [6][N] 6x6 wave speeds between the split pass-1 kernels
 };
 
 extern "C" {
@@ -436,6 +749,10 @@ int hyq_create(const hyq_params* params, hyq_ctx** out) {
   if (ok && p.precision == HYQ_PREC_MIXED_LEARNED)
     ok = hipMalloc(&c->X, sizeof(float) * 16 * 5 * c->N) == hipSuccess &&
          hipMalloc(&c->P, sizeof(float) * 2 * 5 * c->N) == hipSuccess;
+  if (ok && (params->precision == HYQ_PREC_MIXED_NEWTON || params->precision == HYQ_PREC_MIXED_LEARNED))
+    ok = hipMalloc(&c->list1, sizeof(int) * c->N) == hipSuccess &&
+         hipMalloc(&c->list2, sizeof(int) * c->N) == hipSuccess &&
+         (!kSplit1 || hipMalloc(&c->V6, sizeof(double) * 6 * c->N) == hipSuccess);
   if (!ok) {
     hyq_destroy(c);
     return fail(HYQ_ENOMEM, "device allocation failed");
@@ -456,6 +773,9 @@ int hyq_destroy(hyq_ctx* c) {
   hipFree(c->sc);
   hipFree(c->X);
   hipFree(c->P);
+  hipFree(c->list1);
+  hipFree(c->list2);
+  hipFree(c->V6);
   for (int i = 0; i < kNev; ++i)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -493,18 +813,54 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
     hipLaunchKernelGGL((k_pass1<float, 0>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->Mr,
                        (float*)c->F, (float*)c->vp, c->flags, c->sc, c->N, p.flag2D, p.Ma,
                        p.positivity);
-  } else if (p.precision == HYQ_PREC_MIXED_NEWTON || c->mlp == nullptr) {
-    hipLaunchKernelGGL((k_pass1<float, 1>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->Mr,
-                       (float*)c->F, (float*)c->vp, c->flags, c->sc, c->N, p.flag2D, p.Ma,
-                       p.positivity);
   } else {
-    // learned: projection + features, network warm starts, wave speeds + fluxes
-    hipLaunchKernelGGL(k_pass1a, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr, c->X, c->sc,
-                       c->N, p.flag2D, p.Ma);
-    const int rc = hyq_mlp_infer_masked_async(c->mlp, c->X, 5 * c->N, c->P, c->stream);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_pass1b, dim3(g), dim3(kBlock), kQrLds, c->stream, c->Mr, (float*)c->F,
-                       (float*)c->vp, c->flags, c->P, c->sc, c->N, p.flag2D, p.Ma, p.positivity);
+    // fast modes: the common cells in one kernel, the deferred ones in a second
+    const bool learned = p.precision == HYQ_PREC_MIXED_LEARNED && c->mlp != nullptr;
+    if (kSplit1) {
+      // projection (+ features), network, 6x6 wave speeds, closure and fluxes
+      if (learned) {
+        hipLaunchKernelGGL(k_pass1a<true>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
+                           c->X, c->flags, c->sc, c->N, p.flag2D, p.Ma);
+        const int rc = hyq_mlp_infer_masked_async(c->mlp, c->X, 5 * c->N, c->P, c->stream);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_pass1_speeds<true>, dim3(g), dim3(kBlock), 0, c->stream,
+                           (const double*)c->Mr, c->flags, (const float*)c->P, c->V6, c->list1,
+                           c->sc, c->N, p.positivity);
+      } else {
+        hipLaunchKernelGGL(k_pass1a<false>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
+                           c->X, c->flags, c->sc, c->N, p.flag2D, p.Ma);
+        hipLaunchKernelGGL(k_pass1_speeds<false>, dim3(g), dim3(kBlock), 0, c->stream,
+                           (const double*)c->Mr, c->flags, (const float*)nullptr, c->V6,
+                           c->list1, c->sc, c->N, p.positivity);
+      }
+      if (kSplitClosure) {
+        hipLaunchKernelGGL(k_pass1_project, dim3(g), dim3(kBlock), 0, c->stream, c->Mr, c->flags,
+                           (const double*)c->V6, c->sc, c->N, p.flag2D, p.Ma);
+        hipLaunchKernelGGL(k_pass1_emit, dim3(g), dim3(kBlock), 0, c->stream, c->Mr,
+                           (float*)c->F, (float*)c->vp, (const double*)c->V6, c->sc, c->N);
+      } else {
+        hipLaunchKernelGGL(k_pass1_finish, dim3(g), dim3(kBlock), 0, c->stream, c->Mr,
+                           (float*)c->F, (float*)c->vp, c->flags, (const double*)c->V6, c->sc,
+                           c->N, p.flag2D, p.Ma);
+      }
+    } else if (learned) {
+      // projection + features, then the network's warm starts
+      hipLaunchKernelGGL(k_pass1a<true>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr, c->X,
+                         c->flags, c->sc, c->N, p.flag2D, p.Ma);
+      const int rc = hyq_mlp_infer_masked_async(c->mlp, c->X, 5 * c->N, c->P, c->stream);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_pass1_fast<false>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
+                         (float*)c->F, (float*)c->vp, c->flags, (const float*)c->P, c->list1,
+                         c->sc, c->N, p.flag2D, p.Ma, p.positivity);
+    } else {
+      hipLaunchKernelGGL(k_pass1_fast<true>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
+                         (float*)c->F, (float*)c->vp, c->flags, (const float*)nullptr, c->list1,
+                         c->sc, c->N, p.flag2D, p.Ma, p.positivity);
+    }
+    hipLaunchKernelGGL(k_pass1_full, dim3(kListBlocks), dim3(kBlock), kQrLds, c->stream, c->Mr,
+                       (float*)c->F, (float*)c->vp, c->flags,
+                       learned ? (const float*)c->P : (const float*)nullptr, (const int*)c->list1,
+                       c->sc, c->N, p.flag2D, p.Ma, p.positivity);
   }
   if (ev) hipEventRecord(ev[1], c->stream);
   hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, c->stream, c->sc, p.CFL, hmin, p.dtmax, p.Kn);
@@ -526,8 +882,17 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
     hipLaunchKernelGGL((k_pass2<true, 0>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->sc,
                        c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
   } else {
-    hipLaunchKernelGGL((k_pass2<true, 1>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->sc,
-                       c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
+    if (kSplit2) {
+      hipLaunchKernelGGL(k_pass2_check, dim3(g), dim3(kBlock), 0, c->stream, (const double*)c->M,
+                         c->Mr, c->list2, c->sc, c->N, p.flag2D, p.Ma);
+      hipLaunchKernelGGL(k_pass2_relax, dim3(g), dim3(kBlock), 0, c->stream, c->M,
+                         (const double*)c->Mr, c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
+    } else {
+      hipLaunchKernelGGL(k_pass2_fast, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->list2,
+                         c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
+    }
+    hipLaunchKernelGGL(k_pass2_full, dim3(kListBlocks), dim3(kBlock), kQrLds, c->stream, c->M,
+                       (const int*)c->list2, c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
   }
   if (ev) hipEventRecord(ev[4], c->stream);
   const hipError_t e = hipGetLastError();
@@ -560,7 +925,6 @@ static int accumulate_times(hyq_ctx* c, hipEvent_t* ev) {
 
 int hyq_step(hyq_ctx* c, double* t_inout, double* dt_out, int64_t* nn_inout) {
   if (!c || !t_inout || !nn_inout) return fail(HYQ_EINVAL, "null argument");
-  // one step regardless of tmax (the caller owns the loop condition)
   // one pass of the loop body; a no-op if t >= tmax (the caller owns the loop condition)
   int rc = load_scalars(c, *t_inout, *nn_inout, c->p.tmax, (long long)*nn_inout + 1);
   if (rc) return rc;
@@ -637,6 +1001,16 @@ int hyq_kernel_times(hyq_ctx* c, double* ms_pass1, double* ms_update, double* ms
   if (ms_update) *ms_update = c->ms_update / n;
   if (ms_pass2) *ms_pass2 = c->ms_pass2 / n;
   if (nsteps) *nsteps = c->timed_steps;
+  return HYQ_OK;
+}
+
+int hyq_deferred_cells(hyq_ctx* c, int64_t* pass1, int64_t* pass2) {
+  if (!c) return fail(HYQ_EINVAL, "null argument");
+  DevScalars s;
+  HIPCHK(hipMemcpyAsync(&s, c->sc, sizeof(s), hipMemcpyDeviceToHost, c->stream), "scalars back");
+  HIPCHK(hipStreamSynchronize(c->stream), "sync");
+  if (pass1) *pass1 = (int64_t)s.nhard1;
+  if (pass2) *pass2 = (int64_t)s.nhard2;
   return HYQ_OK;
 }
 

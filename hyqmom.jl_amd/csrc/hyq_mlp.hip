@@ -550,16 +550,20 @@ static int infer_fused(hyq_mlp* m, const float* X, int64_t n, float* P, hipStrea
     m->packed_version = m->version;
   }
   if (m->shape < 0 && n >= 65536) {
-    // pick the faster launch shape on this workload (one timed run of each)
-    float ms[2] = {0.f, 0.f};
-    for (int sh = 0; sh < 2; ++sh) {
-      MCHK(hipEventRecord(m->ev[0], stream), "mlp event");
-      launch_fused(sh, mask, X, (long)n, m->blob, m->packed, P, stream);
-      MCHK(hipEventRecord(m->ev[1], stream), "mlp event");
-      MCHK(hipEventSynchronize(m->ev[1]), "mlp event");
-      MCHK(hipEventElapsedTime(&ms[sh], m->ev[0], m->ev[1]), "mlp event");
-    }
-    m->shape = ms[1] < ms[0] ? 1 : 0;
+    // pick the faster launch shape: every row computed (no mask), best of three runs each,
+    // on this call's own rows (the real launch below overwrites the trial predictions)
+    float best[2] = {1e30f, 1e30f};
+    for (int rep = 0; rep < 3; ++rep)
+      for (int sh = 0; sh < 2; ++sh) {
+        float ms = 0.f;
+        MCHK(hipEventRecord(m->ev[0], stream), "mlp event");
+        launch_fused(sh, false, X, (long)n, m->blob, m->packed, P, stream);
+        MCHK(hipEventRecord(m->ev[1], stream), "mlp event");
+        MCHK(hipEventSynchronize(m->ev[1]), "mlp event");
+        MCHK(hipEventElapsedTime(&ms, m->ev[0], m->ev[1]), "mlp event");
+        best[sh] = ms < best[sh] ? ms : best[sh];
+      }
+    m->shape = best[1] < best[0] ? 1 : 0;
   }
   launch_fused(m->shape < 0 ? 0 : m->shape, mask, X, (long)n, m->blob, m->packed, P, stream);
   MCHK(hipGetLastError(), "mlp infer launch");

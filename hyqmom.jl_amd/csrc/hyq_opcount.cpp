@@ -67,7 +67,7 @@ static void count_range(const double* M, long long n, int flag2D, double Ma, dou
 
 extern "C" {
 // Sum over the n cells of M ([35][n], host) of the FLOPs of pass 1 (closure, wave speeds,
-// fluxes) and of pass 2 (closure, eigen checks, BGK); em = 0 full eigen solve, 1 Newton
+// fluxes) and of pass 2 (closure, eigen checks, BGK); em = 0 full eigen solve, 1 fast (Sturm + Laguerre)
 // wave speeds from the analytic bounds.  Returns 0, or -1 on bad arguments.
 int hyq_opcount(const double* M, long long n, int flag2D, double Ma, double dt, double Kn, int em,
                 double* flops_pass1, double* flops_pass2) {

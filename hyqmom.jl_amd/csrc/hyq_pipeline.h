@@ -48,37 +48,44 @@ HD double plane_gauss_dev(const Plane& p) {
 }
 HD bool plane_near_gaussian(const Plane& p) { return p.ok && plane_gauss_dev(p) < kGaussTol; }
 
-// Eigen solve of one plane in mode EM: 0 = full FP64 solve (QR); 1 = Sturm test + Laguerre
-// from warm starts (wmin/wmax in standardized units, NaN = analytic bounds), falling back
-// to the full solve when hyperbolicity is not proven or the iteration does not converge.
-template <int EM>
-HD EigRes plane_solve(const Plane& p, float wmin, float wmax, bool need_vals, uint32_t& fl) {
-  if (EM == 0) return plane_eigs(p, fl);
+// Eigen solve of one plane in mode EM: 0 = full FP64 solve (QR); 1 = the fast path from
+// warm starts (wmin/wmax in standardized units, NaN = none), falling back to the full solve
+// when hyperbolicity is not proven or the iteration does not converge.
+// One plane by the fast path: Gaussian seed (when no warm start is given and the plane is
+// near-Gaussian), Sturm test, Laguerre extremes.  Returns false when the plane is not
+// proven hyperbolic (or not finite): the caller then needs the full solve.
+HD bool plane_fast(const Plane& p, float wmin, float wmax, bool need_vals, uint32_t& fl,
+                   double& vmin, double& vmax) {
+  if (!p.ok) return false;
   if (wmin != wmin && plane_near_gaussian(p)) {
     // a Gaussian plane's standardized extreme eigenvalues are -sqrt(3), +sqrt(3)
     wmin = -1.7320508075688772f;
     wmax = 1.7320508075688772f;
   }
-  if (p.ok) {
-    double J[36];
-    jacobian_std(p, J);
-    bool fin = true;
+  double J[36];
+  jacobian_std(p, J);
+  bool fin = true;
 #pragma unroll
-    for (int t = 0; t < 36; ++t) fin = fin && (fabs(J[t]) < 1e300);
-    if (fin) {
-      const FastEig fe = fast_eigs_std(J, (double)wmin, (double)wmax, need_vals);
-      // root-polishing iterations, summed over the cell's plane solves, in flag bits 16..27
-      const uint32_t it = (fl >> 16) + (uint32_t)fe.iters;
-      fl = (fl & 0xFFFFu) | ((it < 0xFFFu ? it : 0xFFFu) << 16);
-      if (fe.real) {
-        EigRes r;
-        r.vmin = need_vals ? p.a + p.sig * fe.lmin : 0.0;
-        r.vmax = need_vals ? p.a + p.sig * fe.lmax : 0.0;
-        r.cplx = false;
-        r.ok = true;
-        return r;
-      }
-    }
+  for (int t = 0; t < 36; ++t) fin = fin && (fabs(J[t]) < 1e300);
+  if (!fin) return false;
+  const FastEig fe = fast_eigs_std(J, (double)wmin, (double)wmax, need_vals);
+  // root-polishing iterations, summed over the cell's plane solves, in flag bits 16..27
+  const uint32_t it = (fl >> 16) + (uint32_t)fe.iters;
+  fl = (fl & 0xFFFFu) | ((it < 0xFFFu ? it : 0xFFFu) << 16);
+  if (!fe.real) return false;
+  vmin = need_vals ? p.a + p.sig * fe.lmin : 0.0;
+  vmax = need_vals ? p.a + p.sig * fe.lmax : 0.0;
+  return true;
+}
+
+template <int EM>
+HD EigRes plane_solve(const Plane& p, float wmin, float wmax, bool need_vals, uint32_t& fl) {
+  if (EM == 0) return plane_eigs(p, fl);
+  EigRes r;
+  if (plane_fast(p, wmin, wmax, need_vals, fl, r.vmin, r.vmax)) {
+    r.cplx = false;
+    r.ok = true;
+    return r;
   }
   fl |= F_FALLBACK;
   return plane_eigs(p, fl);
@@ -293,6 +300,104 @@ HD void pass2_cell(const double* M, int flag2D, double Ma, bool positivity, doub
   eig6_axis<1, EM>(B, flag2D, Ma, d0, d1, A, fl, no_warm(), EM == 0);
   flux_closure35<false, double>(A, flag2D, Ma, B, nullptr, nullptr, nullptr, 0, fl);
   collision_cell<MIXED>(B, dt, Kn, positivity, Mo);
+}
+
+// ---- the fast step with deferred cells (mixed_newton / mixed_learned modes) -----------------
+// Most cells of a step are hyperbolic on every plane with no density clamp or
+// re-projection on the z path: for them pass 1 and pass 2 reduce to closures and the five
+// fast plane solves.  The rest (a QR fallback, a complex-eigenvalue correction, the z
+// path's clamp or re-projection) are deferred to a second kernel that runs the full path
+// on a compacted list, so the rare, register-hungry branches neither widen the common
+// kernel nor hold up its waves.
+
+// x/y/z (min, max) of the 6x6 Jacobians of projected state A when the cell takes none of
+// the rare branches (the planes UV, UW, VU, VW, WU; the z direction's second plane is the
+// y direction's VW, as eigenvalues6z_hyperbolic_3D.jl:117-120 maps it).  Returns false
+// when the cell needs the full path.
+HD bool fast_cell_speeds(const double* A, bool positivity, bool need_vals, const Warm& w,
+                         double v6[6], uint32_t& fl) {
+  // z path preconditions (eigenvalues6z_hyperbolic_3D.jl:27-104): no clamp, no re-projection
+  if (positivity && !(A[0] > 0.0)) return false;
+  if (positivity && A[0] > 1e-15) {
+    const double m0 = A[0];
+    const double vx = A[2] / m0 - (A[1] / m0) * (A[1] / m0);
+    const double vy = A[9] / m0 - (A[5] / m0) * (A[5] / m0);
+    const double vz = A[19] / m0 - (A[15] / m0) * (A[15] / m0);
+    if (vx < 0.0 || vy < 0.0 || vz < 0.0) return false;
+  }
+  double c[35], M000, um, vm, wm;
+  raw_to_central4(A, c, M000, um, vm, wm);
+  double uv0, uv1, uw0, uw1, vu0, vu1, vw0, vw1, wu0, wu1;
+  if (!plane_fast(plane<0, 1>(c, um, vm, wm), w.w[0], w.w[1], need_vals, fl, uv0, uv1)) return false;
+  if (!plane_fast(plane<0, 2>(c, um, vm, wm), w.w[2], w.w[3], need_vals, fl, uw0, uw1)) return false;
+  if (!plane_fast(plane<1, 0>(c, um, vm, wm), w.w[4], w.w[5], need_vals, fl, vu0, vu1)) return false;
+  if (!plane_fast(plane<1, 2>(c, um, vm, wm), w.w[6], w.w[7], need_vals, fl, vw0, vw1)) return false;
+  if (need_vals &&
+      !plane_fast(plane<2, 0>(c, um, vm, wm), w.w[8], w.w[9], need_vals, fl, wu0, wu1))
+    return false;
+  if (need_vals) {
+    v6[0] = jmin(uv0, uw0);
+    v6[1] = jmax(uv1, uw1);
+    v6[2] = jmin(vu0, vw0);
+    v6[3] = jmax(vu1, vw1);
+    v6[4] = jmin(wu0, vw0);
+    v6[5] = jmax(wu1, vw1);
+  }
+  return true;
+}
+
+// The rest of pass 1 for a cell that took none of the rare branches: closure and fluxes of
+// A and the 1D speeds, combined with the 6x6 speeds v6.
+HD void combine_speeds(const double* Mr, const double v6[6], double vp[6]);
+template <typename FT>
+HD void pass1_finish(const double* A, int flag2D, double Ma, double* Mr, FT* Fx, FT* Fy, FT* Fz,
+                     size_t st, const double v6[6], double vp[6], uint32_t& fl) {
+  flux_closure35<true, FT>(A, flag2D, Ma, Mr, Fx, Fy, Fz, st, fl);
+  combine_speeds(Mr, v6, vp);
+}
+// The 1D closure speeds of the final state Mr, combined with the 6x6 speeds v6.
+HD void combine_speeds(const double* Mr, const double v6[6], double vp[6]) {
+  double a, b;
+  closure1d_speeds(Mr[0], Mr[1], Mr[2], Mr[3], Mr[4], a, b);
+  vp[0] = jmin(a, v6[0]);
+  vp[1] = jmax(b, v6[1]);
+  closure1d_speeds(Mr[0], Mr[5], Mr[9], Mr[12], Mr[14], a, b);
+  vp[2] = jmin(a, v6[2]);
+  vp[3] = jmax(b, v6[3]);
+  closure1d_speeds(Mr[0], Mr[15], Mr[19], Mr[22], Mr[24], a, b);
+  vp[4] = jmin(a, v6[4]);
+  vp[5] = jmax(b, v6[5]);
+}
+
+// Pass 2 of a cell when its x/y planes need no correction: closure, hyperbolicity check of
+// UV, UW, VU, VW, closure, BGK.  Returns false (Mo untouched) when the cell needs the full
+// pass 2.
+// First half: projection of M to A and the hyperbolicity checks of UV, UW, VU, VW.
+HD bool pass2_check(const double* M, int flag2D, double Ma, double* A, uint32_t& fl) {
+  flux_closure35<false, double>(M, flag2D, Ma, A, nullptr, nullptr, nullptr, 0, fl);
+  double c[35], M000, um, vm, wm, d0, d1;
+  raw_to_central4(A, c, M000, um, vm, wm);
+  const float nan = NAN;
+  if (!plane_fast(plane<0, 1>(c, um, vm, wm), nan, nan, false, fl, d0, d1)) return false;
+  if (!plane_fast(plane<0, 2>(c, um, vm, wm), nan, nan, false, fl, d0, d1)) return false;
+  if (!plane_fast(plane<1, 0>(c, um, vm, wm), nan, nan, false, fl, d0, d1)) return false;
+  return plane_fast(plane<1, 2>(c, um, vm, wm), nan, nan, false, fl, d0, d1);
+}
+// Second half: projection of A and BGK.
+template <bool MIXED>
+HD void pass2_relax(const double* A, int flag2D, double Ma, bool positivity, double dt, double Kn,
+                    double* Mo, uint32_t& fl) {
+  double B[35];
+  flux_closure35<false, double>(A, flag2D, Ma, B, nullptr, nullptr, nullptr, 0, fl);
+  collision_cell<MIXED>(B, dt, Kn, positivity, Mo);
+}
+template <bool MIXED>
+HD bool pass2_fast(const double* M, int flag2D, double Ma, bool positivity, double dt, double Kn,
+                   double* Mo, uint32_t& fl) {
+  double A[35];
+  if (!pass2_check(M, flag2D, Ma, A, fl)) return false;
+  pass2_relax<MIXED>(A, flag2D, Ma, positivity, dt, Kn, Mo, fl);
+  return true;
 }
 
 }  // namespace hyq

@@ -20,7 +20,7 @@ FLAG_NAMES = ["UNIVARIATE", "EDGE_CORNER", "FACES", "FLAG220", "ROOTS2D", "ROOTS
 
 
 def newton_iterations(flags):
-    """Newton iterations per cell (flag bits 16..27) of the fast wave-speed modes."""
+    """Laguerre iterations per cell (flag bits 16..27) of the fast wave-speed modes."""
     return (flags >> 16) & 0xFFF
 
 
@@ -55,6 +55,7 @@ SIGNATURES = {
     "hyq_get_flags": ([VP, ctypes.POINTER(ctypes.c_uint32)], ctypes.c_int),
     "hyq_kernel_times": ([VP, D, D, D, ctypes.POINTER(I64)], ctypes.c_int),
     "hyq_reset_kernel_times": ([VP], ctypes.c_int),
+    "hyq_deferred_cells": ([VP, ctypes.POINTER(I64), ctypes.POINTER(I64)], ctypes.c_int),
     "hyq_closure_batch": ([VP, I64, ctypes.c_double, ctypes.c_int32, VP, VP, VP, VP, VP], ctypes.c_int),
     "hyq_wavespeed_batch": ([VP, I64, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                              VP, VP, VP, VP, VP, VP], ctypes.c_int),

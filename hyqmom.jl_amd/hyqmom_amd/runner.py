@@ -114,7 +114,7 @@ class Solver:
 
     def set_warm_start(self, model):
         """Attach a warmstart.WarmStartMLP (precision "mixed_learned"); None detaches and the
-        Newton solve then starts from the analytic root bounds."""
+        root iteration then starts from the analytic root bounds."""
         _lib.check(_lib.lib().hyq_set_mlp(self.h, model.handle if model is not None else None),
                    "hyq_set_mlp")
         self._mlp = model
@@ -179,6 +179,12 @@ class Solver:
 
     def reset_kernel_times(self):
         _lib.check(_lib.lib().hyq_reset_kernel_times(self.h), "reset times")
+
+    def deferred_cells(self):
+        """(pass 1, pass 2) cells of the last step that the fast modes sent to the full path."""
+        a, b = _lib.I64(), _lib.I64()
+        _lib.check(_lib.lib().hyq_deferred_cells(self.h, ctypes.byref(a), ctypes.byref(b)), "deferred cells")
+        return a.value, b.value
 
 
 def grid(params):

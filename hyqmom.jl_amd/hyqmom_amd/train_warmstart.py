@@ -10,7 +10,7 @@ solve; planes with complex or non-finite eigenvalues are left out (the step neve
 warm start there: the hyperbolicity test sends them to the full solve).
 
 The last production-grid state is held out: the report gives the prediction error on it
-and the Newton iterations / fallback rate of one step from it with the analytic start
+and the Laguerre iterations / fallback rate of one step from it with the analytic start
 ("mixed_newton") and with the network ("mixed_learned").
 """
 import argparse

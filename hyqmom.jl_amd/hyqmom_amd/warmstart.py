@@ -3,9 +3,10 @@ on bf16 MFMA that predicts the standardized extreme eigenvalues of each plane's 
 Jacobian (no counterpart in HyQMOM.jl).  Everything runs in libhyqmom_hip.so; this module
 holds device buffers (torch tensors, plumbing only) and the training loop.
 
-Precision "mixed_learned" uses the predictions as Newton starting points; Newton converges
-to the exact FP64 extreme roots or the cell falls back to the full eigen solve, so the
-network changes the iteration count, never the answer beyond the Newton tolerance.
+Precision "mixed_learned" uses the predictions as starting points of Laguerre's iteration,
+which converges to the FP64 extreme roots (a start is used only when a Sturm count proves it
+lies outside all roots) or the cell falls back to the full eigen solve, so the network
+changes the iteration count, never the answer beyond the iteration's tolerance.
 """
 import ctypes
 import os

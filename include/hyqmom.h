@@ -119,6 +119,11 @@ int hyq_kernel_times(hyq_ctx* ctx, double* ms_pass1, double* ms_update, double* 
                      int64_t* nsteps);
 int hyq_reset_kernel_times(hyq_ctx* ctx);
 
+/* Cells of the last step that the fast modes (mixed_newton, mixed_learned) handed to the
+ * full pass 1 and the full pass 2 (QR fallback, eigenvalue corrections, z-path clamps); 0 in
+ * the other modes.  No reference counterpart (the reference has a single path). */
+int hyq_deferred_cells(hyq_ctx* ctx, int64_t* pass1, int64_t* pass2);
+
 /* Batched per-cell kernels on device SoA planes ([35][n] doubles). */
 
 /* Flux_closure35_and_realizable_3D (src/numerics/Flux_closure35_and_realizable_3D.jl:33-234)
@@ -145,7 +150,7 @@ int hyq_collision_batch(const double* M, int64_t n, double dt, double Kn, int32_
 /* ---- learned warm start (a 15 -> 128 -> 128 -> 128 -> 2 GELU MLP on bf16 MFMA) ----------
  * No counterpart in the reference: the network predicts the extreme eigenvalues of the
  * standardized flux Jacobian of a plane from its 15 standardized moments; the predictions
- * only seed a Newton solve whose result is exact to FP64 or falls back to the full solve.
+ * only seed a Laguerre iteration whose result is exact to FP64 or falls back to the full solve.
  * Parameters travel as one FP32 blob of hyq_mlp_blob_size() floats (weights, biases, input
  * and label normalization). */
 typedef struct hyq_mlp hyq_mlp;
@@ -167,7 +172,7 @@ int hyq_mlp_train_step(hyq_mlp* m, const float* X, const float* T, int64_t n, fl
                        float* loss_out);
 const char* hyq_mlp_last_error(void);
 /* attach a network to a HYQ_PREC_MIXED_LEARNED context (not owned; NULL detaches, which
- * makes the context use analytic Newton starts) */
+ * makes the context use analytic starts) */
 int hyq_set_mlp(hyq_ctx* ctx, hyq_mlp* mlp);
 /* training data: features X [5n][16] of the five plane Jacobians (UV, UW, VU, VW, WU; sample
  * = plane*n + cell) of n states M [35][n], and optionally labels L [5n][2] (standardized

@@ -68,7 +68,7 @@ def closure1d(m5):
 
 
 def plane_compare(M, f, g, wmin=float("nan"), wmax=float("nan")):
-    """QR and fast (Sturm + Newton) solves of plane (f, g): dict of both results."""
+    """QR and fast (Sturm + Laguerre) solves of plane (f, g): dict of both results."""
     M = _f(M); out = np.zeros(9)
     lib().cell_plane_compare(_p(M), f, g, ctypes.c_double(wmin), ctypes.c_double(wmax), _p(out))
     return {"qr": (out[0], out[1]), "qr_cplx": bool(out[2]), "fast": (out[3], out[4]),

@@ -1,4 +1,4 @@
-"""The fast wave-speed path (characteristic polynomial, Sturm test, Newton) against the full
+"""The fast wave-speed path (characteristic polynomial, Sturm test, Laguerre) against the full
 QR solve, on the host build of the device code."""
 import numpy as np
 import pytest

@@ -11,7 +11,8 @@ pytestmark = pytest.mark.gpu
 def _params(tmp_path=None, **kw):
     import hyqmom_amd as H
     bg, jets = H.crossing_jets(1.0)
-    p = dict(Nx=12, Ny=12, Nz=6, tmax=10.0, Kn=1.0, Ma=1.0, flag2D=0, CFL=0.7, Nmom=35, nnmax=5,
+    # the 20x20x10 crossing jets at Ma = 1, Kn = 1 (stable for these few steps)
+    p = dict(Nx=20, Ny=20, Nz=10, tmax=10.0, Kn=1.0, Ma=1.0, flag2D=0, CFL=0.7, Nmom=35, nnmax=5,
              dtmax=1e-2, rhol=1.0, rhor=0.1, T=1.0, r110=0.0, r101=0.0, r011=0.0,
              symmetry_check_interval=0, homogeneous_z=False, debug_output=False,
              use_custom_ic=True, ic_background=bg, ic_jets=jets)
@@ -22,11 +23,11 @@ def _params(tmp_path=None, **kw):
 def test_simulation_runner_contract():
     import hyqmom_amd as H
     M, t, nn, grid = H.simulation_runner(_params())
-    assert M.shape == (12, 12, 6, 35) and M.flags["F_CONTIGUOUS"]
+    assert M.shape == (20, 20, 10, 35) and M.flags["F_CONTIGUOUS"]
     assert nn == 5 and 0 < t < 10.0
     for k in ("x", "y", "z", "dx", "dy", "dz", "xm", "ym", "zm"):
         assert k in grid
-    assert len(grid["x"]) == 13 and len(grid["xm"]) == 12 and grid["dz"] == 1.0 / 6
+    assert len(grid["x"]) == 21 and len(grid["xm"]) == 20 and grid["dz"] == 1.0 / 10
     assert np.isfinite(M).all()
     # the loop condition t < tmax ends the run at tmax exactly (dt = min(..., tmax - t))
     M2, t2, nn2, _ = H.simulation_runner(_params(tmax=0.005, nnmax=100))

@@ -1,5 +1,5 @@
 """The warm-start network on the GPU (bf16 MFMA kernels) against its CPU reference, the
-training dataset against the oracle's eigen solve, and the learned / Newton precision
+training dataset against the oracle's eigen solve, and the mixed_learned / mixed_newton precision
 modes against the plain mixed step."""
 import numpy as np
 import pytest
@@ -142,7 +142,7 @@ def _params(N, Ma, Kn, prec):
 
 def test_newton_and_learned_modes_match_mixed():
     """From one state, mixed_newton and mixed_learned (an untrained network: any warm start
-    is only a Newton seed) give the wave speeds of the full solve to the Newton tolerance
+    is only a Laguerre seed) give the wave speeds of the full solve to the iteration tolerance
     (so dt agrees to 1e-12), hence the same step as the mixed mode up to FP32 rounding."""
     import hyqmom_amd as H
     from hyqmom_amd.warmstart import WarmStartMLP
