@@ -233,6 +233,131 @@ HD bool hqr_core(Mat& H, Sink& sink) {
   return true;
 }
 
+// The same Francis iteration for N = 6 on a row-major array H[36] that stays in registers:
+// every index is a compile-time constant.  One double-shift step on the active window
+// [LO, HI] is instantiated per window (10 windows of size >= 3), the deflation search and
+// the reads at the dynamic bottom index are unrolled over all positions with predicates.
+// The arithmetic, its order and the sink calls are those of hqr_core<6>, so both give the
+// same bits.
+template <int LO, int HI>
+HD void francis_step6(double* H, int its) {
+  auto A = [H](int i, int j) -> double& { return H[i * 6 + j]; };
+  double s, t;
+  if (its % 10 == 0) {
+    const double w = fabs(A(HI, HI - 1)) + fabs(A(HI - 1, HI - 2));
+    s = 1.5 * w;
+    t = w * w;
+  } else {
+    s = A(HI - 1, HI - 1) + A(HI, HI);
+    t = A(HI - 1, HI - 1) * A(HI, HI) - A(HI - 1, HI) * A(HI, HI - 1);
+  }
+  double x = A(LO, LO) * A(LO, LO) + A(LO, LO + 1) * A(LO + 1, LO) - s * A(LO, LO) + t;
+  double y = A(LO + 1, LO) * (A(LO, LO) + A(LO + 1, LO + 1) - s);
+  double z = A(LO + 1, LO) * A(LO + 2, LO + 1);
+#pragma unroll
+  for (int k = LO - 1; k <= HI - 3; ++k) {
+    double v1, v2, beta;
+    house(x, y, z, true, v1, v2, beta);
+    const int q = LO > k ? LO : k;
+#pragma unroll
+    for (int j = q; j <= HI; ++j) {
+      const double d = beta * (A(k + 1, j) + v1 * A(k + 2, j) + v2 * A(k + 3, j));
+      A(k + 1, j) -= d;
+      A(k + 2, j) -= d * v1;
+      A(k + 3, j) -= d * v2;
+    }
+    const int r = k + 4 < HI ? k + 4 : HI;
+#pragma unroll
+    for (int i = LO; i <= r; ++i) {
+      const double d = beta * (A(i, k + 1) + v1 * A(i, k + 2) + v2 * A(i, k + 3));
+      A(i, k + 1) -= d;
+      A(i, k + 2) -= d * v1;
+      A(i, k + 3) -= d * v2;
+    }
+    x = A(k + 2, k + 1);
+    y = A(k + 3, k + 1);
+    if (k < HI - 3) z = A(k + 4, k + 1);
+  }
+  double v1, v2, beta;
+  house(x, y, 0.0, false, v1, v2, beta);
+#pragma unroll
+  for (int j = HI - 2; j <= HI; ++j) {
+    const double d = beta * (A(HI - 1, j) + v1 * A(HI, j));
+    A(HI - 1, j) -= d;
+    A(HI, j) -= d * v1;
+  }
+#pragma unroll
+  for (int i = LO; i <= HI; ++i) {
+    const double d = beta * (A(i, HI - 1) + v1 * A(i, HI));
+    A(i, HI - 1) -= d;
+    A(i, HI) -= d * v1;
+  }
+}
+
+template <typename Sink>
+HD bool hqr6_reg(double* H, Sink& sink) {
+  int hi = 5, its = 0;
+  while (hi >= 0) {
+    int lo = 0;
+    bool found = false;
+#pragma unroll
+    for (int q = 5; q >= 1; --q) {
+      if (!found && q <= hi) {
+        double s = fabs(H[(q - 1) * 7]) + fabs(H[q * 7]);
+        if (s == 0.0) s = 1.0;
+        if (fabs(H[q * 7 - 1]) <= EPS * s) {
+          H[q * 7 - 1] = 0.0;
+          lo = q;
+          found = true;
+        }
+      }
+    }
+    if (lo == hi) {
+      double d = 0.0;
+#pragma unroll
+      for (int q = 0; q < 6; ++q)
+        if (q == hi) d = H[q * 7];
+      sink(hi, d, 0.0);
+      --hi;
+      its = 0;
+      continue;
+    }
+    if (lo == hi - 1) {
+      double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+#pragma unroll
+      for (int q = 1; q < 6; ++q)
+        if (q == hi) {
+          a = H[(q - 1) * 7];
+          b = H[(q - 1) * 7 + 1];
+          c = H[q * 7 - 1];
+          d = H[q * 7];
+        }
+      double r1, i1, r2, i2;
+      eig2x2(a, b, c, d, r1, i1, r2, i2);
+      sink(hi - 1, r1, i1);
+      sink(hi, r2, i2);
+      hi -= 2;
+      its = 0;
+      continue;
+    }
+    if (its >= 100) return false;
+    ++its;
+    switch (lo * 8 + hi) {
+      case 0 * 8 + 2: francis_step6<0, 2>(H, its); break;
+      case 0 * 8 + 3: francis_step6<0, 3>(H, its); break;
+      case 1 * 8 + 3: francis_step6<1, 3>(H, its); break;
+      case 0 * 8 + 4: francis_step6<0, 4>(H, its); break;
+      case 1 * 8 + 4: francis_step6<1, 4>(H, its); break;
+      case 2 * 8 + 4: francis_step6<2, 4>(H, its); break;
+      case 0 * 8 + 5: francis_step6<0, 5>(H, its); break;
+      case 1 * 8 + 5: francis_step6<1, 5>(H, its); break;
+      case 2 * 8 + 5: francis_step6<2, 5>(H, its); break;
+      default: francis_step6<3, 5>(H, its); break;
+    }
+  }
+  return true;
+}
+
 // Row-major N x N array accessor.
 template <int N>
 struct ArrMat {
@@ -262,11 +387,16 @@ HD bool hqr_n(double* H, double* wr, double* wi) {
   return hqr_core<N>(m, s);
 }
 
-// Storage of the 6x6 matrix during the QR iteration, whose data-dependent indices would
-// otherwise put it in per-lane scratch memory: on the device a slot of LDS per lane
-// (element-major, lane-minor: conflict-free), provided by the launch as dynamic shared memory
-// of qr_lds_bytes(block) bytes; on the host a local array.
-HD constexpr unsigned qr_lds_bytes(unsigned block) { return 36u * 8u * block; }
+// Where plane_eigs keeps the 6x6 matrix during the QR iteration: registers (hqr6_reg, the
+// default) or, with HYQ_QR_LDS, the LDS slot below.
+#ifndef HYQ_QR_LDS
+#define HYQ_QR_LDS 0
+#endif
+// The LDS form: hqr_core's data-dependent indices would put the matrix in per-lane scratch
+// memory, so on the device it takes a slot of LDS per lane (element-major, lane-minor:
+// conflict-free), provided by the launch as dynamic shared memory of qr_lds_bytes(block)
+// bytes (0 for the register form); on the host a local array.
+HD constexpr unsigned qr_lds_bytes(unsigned block) { return HYQ_QR_LDS ? 36u * 8u * block : 0u; }
 #if defined(__HIP_DEVICE_COMPILE__)
 struct QrMat {
   double* b;
@@ -290,8 +420,7 @@ struct EigRes {
   bool cplx;          // some |imag| > 1000 eps (raw scale)
   bool ok;
 };
-// Eigenvalues of the plane Jacobian mapped back to the raw scale (the QR matrix in the
-// launch's dynamic LDS, qr_lds_bytes).
+// Eigenvalues of the plane Jacobian mapped back to the raw scale.
 HD EigRes plane_eigs(const Plane& p, uint32_t& fl) {
   EigRes r;
   double H[36];
@@ -307,13 +436,18 @@ HD EigRes plane_eigs(const Plane& p, uint32_t& fl) {
   }
   balance_n<6>(H);
   hessenberg_n<6>(H);
+  ExtremeSink ex{0.0, 0.0, 0.0, true};
+#if HYQ_QR_LDS
   QrMat Q = qr_mat();
 #pragma unroll
   for (int i = 0; i < 6; ++i)
 #pragma unroll
     for (int j = 0; j < 6; ++j) Q(i, j) = H[i * 6 + j];
-  ExtremeSink ex{0.0, 0.0, 0.0, true};
-  if (!hqr_core<6>(Q, ex)) {
+  const bool conv = hqr_core<6>(Q, ex);
+#else
+  const bool conv = hqr6_reg(H, ex);
+#endif
+  if (!conv) {
     fl |= F_NOCONV;
     r.vmin = r.vmax = NAN;
     r.cplx = false;

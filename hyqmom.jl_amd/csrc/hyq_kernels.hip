@@ -70,9 +70,16 @@ constexpr int kPassWaves = HYQ_PASS_WAVES, kFastWaves = HYQ_FAST_WAVES;
 #endif
 constexpr bool kSplit1 = HYQ_SPLIT1 != 0, kSplit2 = HYQ_SPLIT2 != 0;
 constexpr bool kSplitClosure = HYQ_SPLIT1 == 2;  // the closure's two halves in two kernels
+// mixed_newton: 1 = common cells and deferred cells in separate kernels, 0 = one kernel per
+// pass (k_pass1<float, 1>, k_pass2<true, 1>)
+#ifndef HYQ_DEFER
+#define HYQ_DEFER 1
+#endif
+constexpr bool kDefer = HYQ_DEFER != 0;
 
 constexpr int kNev = 5;  // timing events per step
-// dynamic LDS of the kernels that run the QR eigen solve (hyq_eig.h: qr_mat)
+// dynamic LDS of the kernels that run the QR eigen solve (hyq_eig.h: qr_mat; 0 unless the
+// matrix is kept in LDS)
 constexpr unsigned kQrLds = qr_lds_bytes(kBlock);
 constexpr const char* kNonfiniteMsg =
     "non-finite Jacobian in the z wave-speed solve (eigenvalues6z_hyperbolic_3D.jl:108,121 throws)";
@@ -214,6 +221,34 @@ __global__ __launch_bounds__(kBlock) void k_collision_batch(const double* __rest
   collision_cell(m, dt, Kn, positivity != 0, o);
 #pragma unroll
   for (int n = 0; n < 35; ++n) Mo[(size_t)n * N + c] = o[n];
+}
+
+// closure_and_eigenvalues speeds of n 5-moment sets ([5][n]).
+__global__ __launch_bounds__(kBlock) void k_closure1d_batch(const double* __restrict__ m5,
+                                                            long long N, double* vmin,
+                                                            double* vmax) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  double a, b;
+  closure1d_speeds(m5[c], m5[N + c], m5[2 * N + c], m5[3 * N + c], m5[4 * N + c], a, b);
+  vmin[c] = a;
+  vmax[c] = b;
+}
+
+// pas_HLL on L independent lines of np cells (cell s of line l at l*np + s), FP64: the
+// x-direction part of the step's update (hyq_update.h) on its own.
+__global__ __launch_bounds__(kBlock) void k_hll_lines(const double* __restrict__ M,
+                                                      const double* __restrict__ F,
+                                                      const double* __restrict__ vmin,
+                                                      const double* __restrict__ vmax,
+                                                      long long N, int np, double dtdx,
+                                                      double* __restrict__ out) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const int s = (int)(c % np);
+  const UpdDir<double> d = upd_dir(vmin + c, vmax + c, s, np, 1);
+  for (int n = 0; n < 35; ++n)
+    out[(size_t)n * N + c] = upd_value(d, M + (size_t)n * N + c, F + (size_t)n * N + c, dtdx);
 }
 
 // eigvals of jacobian6 for a 15-moment slice (jacobian6.jl argument order), sorted by real
@@ -364,8 +399,12 @@ struct ListSpread {
   unsigned n, per, wave, lane, stride;
   __device__ explicit ListSpread(unsigned n_) : n(n_) {
     const unsigned waves = gridDim.x * (blockDim.x / 64);
+#ifdef HYQ_LIST_PER
+    per = HYQ_LIST_PER;
+#else
     per = (n + waves - 1) / waves;
     per = per < 1 ? 1 : (per > 64 ? 64 : per);
+#endif
     wave = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
     lane = threadIdx.x & 63;
     stride = waves * per;
@@ -374,8 +413,15 @@ struct ListSpread {
   __device__ unsigned first() const { return wave * per + lane; }
   __device__ bool mine(unsigned i) const { return lane < per && i < n; }
 };
-// Grid of the list kernels: two waves per SIMD on every CU (their occupancy), 128-thread blocks.
-constexpr unsigned kListBlocks = 256 * 4 * kPassWaves / 2;
+// Waves per SIMD of the list kernels.  With 1 they keep their state in the 512 registers
+// without spilling, but on the 120x120x60 workload that was 15-20% slower than 2 (their
+// time is one wave's latency through the full path, and two waves per SIMD share the cells).
+// Grid: that many waves per SIMD on every CU, 128-thread blocks.
+#ifndef HYQ_FULL_WAVES
+#define HYQ_FULL_WAVES 2
+#endif
+constexpr int kFullWaves = HYQ_FULL_WAVES;
+constexpr unsigned kListBlocks = 256 * 4 * kFullWaves / 2;
 
 // ---- the fast step with deferred cells (mixed_newton / mixed_learned modes) ----------------
 // Pass 1 of the cells that take no rare branch; the others keep their projected state in Mr
@@ -560,7 +606,7 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_emit(
 
 // The full pass 1 (QR fallbacks, corrections, z clamp / re-projection) from the projected
 // state in Mr, for the listed cells.  P: the network's seeds (learned mode) or null.
-__global__ __launch_bounds__(kBlock, kPassWaves) void k_pass1_full(
+__global__ __launch_bounds__(kBlock, kFullWaves) void k_pass1_full(
     double* __restrict__ AMr, float* __restrict__ F, float* __restrict__ vp, uint32_t* __restrict__ flags,
     const float* __restrict__ P, const int* __restrict__ list, DevScalars* sc, long long N, int flag2D,
     double Ma, int positivity) {
@@ -665,7 +711,7 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_relax(double* __re
 }
 
 // The full pass 2 for the listed cells.
-__global__ __launch_bounds__(kBlock, kPassWaves) void k_pass2_full(double* __restrict__ M,
+__global__ __launch_bounds__(kBlock, kFullWaves) void k_pass2_full(double* __restrict__ M,
                                                                   const int* __restrict__ list,
                                                                   DevScalars* sc, long long N,
                                                                   int flag2D, double Ma,
@@ -813,6 +859,10 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
     hipLaunchKernelGGL((k_pass1<float, 0>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->Mr,
                        (float*)c->F, (float*)c->vp, c->flags, c->sc, c->N, p.flag2D, p.Ma,
                        p.positivity);
+  } else if (!kDefer && !(p.precision == HYQ_PREC_MIXED_LEARNED && c->mlp != nullptr)) {
+    hipLaunchKernelGGL((k_pass1<float, 1>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->Mr,
+                       (float*)c->F, (float*)c->vp, c->flags, c->sc, c->N, p.flag2D, p.Ma,
+                       p.positivity);
   } else {
     // fast modes: the common cells in one kernel, the deferred ones in a second
     const bool learned = p.precision == HYQ_PREC_MIXED_LEARNED && c->mlp != nullptr;
@@ -880,6 +930,9 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
                        c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
   } else if (p.precision == HYQ_PREC_MIXED) {
     hipLaunchKernelGGL((k_pass2<true, 0>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->sc,
+                       c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
+  } else if (!kDefer) {
+    hipLaunchKernelGGL((k_pass2<true, 1>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->sc,
                        c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
   } else {
     if (kSplit2) {
@@ -1074,6 +1127,29 @@ int hyq_plane_dataset(const double* M, int64_t n, float* X, float* L, uint8_t* v
                      valid);
   HIPCHK(hipGetLastError(), "dataset launch");
   HIPCHK(hipDeviceSynchronize(), "dataset sync");
+  return HYQ_OK;
+}
+
+int hyq_closure1d_batch(const double* m5, int64_t n, double* vmin, double* vmax) {
+  if (!m5 || !vmin || !vmax || n < 0) return fail(HYQ_EINVAL, "bad argument");
+  if (n == 0) return HYQ_OK;
+  hipLaunchKernelGGL(k_closure1d_batch, dim3(grid_for(n)), dim3(kBlock), 0, 0, m5, (long long)n,
+                     vmin, vmax);
+  HIPCHK(hipGetLastError(), "closure1d launch");
+  HIPCHK(hipDeviceSynchronize(), "closure1d sync");
+  return HYQ_OK;
+}
+
+int hyq_hll_lines(const double* M, const double* F, const double* vmin, const double* vmax,
+                  int64_t nlines, int32_t np, double dt, double dx, double* Mout) {
+  if (!M || !F || !vmin || !vmax || !Mout || nlines < 0 || np < 1 || !(dx > 0))
+    return fail(HYQ_EINVAL, "bad argument");
+  const long long n = (long long)nlines * np;
+  if (n == 0) return HYQ_OK;
+  hipLaunchKernelGGL(k_hll_lines, dim3(grid_for(n)), dim3(kBlock), 0, 0, M, F, vmin, vmax, n,
+                     (int)np, dt / dx, Mout);
+  HIPCHK(hipGetLastError(), "hll launch");
+  HIPCHK(hipDeviceSynchronize(), "hll sync");
   return HYQ_OK;
 }
 

@@ -61,6 +61,8 @@ SIGNATURES = {
                              VP, VP, VP, VP, VP, VP], ctypes.c_int),
     "hyq_eig6_batch": ([VP, I64, VP, VP], ctypes.c_int),
     "hyq_collision_batch": ([VP, I64, ctypes.c_double, ctypes.c_double, ctypes.c_int32, VP], ctypes.c_int),
+    "hyq_closure1d_batch": ([VP, I64, VP, VP], ctypes.c_int),
+    "hyq_hll_lines": ([VP, VP, VP, VP, I64, ctypes.c_int32, ctypes.c_double, ctypes.c_double, VP], ctypes.c_int),
     "hyq_mlp_blob_size": ([], ctypes.c_int64),
     "hyq_mlp_param_count": ([], ctypes.c_int64),
     "hyq_mlp_create": ([VP, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),

@@ -143,6 +143,16 @@ int hyq_wavespeed_batch(const double* M, int64_t n, double Ma, int32_t flag2D,
  * ([6][n] real, [6][n] imaginary). */
 int hyq_eig6_batch(const double* m15, int64_t n, double* wr, double* wi);
 
+/* closure_and_eigenvalues (src/numerics/closure_and_eigenvalues.jl:17-78): min/max real
+ * eigenvalue of the 1D HyQMOM Jacobi matrix of n 5-moment sets m5 [5][n]. */
+int hyq_closure1d_batch(const double* m5, int64_t n, double* vmin, double* vmax);
+
+/* pas_HLL (src/numerics/pas_HLL.jl:24-84 with flux_HLL.jl:20-32) on nlines independent lines
+ * of np cells: M, F, Mout [35][nlines*np] (cell s of line l at l*np + s), wave-speed bounds
+ * vmin, vmax [nlines*np], Mout = the line's updated state with the reference's BC copies. */
+int hyq_hll_lines(const double* M, const double* F, const double* vmin, const double* vmax,
+                  int64_t nlines, int32_t np, double dt, double dx, double* Mout);
+
 /* collision35 (src/numerics/collision35.jl:17-45) on n cells. */
 int hyq_collision_batch(const double* M, int64_t n, double dt, double Kn, int32_t positivity,
                         double* Mout);

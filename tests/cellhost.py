@@ -75,6 +75,13 @@ def plane_compare(M, f, g, wmin=float("nan"), wmax=float("nan")):
             "fast_real": bool(out[5]), "iters": int(out[6]), "shift": out[7], "scale": out[8]}
 
 
+def jacobian_std(M, f, g):
+    """Standardized 6x6 Jacobian of plane (f, g)."""
+    M = _f(M); J = np.zeros(36)
+    lib().cell_jacobian_std(_p(M), f, g, _p(J))
+    return J.reshape(6, 6)
+
+
 def plane_std_extremes(M, f, g):
     """Standardized QR extremes of plane (f, g)."""
     r = plane_compare(M, f, g)

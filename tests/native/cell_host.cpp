@@ -127,3 +127,35 @@ extern "C" void cell_plane_compare(const double* M, int f, int g, double wmin, d
   out[6] = fe.iters;
   out[7] = p.a; out[8] = p.sig;
 }
+
+// The QR iteration both ways on a 6x6 matrix (balanced and reduced to Hessenberg form first):
+// hqr_core on an array accessor and the register form hqr6_reg.  out: for each, the smallest
+// and largest real part, the largest |imag| and the convergence flag (8 values).
+extern "C" void cell_qr_both(const double* J36, double* out) {
+  double H[36];
+  for (int t = 0; t < 36; ++t) H[t] = J36[t];
+  balance_n<6>(H);
+  hessenberg_n<6>(H);
+  double G[36];
+  for (int t = 0; t < 36; ++t) G[t] = H[t];
+  ArrMat<6> m{H};
+  ExtremeSink a{0.0, 0.0, 0.0, true}, b{0.0, 0.0, 0.0, true};
+  const bool ca = hqr_core<6>(m, a);
+  const bool cb = hqr6_reg(G, b);
+  out[0] = a.mn; out[1] = a.mx; out[2] = a.mi; out[3] = ca ? 1 : 0;
+  out[4] = b.mn; out[5] = b.mx; out[6] = b.mi; out[7] = cb ? 1 : 0;
+}
+
+// The standardized Jacobian of plane (f, g) of a 35-moment state (row-major 6x6).
+extern "C" void cell_jacobian_std(const double* M, int f, int g, double* J) {
+  double c[35], M000, um, vm, wm;
+  raw_to_central4(M, c, M000, um, vm, wm);
+  Plane p;
+  if (f == 0 && g == 1) p = plane<0, 1>(c, um, vm, wm);
+  else if (f == 0 && g == 2) p = plane<0, 2>(c, um, vm, wm);
+  else if (f == 1 && g == 0) p = plane<1, 0>(c, um, vm, wm);
+  else if (f == 1 && g == 2) p = plane<1, 2>(c, um, vm, wm);
+  else p = plane<2, 0>(c, um, vm, wm);
+  jacobian_std(p, J);
+}
+

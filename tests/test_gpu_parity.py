@@ -114,6 +114,72 @@ def test_collision_batch():
     assert scaled_err(to_host(out).T, ref) < RTOL
 
 
+def test_closure1d_batch():
+    """closure_and_eigenvalues speeds on the x, y, z 5-moment slices of mixture cells, the
+    reference's golden [1,0,1,0,3] -> +-sqrt(6), and Gaussians (u +- sqrt(6) sigma)."""
+    from _common import load_json
+    from gpu_util import empty_dev, to_dev, to_host
+    L = _lib()
+    M = cells(128, 5)
+    sets = [M[:, [0, 1, 2, 3, 4]], M[:, [0, 5, 9, 12, 14]], M[:, [0, 15, 19, 22, 24]]]
+    g = load_json("mat_goldens.json")["test_closures_golden"]["closure_1d"]
+    gauss = [[r, r * u, r * (u * u + s * s), r * (u ** 3 + 3 * u * s * s), r * (u ** 4 + 6 * u * u * s * s + 3 * s ** 4)]
+             for r, u, s in [(1.0, 0.0, 1.0), (0.3, 1.2, 0.5), (2.0, -0.7, 1.7)]]
+    m5 = np.vstack(sets + [np.array([g["input"]]), np.array(gauss)])
+    n = len(m5)
+    d = to_dev(np.ascontiguousarray(m5.T))
+    lo, hi = empty_dev((n,)), empty_dev((n,))
+    L.check(L.lib().hyq_closure1d_batch(d.data_ptr(), n, lo.data_ptr(), hi.data_ptr()), "closure1d")
+    glo, ghi = to_host(lo), to_host(hi)
+    ref = np.array([O.closure_and_eigenvalues(m)[1:] for m in m5])
+    scale = np.abs(ref).max()
+    assert np.abs(glo - ref[:, 0]).max() / scale < 1e-12
+    assert np.abs(ghi - ref[:, 1]).max() / scale < 1e-12
+    k = len(sets[0]) * 3
+    assert abs(glo[k] - g["output"]["vpmin"]) < 1e-8 and abs(ghi[k] - g["output"]["vpmax"]) < 1e-8
+    for q, (r, u, sg) in enumerate([(1.0, 0.0, 1.0), (0.3, 1.2, 0.5), (2.0, -0.7, 1.7)]):
+        assert abs(glo[k + 1 + q] - (u - 6 ** 0.5 * sg)) < 1e-12 * (abs(u) + sg)
+        assert abs(ghi[k + 1 + q] - (u + 6 ** 0.5 * sg)) < 1e-12 * (abs(u) + sg)
+
+
+@pytest.mark.parametrize("np_", [1, 2, 3, 5, 20])
+def test_hll_lines(np_):
+    """pas_HLL on independent lines against the oracle, including the one- and two-cell
+    lines and the BC copies at both ends, and the reference's golden (constant state and
+    fluxes with speeds -1, +1 stay unchanged)."""
+    from _common import load_json
+    from gpu_util import empty_dev, to_dev, to_host
+    L = _lib()
+    rng = np.random.default_rng(10 + np_)
+    nl = 7
+    Ms = [1.0 + rng.random((np_, 35)) for _ in range(nl)]
+    Fs = [rng.standard_normal((np_, 35)) for _ in range(nl)]
+    lo = [-0.5 - rng.random(np_) for _ in range(nl)]
+    hi = [0.5 + rng.random(np_) for _ in range(nl)]
+    lo[0][:] = hi[0][:] = 0.3  # |ll - lr| <= 1e-10: W* = 0 on every face
+    dt, dx = 0.004, 1.0 / 60
+    dev = lambda blocks: to_dev(np.ascontiguousarray(np.vstack(blocks).T))
+    Md, Fd = dev(Ms), dev(Fs)
+    lod, hid = to_dev(np.concatenate(lo)), to_dev(np.concatenate(hi))
+    out = empty_dev((35, nl * np_))
+    L.check(L.lib().hyq_hll_lines(Md.data_ptr(), Fd.data_ptr(), lod.data_ptr(), hid.data_ptr(), nl, np_,
+                                  dt, dx, out.data_ptr()), "hll")
+    got = to_host(out).T.reshape(nl, np_, 35)
+    for l in range(nl):
+        ref = O.pas_hll(Ms[l], Fs[l], dt, dx, lo[l], hi[l])
+        assert scaled_err(got[l], ref) < RTOL, l
+    if np_ == 20:
+        i = load_json("mat_goldens.json")["test_numerical_schemes_golden"]["pas_hll"]["input"]
+        gM, gF = np.asarray(i["M"], float), np.asarray(i["F"], float)
+        out1 = empty_dev((35, len(gM)))
+        L.check(L.lib().hyq_hll_lines(to_dev(np.ascontiguousarray(gM.T)).data_ptr(),
+                                      to_dev(np.ascontiguousarray(gF.T)).data_ptr(),
+                                      to_dev(np.asarray(i["vpmin"], float)).data_ptr(),
+                                      to_dev(np.asarray(i["vpmax"], float)).data_ptr(), 1, len(gM),
+                                      float(i["dt"]), float(i["dx"]), out1.data_ptr()), "hll golden")
+        np.testing.assert_allclose(to_host(out1).T, gM, atol=1e-12)
+
+
 def _params(N, Ma, Kn, nnmax, tmax, precision="fp64"):
     import hyqmom_amd as H
     bg, jets = H.crossing_jets(Ma)
