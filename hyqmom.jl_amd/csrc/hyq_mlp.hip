@@ -229,9 +229,15 @@ __global__ void k_pack(const float* __restrict__ p, short* __restrict__ pk) {
 
 
 // WAVES waves per block share the weights; each wave takes RT 16-row tiles at a time, so
-// every B operand read from LDS feeds RT MFMAs.  MASK: rows whose column 15 is 0 need no
-// prediction; a wave whose rows all are such writes NaN predictions and skips the work
-// (it still takes part in the block's barriers).
+// every B operand read from LDS feeds RT MFMAs.  A wave's activations live in its own LDS
+// tile, so after the weights are loaded the waves run independently (no block barrier in the
+// loop; a wavefront fence orders a wave's activation stores before its loads).  MASK: rows
+// whose column 15 is 0 need no prediction; a wave whose rows all are such writes NaN
+// predictions and skips the work.
+__device__ __forceinline__ void wave_lds_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
 template <int WAVES, int RT, bool MASK>
 __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restrict__ X, long n,
                                                           const float* __restrict__ p,
@@ -251,107 +257,104 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
   if (tid < 2) bias[3 * HID + tid] = p[OFF_B4 + tid];
   __syncthreads();
   const int r = lane & 15, q = lane >> 4;
+  // this lane's bias values: column 16 t + r of each hidden layer
+  float bl[3][8];
+#pragma unroll
+  for (int L = 0; L < 3; ++L)
+#pragma unroll
+    for (int t = 0; t < 8; ++t) bl[L][t] = bias[HID * L + 16 * t + r];
   const long ngroups = (n + 16 * RT - 1) / (16 * RT);  // RT-tile groups
   short* A = act[w];
-  // block-uniform loop: every wave runs every iteration (rows past n are zeros, not stored)
-  for (long base = (long)blockIdx.x * WAVES; base < ngroups; base += (long)gridDim.x * WAVES) {
-    const long row0 = (base + w) * 16 * RT;
-    bool work = true;
+  for (long grp = (long)blockIdx.x * WAVES + w; grp < ngroups; grp += (long)gridDim.x * WAVES) {
+    const long row0 = grp * 16 * RT;
     if (MASK) {
       const long row = row0 + lane;
       const bool need = lane < 16 * RT && row < n && X[row * 16 + 15] != 0.f;
-      work = __ballot(need) != 0ull;
-      if (!work && lane < 16 * RT && row < n) {
-        P[row * 2] = NAN;
-        P[row * 2 + 1] = NAN;
+      if (__ballot(need) == 0ull) {
+        if (lane < 16 * RT && row < n) {
+          P[row * 2] = NAN;
+          P[row * 2 + 1] = NAN;
+        }
+        continue;
       }
     }
     f4 acc[RT][8];
-    if (work) {
-      // layer 1: normalized input (k = 4q..4q+3 of 16) times W1
+    // layer 1: normalized input (k = 4q..4q+3 of 16) times W1
 #pragma unroll
-      for (int u = 0; u < RT; ++u) {
-        const long row = row0 + 16 * u + r;
-        s4 a;
+    for (int u = 0; u < RT; ++u) {
+      const long row = row0 + 16 * u + r;
+      s4 a;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int k = 4 * q + e;
-          float v = 0.f;
-          if (row < n && k < 15) v = (X[row * 16 + k] - p[OFF_XMU + k]) / p[OFF_XSD + k];
-          a[e] = f2bf(v);
-        }
-#pragma unroll
-        for (int t = 0; t < 8; ++t)
-          acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wts[(PK_W1 / 4) + t * 64 + lane],
-                                                                f4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+      for (int e = 0; e < 4; ++e) {
+        const int k = 4 * q + e;
+        float v = 0.f;
+        if (row < n && k < 15) v = (X[row * 16 + k] - p[OFF_XMU + k]) / p[OFF_XSD + k];
+        a[e] = f2bf(v);
       }
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wts[(PK_W1 / 4) + t * 64 + lane],
+                                                              f4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < RT; ++u)
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          A[(16 * u + 4 * q + e) * ACT_LD + 16 * t + r] = f2bf(gelu(acc[u][t][e] + bl[0][t]));
+    wave_lds_fence();
+    // layers 2 and 3
+#pragma unroll
+    for (int L = 0; L < 2; ++L) {
+      const int wb = (L == 0 ? PK_W2 : PK_W3) / 4;
+#pragma unroll
+      for (int u = 0; u < RT; ++u)
+#pragma unroll
+        for (int t = 0; t < 8; ++t) acc[u][t] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < 8; ++ks) {
+        s4 av[RT];
+#pragma unroll
+        for (int u = 0; u < RT; ++u)
+          av[u] = *reinterpret_cast<const s4*>(&A[(16 * u + r) * ACT_LD + 16 * ks + 4 * q]);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const s4 bv = wts[wb + (ks * 8 + t) * 64 + lane];
+#pragma unroll
+          for (int u = 0; u < RT; ++u)
+            acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[u], bv, acc[u][t], 0, 0, 0);
+        }
+      }
+      wave_lds_fence();  // every lane has read the tile before it is overwritten
 #pragma unroll
       for (int u = 0; u < RT; ++u)
 #pragma unroll
         for (int t = 0; t < 8; ++t)
 #pragma unroll
           for (int e = 0; e < 4; ++e)
-            A[(16 * u + 4 * q + e) * ACT_LD + 16 * t + r] = f2bf(gelu(acc[u][t][e] + bias[16 * t + r]));
-    }
-    __syncthreads();
-    // layers 2 and 3
-#pragma unroll
-    for (int L = 0; L < 2; ++L) {
-      const int wb = (L == 0 ? PK_W2 : PK_W3) / 4;
-      if (work) {
-#pragma unroll
-        for (int u = 0; u < RT; ++u)
-#pragma unroll
-          for (int t = 0; t < 8; ++t) acc[u][t] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int ks = 0; ks < 8; ++ks) {
-          s4 av[RT];
-#pragma unroll
-          for (int u = 0; u < RT; ++u)
-            av[u] = *reinterpret_cast<const s4*>(&A[(16 * u + r) * ACT_LD + 16 * ks + 4 * q]);
-#pragma unroll
-          for (int t = 0; t < 8; ++t) {
-            const s4 bv = wts[wb + (ks * 8 + t) * 64 + lane];
-#pragma unroll
-            for (int u = 0; u < RT; ++u)
-              acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[u], bv, acc[u][t], 0, 0, 0);
-          }
-        }
-      }
-      __syncthreads();
-      if (work) {
-        const float* bl = bias + HID * (L + 1);
-#pragma unroll
-        for (int u = 0; u < RT; ++u)
-#pragma unroll
-          for (int t = 0; t < 8; ++t)
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-              A[(16 * u + 4 * q + e) * ACT_LD + 16 * t + r] = f2bf(gelu(acc[u][t][e] + bl[16 * t + r]));
-      }
-      __syncthreads();
+            A[(16 * u + 4 * q + e) * ACT_LD + 16 * t + r] = f2bf(gelu(acc[u][t][e] + bl[L + 1][t]));
+      wave_lds_fence();
     }
     // layer 4: one 16-column tile, outputs 0 and 1 kept
-    if (work) {
 #pragma unroll
-      for (int u = 0; u < RT; ++u) {
-        f4 o = f4{0.f, 0.f, 0.f, 0.f};
+    for (int u = 0; u < RT; ++u) {
+      f4 o = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int ks = 0; ks < 8; ++ks) {
-          const s4 av = *reinterpret_cast<const s4*>(&A[(16 * u + r) * ACT_LD + 16 * ks + 4 * q]);
-          o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, wts[(PK_W4 / 4) + ks * 64 + lane], o, 0, 0, 0);
-        }
-        if (r < 2) {
-          const float mu = p[OFF_YMU + r], sd = p[OFF_YSD + r], b = bias[3 * HID + r];
+      for (int ks = 0; ks < 8; ++ks) {
+        const s4 av = *reinterpret_cast<const s4*>(&A[(16 * u + r) * ACT_LD + 16 * ks + 4 * q]);
+        o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, wts[(PK_W4 / 4) + ks * 64 + lane], o, 0, 0, 0);
+      }
+      if (r < 2) {
+        const float mu = p[OFF_YMU + r], sd = p[OFF_YSD + r], b = bias[3 * HID + r];
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const long row = row0 + 16 * u + 4 * q + e;
-            if (row < n) P[row * 2 + r] = (o[e] + b) * sd + mu;
-          }
+        for (int e = 0; e < 4; ++e) {
+          const long row = row0 + 16 * u + 4 * q + e;
+          if (row < n) P[row * 2 + r] = (o[e] + b) * sd + mu;
         }
       }
     }
-    __syncthreads();  // the next group overwrites A
+    wave_lds_fence();  // the next group overwrites A
   }
 }
 
