@@ -185,10 +185,17 @@ def fields(M):
 
 
 def one_step(N, precision, M, model=None):
+    """One step from M.  A non-finite z Jacobian (HYQ_ENONFINITE) still completes the step on
+    the device; the state is returned with whatever the step left in that cell."""
     import hyqmom_amd as H
+    from hyqmom_amd._lib import HyqError
     s = H.Solver(jets_params(N, precision, model))
     s.set_state(M)
-    s.step()
+    try:
+        s.step()
+    except HyqError as e:
+        if "HYQ_ENONFINITE" not in str(e):
+            raise
     out = s.get_state()
     s.close()
     return out
@@ -197,27 +204,48 @@ def one_step(N, precision, M, model=None):
 def accuracy(N, precision, M, model=None):
     """One step of `precision` and of FP64 from the same FP64-trajectory state.  Cells whose
     FP64 result itself moves by more than 1e-8 (field-scaled) when the input is perturbed by
-    one ulp are ill-conditioned (realizability / eigenvalue branch knife edges) and are
-    counted, not compared; the max relative error (field-scaled) is over the other cells."""
+    one ulp, or becomes non-finite, are ill-conditioned (realizability / eigenvalue branch
+    knife edges) and are counted, not compared; the max relative error (field-scaled) is
+    over the other cells."""
     import numpy as np
-    rng = np.random.default_rng(7)
-    Mp = np.asfortranarray(M.copy())
-    flip = rng.random(M.shape) < 0.5
-    Mp[flip] = np.nextafter(Mp[flip], np.inf)
     ref = one_step(N, "fp64", M)
-    pert = one_step(N, "fp64", Mp)
     got = one_step(N, precision, M, model)
-    fr, fp, fg = fields(ref), fields(pert), fields(got)
+    # the perturbed step can itself meet a non-finite z Jacobian, which makes its dt and so
+    # every cell non-finite: then another perturbation is drawn, and if none gives a finite
+    # step, no cell is classed ill-conditioned
+    pert, used = None, None
+    for seed in range(7, 15):
+        rng = np.random.default_rng(seed)
+        Mp = np.asfortranarray(M.copy())
+        flip = rng.random(M.shape) < 0.5
+        Mp[flip] = np.nextafter(Mp[flip], np.inf)
+        p = one_step(N, "fp64", Mp)
+        if np.isfinite(p[..., 0]).mean() > 0.99:
+            pert, used = p, seed
+            break
+    fr, fg = fields(ref), fields(got)
+    bad = np.zeros(M.shape[:3], bool)  # non-finite in the FP64 step itself
+    for k in fr:
+        bad |= ~np.isfinite(fr[k])
     ill = np.zeros(M.shape[:3], bool)
+    if pert is not None:
+        fp = fields(pert)
+        for k in fr:
+            sc = np.abs(fr[k][~bad]).max()
+            with np.errstate(invalid="ignore"):
+                ill |= ~bad & ~(np.abs(fp[k] - fr[k]) / sc <= 1e-8)
+    keep = ~bad & ~ill
+    out = {"ill_conditioned_cells": int(ill.sum()), "nonfinite_cells": int(bad.sum()),
+           "perturbation_seed": used, "mode_step_finite": float(np.isfinite(got[..., 0]).mean())}
     for k in fr:
-        sc = np.abs(fr[k]).max()
-        ill |= np.abs(fp[k] - fr[k]) / sc > 1e-8
-    out = {"ill_conditioned_cells": int(ill.sum())}
-    for k in fr:
-        sc = np.abs(fr[k]).max()
+        if not keep.any():
+            out[k] = None
+            continue
+        sc = np.abs(fr[k][keep]).max()
         e = np.abs(fg[k] - fr[k]) / sc
-        out[k] = float(e[~ill].max())
-    out["max"] = max(out[k] for k in fr)
+        out[k] = float(e[keep].max())
+    vals = [out[k] for k in fr if out[k] is not None]
+    out["max"] = max(vals) if vals else None
     return out
 
 

@@ -76,6 +76,15 @@ constexpr bool kSplitClosure = HYQ_SPLIT1 == 2;  // the closure's two halves in 
 #define HYQ_DEFER 1
 #endif
 constexpr bool kDefer = HYQ_DEFER != 0;
+// Fast modes: run the pass-2 list kernel of step n on a second stream, beside the pass 1 of
+// step n+1 (a cell's pass 1 needs only its own pass-2 result; the listed cells take their
+// pass 1 after the list kernel is done).  Measured on the 120x120x60 workload (100 steps) it
+// hid the list kernel (pass 2 1.03 -> 0.73 ms) but slowed pass 1 by more (1.76 -> 2.09 ms),
+// the list kernel's waves holding registers the fast kernel needs, so it is off by default.
+#ifndef HYQ_OVERLAP
+#define HYQ_OVERLAP 0
+#endif
+constexpr bool kOverlap = HYQ_OVERLAP != 0;
 
 constexpr int kNev = 5;  // timing events per step
 // dynamic LDS of the kernels that run the QR eigen solve (hyq_eig.h: qr_mat; 0 unless the
@@ -136,13 +145,15 @@ __global__ __launch_bounds__(kBlock, EM == 0 ? kPassWaves : kFastWaves) void k_p
 }
 
 // ---- dt (simulation_runner.jl:402-433, :496) -------------------------------------------
-__global__ void k_step_begin(DevScalars* sc) {
+// reset2 = 0 when the previous step's pass-2 list may still be in use on the second stream
+// (k_reset2 empties it once that is done).
+__global__ void k_step_begin(DevScalars* sc, int reset2) {
   const bool act = (sc->t < sc->tmax) && (sc->nn < sc->nnmax);
   sc->active = act ? 1.0 : 0.0;
   if (act) sc->nn += 1;
   sc->vmax_bits = 0ull;
   sc->nhard1 = 0u;
-  sc->nhard2 = 0u;
+  if (reset2) sc->nhard2 = 0u;
 }
 __global__ void k_dt(DevScalars* sc, double CFL, double hmin, double dtmax, double Kn) {
   if (sc->active == 0.0) return;
@@ -387,10 +398,11 @@ __global__ __launch_bounds__(kBlock) void k_plane_dataset(const double* __restri
 template <bool FEAT>
 __global__ __launch_bounds__(kBlock, kPassWaves) void k_pass1a(const double* __restrict__ M, double* __restrict__ A,
                                                    float* __restrict__ X, uint32_t* __restrict__ flags,
-                                                   DevScalars* sc, long long N, int flag2D, double Ma) {
+                                                   DevScalars* sc, long long N, int flag2D, double Ma,
+                                                   const unsigned char* __restrict__ pend) {
   if (sc->active == 0.0) return;
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= N) return;
+  if (c >= N || (pend && pend[c])) return;
   double m[35], a[35];
 #pragma unroll
   for (int t = 0; t < 35; ++t) m[t] = M[(size_t)t * N + c];
@@ -415,9 +427,11 @@ __global__ __launch_bounds__(kBlock, kPassWaves) void k_pass1a(const double* __r
 // `per` consecutive entries (per = ceil(n / waves), at most 64) on its first `per` lanes, and
 // the grid strides by waves * per.  Few cells per wave keep the union of their branch paths,
 // which a wave executes, short; the deferred cells are the rare, branch-heavy ones.
+// compact: 64 entries per wave, so that the list occupies as few wave slots as it can (for a
+// list kernel that runs beside other work).
 struct ListSpread {
   unsigned n, per, wave, lane, stride;
-  __device__ explicit ListSpread(unsigned n_) : n(n_) {
+  __device__ explicit ListSpread(unsigned n_, bool compact = false) : n(n_) {
     const unsigned waves = gridDim.x * (blockDim.x / 64);
 #ifdef HYQ_LIST_PER
     per = HYQ_LIST_PER;
@@ -425,6 +439,7 @@ struct ListSpread {
     per = (n + waves - 1) / waves;
     per = per < 1 ? 1 : (per > 64 ? 64 : per);
 #endif
+    if (compact) per = 64;
     wave = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
     lane = threadIdx.x & 63;
     stride = waves * per;
@@ -448,57 +463,68 @@ constexpr unsigned kListBlocks = 256 * 4 * kFullWaves / 2;
 // and go to the list for k_pass1_full.  PROJECT: the projection runs here (mixed_newton);
 // otherwise k_pass1a already left it in Mr and the network's seeds are in P (learned).
 template <bool PROJECT>
+__device__ __forceinline__ unsigned long long pass1_fast_one(
+    long long c, const double* __restrict__ M, double* __restrict__ AMr, float* __restrict__ F,
+    float* __restrict__ vp, uint32_t* __restrict__ flags, const float* __restrict__ P,
+    int* __restrict__ list, DevScalars* sc, long long N, int flag2D, double Ma, int positivity) {
+  unsigned long long vb = 0;
+  double a[35];
+  uint32_t fl = 0;
+  if (PROJECT) {
+    double m[35];
+#pragma unroll
+    for (int t = 0; t < 35; ++t) m[t] = M[(size_t)t * N + c];
+    pass1a_cell(m, flag2D, Ma, a, fl);
+  } else {
+#pragma unroll
+    for (int t = 0; t < 35; ++t) a[t] = AMr[(size_t)t * N + c];
+    fl = flags[c];
+  }
+  Warm w = no_warm();
+  if (!PROJECT) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      w.w[2 * q] = P[((size_t)q * N + c) * 2];
+      w.w[2 * q + 1] = P[((size_t)q * N + c) * 2 + 1];
+    }
+  }
+  double v6[6];
+  if (fast_cell_speeds(a, positivity != 0, true, w, v6, fl)) {
+    double mr[35], v[6];
+    pass1_finish<float>(a, flag2D, Ma, mr, F + c, F + (size_t)35 * N + c, F + (size_t)70 * N + c,
+                        (size_t)N, v6, v, fl);
+#pragma unroll
+    for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = mr[t];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+      vp[(size_t)d * N + c] = (float)v[d];
+      const unsigned long long b = __double_as_longlong(fabs(v[d]));
+      vb = b > vb ? b : vb;
+    }
+    flags[c] = fl;
+  } else {
+    if (PROJECT) {
+#pragma unroll
+      for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = a[t];
+    }
+    flags[c] = fl & 0xFFFFu;  // the full path counts its own iterations
+    list[atomicAdd(&sc->nhard1, 1u)] = (int)c;
+  }
+  return vb;
+}
+
+// pend (or null): cells whose pass 2 of the previous step is still running on the second
+// stream; they are skipped here and take their pass 1 in k_pass1_pend.
+template <bool PROJECT>
 __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_fast(
     const double* __restrict__ M, double* __restrict__ AMr, float* __restrict__ F, float* __restrict__ vp,
     uint32_t* __restrict__ flags, const float* __restrict__ P, int* __restrict__ list, DevScalars* sc,
-    long long N, int flag2D, double Ma, int positivity) {
+    long long N, int flag2D, double Ma, int positivity, const unsigned char* __restrict__ pend) {
   if (sc->active == 0.0) return;
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long vb = 0;
-  if (c < N) {
-    double a[35];
-    uint32_t fl = 0;
-    if (PROJECT) {
-      double m[35];
-#pragma unroll
-      for (int t = 0; t < 35; ++t) m[t] = M[(size_t)t * N + c];
-      pass1a_cell(m, flag2D, Ma, a, fl);
-    } else {
-#pragma unroll
-      for (int t = 0; t < 35; ++t) a[t] = AMr[(size_t)t * N + c];
-      fl = flags[c];
-    }
-    Warm w = no_warm();
-    if (!PROJECT) {
-#pragma unroll
-      for (int q = 0; q < 5; ++q) {
-        w.w[2 * q] = P[((size_t)q * N + c) * 2];
-        w.w[2 * q + 1] = P[((size_t)q * N + c) * 2 + 1];
-      }
-    }
-    double v6[6];
-    if (fast_cell_speeds(a, positivity != 0, true, w, v6, fl)) {
-      double mr[35], v[6];
-      pass1_finish<float>(a, flag2D, Ma, mr, F + c, F + (size_t)35 * N + c, F + (size_t)70 * N + c,
-                          (size_t)N, v6, v, fl);
-#pragma unroll
-      for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = mr[t];
-#pragma unroll
-      for (int d = 0; d < 6; ++d) {
-        vp[(size_t)d * N + c] = (float)v[d];
-        const unsigned long long b = __double_as_longlong(fabs(v[d]));
-        vb = b > vb ? b : vb;
-      }
-      flags[c] = fl;
-    } else {
-      if (PROJECT) {
-#pragma unroll
-        for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = a[t];
-      }
-      flags[c] = fl & 0xFFFFu;  // the full path counts its own iterations
-      list[atomicAdd(&sc->nhard1, 1u)] = (int)c;
-    }
-  }
+  if (c < N && !(pend && pend[c]))
+    vb = pass1_fast_one<PROJECT>(c, M, AMr, F, vp, flags, P, list, sc, N, flag2D, Ma, positivity);
   vb = wave_max_u64(vb);
   if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
 }
@@ -624,6 +650,28 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_emit(
   if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
 }
 
+// Pass 1 of the cells that were still in the previous step's pass-2 list when the fast
+// pass 1 ran (the analytic start; failures go to the pass-1 list).  Then k_reset2 empties the
+// pass-2 list.
+__global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_pend(
+    const double* __restrict__ M, double* __restrict__ AMr, float* __restrict__ F, float* __restrict__ vp,
+    uint32_t* __restrict__ flags, const int* __restrict__ list2, int* __restrict__ list1,
+    DevScalars* sc, long long N, int flag2D, double Ma, int positivity) {
+  if (sc->active == 0.0) return;
+  const ListSpread ls(sc->nhard2);
+  if (ls.idle()) return;
+  unsigned long long vb = 0;
+  for (unsigned i = ls.first(); i < ls.n; i += ls.stride) {
+    if (!ls.mine(i)) continue;
+    const unsigned long long b = pass1_fast_one<true>(list2[i], M, AMr, F, vp, flags, nullptr, list1,
+                                                      sc, N, flag2D, Ma, positivity);
+    vb = b > vb ? b : vb;
+  }
+  vb = wave_max_u64(vb);
+  if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
+}
+__global__ void k_reset2(DevScalars* sc) { sc->nhard2 = 0u; }
+
 // The full pass 1 (QR fallbacks, corrections, z clamp / re-projection) from the projected
 // state in Mr, for the listed cells.  P: the network's seeds (learned mode) or null.
 __global__ __launch_bounds__(kBlock, kFullWaves) void k_pass1_full(
@@ -689,11 +737,13 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_fast(double* __res
 
 // Split form of k_pass2_fast: projection and the x/y hyperbolicity checks (projected state
 // to A; A[0] = NaN marks a deferred cell), then projection and BGK.
+// pend (or null): marks the listed cells for the next step's pass 1.
 __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_check(const double* __restrict__ M,
                                                                    double* __restrict__ A,
                                                                    int* __restrict__ list,
                                                                    DevScalars* sc, long long N,
-                                                                   int flag2D, double Ma) {
+                                                                   int flag2D, double Ma,
+                                                                   unsigned char* __restrict__ pend) {
   if (sc->active == 0.0) return;
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
@@ -701,13 +751,15 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_check(const double
 #pragma unroll
   for (int n = 0; n < 35; ++n) m[n] = M[(size_t)n * N + c];
   uint32_t fl = 0;
-  if (pass2_check(m, flag2D, Ma, a, fl)) {
+  const bool ok = pass2_check(m, flag2D, Ma, a, fl);
+  if (ok) {
 #pragma unroll
     for (int n = 0; n < 35; ++n) A[(size_t)n * N + c] = a[n];
   } else {
     A[c] = NAN;
     list[atomicAdd(&sc->nhard2, 1u)] = (int)c;
   }
+  if (pend) pend[c] = ok ? 0 : 1;
 }
 
 __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_relax(double* __restrict__ M,
@@ -731,13 +783,17 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_relax(double* __re
 }
 
 // The full pass 2 for the listed cells.
+// On the second stream (overlapped, check_active = 0) the next step may already have changed
+// `active`; the list count alone then says whether there is work (0 after an inactive step),
+// and the list is packed into as few waves as possible.
 __global__ __launch_bounds__(kBlock, kFullWaves) void k_pass2_full(double* __restrict__ M,
                                                                   const int* __restrict__ list,
                                                                   DevScalars* sc, long long N,
                                                                   int flag2D, double Ma,
-                                                                  int positivity, double Kn) {
-  if (sc->active == 0.0) return;
-  const ListSpread ls(sc->nhard2);
+                                                                  int positivity, double Kn,
+                                                                  int check_active) {
+  if (check_active && sc->active == 0.0) return;
+  const ListSpread ls(sc->nhard2, check_active == 0);
   if (ls.idle()) return;
   for (unsigned i = ls.first(); i < ls.n; i += ls.stride) {
     if (!ls.mine(i)) continue;
@@ -777,6 +833,10 @@ struct hyq_ctx {
   float* P;      // [5N][2] predictions
   int *list1, *list2;  // cells deferred to the full pass 1 / pass 2 (fast modes)
   double* V6;          // [6][N] 6x6 wave speeds between the split pass-1 kernels
+  bool overlap;            // the pass-2 list kernel runs on stream2 (fast modes)
+  hipStream_t stream2;
+  hipEvent_t evA, evB;     // pass 2 checked (stream -> stream2) / list done (stream2 -> stream)
+  unsigned char* pend;     // [N] cells in the pass-2 list still running on stream2
 };
 
 extern "C" {
@@ -819,6 +879,14 @@ int hyq_create(const hyq_params* params, hyq_ctx** out) {
     ok = hipMalloc(&c->list1, sizeof(int) * c->N) == hipSuccess &&
          hipMalloc(&c->list2, sizeof(int) * c->N) == hipSuccess &&
          (!kSplit1 || hipMalloc(&c->V6, sizeof(double) * 6 * c->N) == hipSuccess);
+  c->overlap = ok && kOverlap && kDefer && kSplit2 && !kSplit1 &&
+               (p.precision == HYQ_PREC_MIXED_NEWTON || p.precision == HYQ_PREC_MIXED_LEARNED);
+  if (c->overlap)
+    ok = hipMalloc(&c->pend, c->N) == hipSuccess &&
+         hipMemset(c->pend, 0, c->N) == hipSuccess &&
+         hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&c->evA, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&c->evB, hipEventDisableTiming) == hipSuccess;
   if (!ok) {
     hyq_destroy(c);
     return fail(HYQ_ENOMEM, "device allocation failed");
@@ -830,7 +898,12 @@ int hyq_create(const hyq_params* params, hyq_ctx** out) {
 
 int hyq_destroy(hyq_ctx* c) {
   if (!c) return HYQ_OK;
+  if (c->stream2) hipStreamSynchronize(c->stream2);
   if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->stream2) hipStreamDestroy(c->stream2);
+  if (c->evA) hipEventDestroy(c->evA);
+  if (c->evB) hipEventDestroy(c->evB);
+  hipFree(c->pend);
   hipFree(c->M);
   hipFree(c->Mr);
   hipFree(c->F);
@@ -869,7 +942,8 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
   const hyq_params& p = c->p;
   const unsigned g = grid_for(c->N);
   const double hmin = jmin(jmin(c->dx, c->dy), c->dz);
-  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, c->stream, c->sc);
+  const unsigned char* pend = c->overlap ? c->pend : nullptr;
+  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, c->stream, c->sc, c->overlap ? 0 : 1);
   if (ev) hipEventRecord(ev[0], c->stream);
   if (p.precision == HYQ_PREC_FP64) {
     hipLaunchKernelGGL((k_pass1<double, 0>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->Mr,
@@ -890,7 +964,8 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
       // projection (+ features), network, 6x6 wave speeds, closure and fluxes
       if (learned) {
         hipLaunchKernelGGL(k_pass1a<true>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
-                           c->X, c->flags, c->sc, c->N, p.flag2D, p.Ma);
+                           c->X, c->flags, c->sc, c->N, p.flag2D, p.Ma,
+                           (const unsigned char*)nullptr);
         const int rc = hyq_mlp_infer_masked_async(c->mlp, c->X, 5 * c->N, c->P, c->stream);
         if (rc) return rc;
         hipLaunchKernelGGL(k_pass1_speeds<true>, dim3(g), dim3(kBlock), 0, c->stream,
@@ -898,7 +973,8 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
                            c->sc, c->N, p.positivity);
       } else {
         hipLaunchKernelGGL(k_pass1a<false>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
-                           c->X, c->flags, c->sc, c->N, p.flag2D, p.Ma);
+                           c->X, c->flags, c->sc, c->N, p.flag2D, p.Ma,
+                           (const unsigned char*)nullptr);
         hipLaunchKernelGGL(k_pass1_speeds<false>, dim3(g), dim3(kBlock), 0, c->stream,
                            (const double*)c->Mr, c->flags, (const float*)nullptr, c->V6,
                            c->list1, c->sc, c->N, p.positivity);
@@ -916,16 +992,24 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
     } else if (learned) {
       // projection + features, then the network's warm starts
       hipLaunchKernelGGL(k_pass1a<true>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr, c->X,
-                         c->flags, c->sc, c->N, p.flag2D, p.Ma);
+                         c->flags, c->sc, c->N, p.flag2D, p.Ma, pend);
       const int rc = hyq_mlp_infer_masked_async(c->mlp, c->X, 5 * c->N, c->P, c->stream);
       if (rc) return rc;
       hipLaunchKernelGGL(k_pass1_fast<false>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
                          (float*)c->F, (float*)c->vp, c->flags, (const float*)c->P, c->list1,
-                         c->sc, c->N, p.flag2D, p.Ma, p.positivity);
+                         c->sc, c->N, p.flag2D, p.Ma, p.positivity, pend);
     } else {
       hipLaunchKernelGGL(k_pass1_fast<true>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
                          (float*)c->F, (float*)c->vp, c->flags, (const float*)nullptr, c->list1,
-                         c->sc, c->N, p.flag2D, p.Ma, p.positivity);
+                         c->sc, c->N, p.flag2D, p.Ma, p.positivity, pend);
+    }
+    if (c->overlap) {
+      // the previous step's pass-2 list is done: pass 1 of its cells, then empty the list
+      hipStreamWaitEvent(c->stream, c->evB, 0);
+      hipLaunchKernelGGL(k_pass1_pend, dim3(kListBlocks), dim3(kBlock), 0, c->stream,
+                         (const double*)c->M, c->Mr, (float*)c->F, (float*)c->vp, c->flags,
+                         (const int*)c->list2, c->list1, c->sc, c->N, p.flag2D, p.Ma, p.positivity);
+      hipLaunchKernelGGL(k_reset2, dim3(1), dim3(1), 0, c->stream, c->sc);
     }
     hipLaunchKernelGGL(k_pass1_full, dim3(kListBlocks), dim3(kBlock), kQrLds, c->stream, c->Mr,
                        (float*)c->F, (float*)c->vp, c->flags,
@@ -957,15 +1041,27 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
   } else {
     if (kSplit2) {
       hipLaunchKernelGGL(k_pass2_check, dim3(g), dim3(kBlock), 0, c->stream, (const double*)c->M,
-                         c->Mr, c->list2, c->sc, c->N, p.flag2D, p.Ma);
+                         c->Mr, c->list2, c->sc, c->N, p.flag2D, p.Ma,
+                         c->overlap ? c->pend : (unsigned char*)nullptr);
+      if (c->overlap) {
+        // the listed cells' full pass 2 on stream2, beside this step's BGK and the next
+        // step's pass 1 of the other cells
+        hipEventRecord(c->evA, c->stream);
+        hipStreamWaitEvent(c->stream2, c->evA, 0);
+        hipLaunchKernelGGL(k_pass2_full, dim3(kListBlocks), dim3(kBlock), kQrLds, c->stream2,
+                           c->M, (const int*)c->list2, c->sc, c->N, p.flag2D, p.Ma, p.positivity,
+                           p.Kn, 0);
+        hipEventRecord(c->evB, c->stream2);
+      }
       hipLaunchKernelGGL(k_pass2_relax, dim3(g), dim3(kBlock), 0, c->stream, c->M,
                          (const double*)c->Mr, c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
     } else {
       hipLaunchKernelGGL(k_pass2_fast, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->list2,
                          c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
     }
-    hipLaunchKernelGGL(k_pass2_full, dim3(kListBlocks), dim3(kBlock), kQrLds, c->stream, c->M,
-                       (const int*)c->list2, c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
+    if (!c->overlap)
+      hipLaunchKernelGGL(k_pass2_full, dim3(kListBlocks), dim3(kBlock), kQrLds, c->stream, c->M,
+                         (const int*)c->list2, c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn, 1);
   }
   if (ev) hipEventRecord(ev[4], c->stream);
   const hipError_t e = hipGetLastError();
@@ -981,7 +1077,13 @@ static int load_scalars(hyq_ctx* c, double t, long long nn, double tmax, long lo
   s.nn = nn;
   s.nnmax = nnmax;
   HIPCHK(hipMemcpyAsync(c->sc, &s, sizeof(s), hipMemcpyHostToDevice, c->stream), "scalars");
+  // no pass-2 list is in flight between calls (each call joins stream2 before it returns)
+  if (c->overlap) HIPCHK(hipMemsetAsync(c->pend, 0, c->N, c->stream), "pending marks");
   return HYQ_OK;
+}
+// The last step's pass-2 list kernel on stream2 completes before the context stream goes on.
+static void join_stream2(hyq_ctx* c) {
+  if (c->overlap) hipStreamWaitEvent(c->stream, c->evB, 0);
 }
 
 static int accumulate_times(hyq_ctx* c, hipEvent_t* ev) {
@@ -1002,7 +1104,11 @@ int hyq_step(hyq_ctx* c, double* t_inout, double* dt_out, int64_t* nn_inout) {
   int rc = load_scalars(c, *t_inout, *nn_inout, c->p.tmax, (long long)*nn_inout + 1);
   if (rc) return rc;
   rc = enqueue_step(c, c->ev);
-  if (rc) return rc;
+  join_stream2(c);
+  if (rc) {
+    hipStreamSynchronize(c->stream);
+    return rc;
+  }
   DevScalars s;
   HIPCHK(hipMemcpyAsync(&s, c->sc, sizeof(s), hipMemcpyDeviceToHost, c->stream), "scalars back");
   HIPCHK(hipStreamSynchronize(c->stream), "step sync");
@@ -1030,6 +1136,8 @@ int hyq_run(hyq_ctx* c, double tmax, int64_t nnmax, double* t_inout, int64_t* nn
     rc = enqueue_step(c, &ev[kNev * s]);
     if (rc) break;
   }
+  join_stream2(c);
+  if (rc) hipStreamSynchronize(c->stream);
   DevScalars s;
   if (rc == HYQ_OK) {
     HIPCHK(hipMemcpyAsync(&s, c->sc, sizeof(s), hipMemcpyDeviceToHost, c->stream), "scalars back");

@@ -358,18 +358,139 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
   }
 }
 
+// The same network computed transposed, Y^T = W^T X^T, with the packed weight tiles as the
+// A operand.  MFMA 16x16x16 lane layouts: the A operand of lane l holds row l%16 and
+// k = 4(l/16)+e, the B operand column l%16 and k = 4(l/16)+e, the result rows 4(l/16)+e of
+// column l%16.  A packed tile of W (k-step ks, column tile t) therefore is the A operand of
+// the W^T tile (t, ks) unchanged, and the result of column tile t (features 16t + 4(l/16)+e
+// of sample l%16) is, in bf16, exactly the B operand of k-step t of the next layer: the
+// activations stay in registers and the hidden layers need no LDS round trip.  Only the
+// weights sit in LDS, so two blocks fit on a CU.
+template <int WAVES, int RT, bool MASK>
+__global__ __launch_bounds__(64 * WAVES) void k_mlp_fused_t(const float* __restrict__ X, long n,
+                                                            const float* __restrict__ p,
+                                                            const short* __restrict__ pk,
+                                                            float* __restrict__ P) {
+  __shared__ s4 wts[PK_N / 4];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const s4* pk4 = reinterpret_cast<const s4*>(pk);
+  for (int i = tid; i < PK_N / 4; i += 64 * WAVES) wts[i] = pk4[i];
+  __syncthreads();
+  const int r = lane & 15, q = lane >> 4;
+  const long ngroups = (n + 16 * RT - 1) / (16 * RT);
+  for (long grp = (long)blockIdx.x * WAVES + w; grp < ngroups; grp += (long)gridDim.x * WAVES) {
+    const long row0 = grp * 16 * RT;
+    {
+      // (without MASK the test never skips: the group's first row exists; the branch is kept
+      // so that both instantiations get the same control flow)
+      const long row = row0 + lane;
+      const bool need = lane < 16 * RT && row < n && (!MASK || X[row * 16 + 15] != 0.f);
+      if (__ballot(need) == 0ull) {
+        if (lane < 16 * RT && row < n) {
+          P[row * 2] = NAN;
+          P[row * 2 + 1] = NAN;
+        }
+        continue;
+      }
+    }
+    // layer 1: B operand = normalized input of sample r, features 4q..4q+3
+    s4 h[RT][8];
+    {
+      s4 x[RT];
+#pragma unroll
+      for (int u = 0; u < RT; ++u) {
+        const long row = row0 + 16 * u + r;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int k = 4 * q + e;
+          float v = 0.f;
+          if (row < n && k < 15) v = (X[row * 16 + k] - p[OFF_XMU + k]) / p[OFF_XSD + k];
+          x[u][e] = f2bf(v);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const s4 wv = wts[(PK_W1 / 4) + t * 64 + lane];
+        float b[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) b[e] = p[OFF_B1 + 16 * t + 4 * q + e];
+#pragma unroll
+        for (int u = 0; u < RT; ++u) {
+          const f4 y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wv, x[u], f4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) h[u][t][e] = f2bf(gelu(y[e] + b[e]));
+        }
+      }
+    }
+    // layers 2 and 3
+#pragma unroll
+    for (int L = 0; L < 2; ++L) {
+      const int wb = (L == 0 ? PK_W2 : PK_W3) / 4;
+      const float* bl = p + (L == 0 ? OFF_B2 : OFF_B3);
+      s4 g[RT][8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        f4 acc[RT];
+#pragma unroll
+        for (int u = 0; u < RT; ++u) acc[u] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) {
+          const s4 wv = wts[wb + (ks * 8 + t) * 64 + lane];
+#pragma unroll
+          for (int u = 0; u < RT; ++u)
+            acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wv, h[u][ks], acc[u], 0, 0, 0);
+        }
+        float b[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) b[e] = bl[16 * t + 4 * q + e];
+#pragma unroll
+        for (int u = 0; u < RT; ++u)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) g[u][t][e] = f2bf(gelu(acc[u][e] + b[e]));
+      }
+#pragma unroll
+      for (int u = 0; u < RT; ++u)
+#pragma unroll
+        for (int t = 0; t < 8; ++t) h[u][t] = g[u][t];
+    }
+    // layer 4: outputs 4q+e of sample r; outputs 0 and 1 are on the lanes with q = 0
+#pragma unroll
+    for (int u = 0; u < RT; ++u) {
+      f4 o = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < 8; ++ks)
+        o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wts[(PK_W4 / 4) + ks * 64 + lane], h[u][ks], o, 0, 0, 0);
+      const long row = row0 + 16 * u + r;
+      if (q == 0 && row < n) {
+#pragma unroll
+        for (int e = 0; e < 2; ++e)
+          P[row * 2 + e] = (o[e] + p[OFF_B4 + e]) * p[OFF_YSD + e] + p[OFF_YMU + e];
+      }
+    }
+  }
+}
+
 // Launch shapes of the fused inference (waves per block, row tiles per wave), both within
 // the 160 KB of LDS per CU next to the 72 KB of packed weights: 16 x 1 (4 waves per SIMD)
 // and 8 x 2 (2 waves per SIMD, each B operand feeding two MFMAs).  They compute the same
 // values in the same order; the first large inference times both and keeps the faster.
+constexpr int kFusedShapes = 4;
 void launch_fused(int shape, bool mask, const float* X, long n, const float* p, const short* pk,
                   float* P, hipStream_t s) {
-  const int waves = shape == 0 ? 16 : 8, rt = shape == 0 ? 1 : 2;
+  const int waves = shape == 0 ? 16 : 8, rt = (shape == 0 || shape == 2) ? 1 : 2;
   const long groups = (n + 16 * rt - 1) / (16 * rt);
   long blocks = (groups + waves - 1) / waves;
   if (blocks > 1024) blocks = 1024;
   const dim3 g((unsigned)blocks);
-  if (shape == 0 && !mask)
+  if (shape == 2 && !mask)
+    hipLaunchKernelGGL((k_mlp_fused_t<8, 1, false>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
+  else if (shape == 2)
+    hipLaunchKernelGGL((k_mlp_fused_t<8, 1, true>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
+  else if (shape == 3 && !mask)
+    hipLaunchKernelGGL((k_mlp_fused_t<8, 2, false>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
+  else if (shape == 3)
+    hipLaunchKernelGGL((k_mlp_fused_t<8, 2, true>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
+  else if (shape == 0 && !mask)
     hipLaunchKernelGGL((k_mlp_fused<16, 1, false>), g, dim3(64 * 16), 0, s, X, n, p, pk, P);
   else if (shape == 0)
     hipLaunchKernelGGL((k_mlp_fused<16, 1, true>), g, dim3(64 * 16), 0, s, X, n, p, pk, P);
@@ -555,9 +676,10 @@ static int infer_fused(hyq_mlp* m, const float* X, int64_t n, float* P, hipStrea
   if (m->shape < 0 && n >= 65536) {
     // pick the faster launch shape: every row computed (no mask), best of three runs each,
     // on this call's own rows (the real launch below overwrites the trial predictions)
-    float best[2] = {1e30f, 1e30f};
+    float best[kFusedShapes];
+    for (int sh = 0; sh < kFusedShapes; ++sh) best[sh] = 1e30f;
     for (int rep = 0; rep < 3; ++rep)
-      for (int sh = 0; sh < 2; ++sh) {
+      for (int sh = 0; sh < kFusedShapes; ++sh) {
         float ms = 0.f;
         MCHK(hipEventRecord(m->ev[0], stream), "mlp event");
         launch_fused(sh, false, X, (long)n, m->blob, m->packed, P, stream);
@@ -566,7 +688,10 @@ static int infer_fused(hyq_mlp* m, const float* X, int64_t n, float* P, hipStrea
         MCHK(hipEventElapsedTime(&ms, m->ev[0], m->ev[1]), "mlp event");
         best[sh] = ms < best[sh] ? ms : best[sh];
       }
-    m->shape = best[1] < best[0] ? 1 : 0;
+    int pick = 0;
+    for (int sh = 1; sh < kFusedShapes; ++sh)
+      if (best[sh] < best[pick]) pick = sh;
+    m->shape = pick;
   }
   launch_fused(m->shape < 0 ? 0 : m->shape, mask, X, (long)n, m->blob, m->packed, P, stream);
   MCHK(hipGetLastError(), "mlp infer launch");
@@ -584,6 +709,12 @@ int hyq_mlp_infer_async(hyq_mlp* m, const float* X, int64_t n, float* P, hipStre
 int hyq_mlp_infer_masked_async(hyq_mlp* m, const float* X, int64_t n, float* P,
                                hipStream_t stream) {
   return infer_fused(m, X, n, P, stream, true);
+}
+
+int hyq_mlp_set_shape(hyq_mlp* m, int32_t shape) {
+  if (!m || shape < -1 || shape >= kFusedShapes) return mfail(HYQ_EINVAL, "bad argument");
+  m->shape = shape;
+  return HYQ_OK;
 }
 
 // The same prediction through the layer-by-layer GEMM kernels of training (a cross-check

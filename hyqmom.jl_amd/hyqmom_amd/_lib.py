@@ -71,6 +71,7 @@ SIGNATURES = {
     "hyq_mlp_set_blob": ([VP, VP], ctypes.c_int),
     "hyq_mlp_infer": ([VP, VP, I64, VP], ctypes.c_int),
     "hyq_mlp_infer_layers": ([VP, VP, I64, VP], ctypes.c_int),
+    "hyq_mlp_set_shape": ([VP, ctypes.c_int32], ctypes.c_int),
     "hyq_mlp_grad": ([VP, VP, VP, I64, VP, VP], ctypes.c_int),
     "hyq_mlp_train_step": ([VP, VP, VP, I64, ctypes.c_float, VP], ctypes.c_int),
     "hyq_mlp_last_error": ([], ctypes.c_char_p),

@@ -86,6 +86,10 @@ class WarmStartMLP:
         self._check(_lib.lib().hyq_mlp_infer(self.handle, _ptr(X), X.shape[0], _ptr(P)), "hyq_mlp_infer")
         return P
 
+    def set_shape(self, shape):
+        """Fused inference launch shape: -1 picks the fastest by a timed trial, 0..3 fix one."""
+        self._check(_lib.lib().hyq_mlp_set_shape(self.handle, int(shape)), "hyq_mlp_set_shape")
+
     def infer_layers(self, X):
         """The layer-by-layer path (training kernels) for the same prediction."""
         torch = _torch()

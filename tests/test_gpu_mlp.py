@@ -57,6 +57,40 @@ def test_fused_inference_equals_layers(n):
     assert not np.array_equal(a, c)
 
 
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+def test_every_fused_shape_equals_layers(shape):
+    """Each launch shape of the fused inference, the transposed ones (activations kept in
+    registers) included, gives the layer-by-layer predictions bit for bit, masked rows aside:
+    with the mask a 16-row tile whose rows all have column 15 = 0 gets NaN."""
+    from hyqmom_amd import _lib
+    from hyqmom_amd.warmstart import WarmStartMLP
+    n = 5000
+    X = np.random.default_rng(21).standard_normal((n, 16)).astype(np.float32)
+    X[:, 15] = 1.0
+    X[1600:1632, 15] = 0.0  # two whole 16-row tiles to skip
+    m = WarmStartMLP(R.random_blob(22))
+    m.set_shape(shape)
+    Xd = _dev(X)
+    want = m.infer_layers(Xd).cpu().numpy()
+    got = m.infer(Xd).cpu().numpy()
+    assert np.array_equal(got, want), (shape, np.abs(got - want).max())
+    import ctypes
+    import torch
+    P = torch.empty((n, 2), dtype=torch.float32, device="cuda")
+    # the step's masked inference (exported for the step code, not in the public header)
+    fn = _lib.lib().hyq_mlp_infer_masked_async
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+    fn.restype = ctypes.c_int
+    rc = fn(m.handle, Xd.data_ptr(), n, P.data_ptr(), None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    Pm = P.cpu().numpy()
+    skip = np.zeros(n, bool)
+    skip[1600:1632] = True
+    assert np.isnan(Pm[skip]).all()
+    assert np.array_equal(Pm[~skip], want[~skip])
+
+
 def test_gradient_matches_reference():
     from hyqmom_amd.warmstart import WarmStartMLP
     blob = R.random_blob(21)
