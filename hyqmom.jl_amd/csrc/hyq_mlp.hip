@@ -474,15 +474,20 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused_t(const float* __restr
 // the 160 KB of LDS per CU next to the 72 KB of packed weights: 16 x 1 (4 waves per SIMD)
 // and 8 x 2 (2 waves per SIMD, each B operand feeding two MFMAs).  They compute the same
 // values in the same order; the first large inference times both and keeps the faster.
-constexpr int kFusedShapes = 4;
+constexpr int kFusedShapes = 5;
 void launch_fused(int shape, bool mask, const float* X, long n, const float* p, const short* pk,
                   float* P, hipStream_t s) {
-  const int waves = shape == 0 ? 16 : 8, rt = (shape == 0 || shape == 2) ? 1 : 2;
+  const int waves = (shape == 0 || shape == 4) ? 16 : 8;
+  const int rt = (shape == 0 || shape == 2 || shape == 4) ? 1 : 2;
   const long groups = (n + 16 * rt - 1) / (16 * rt);
   long blocks = (groups + waves - 1) / waves;
   if (blocks > 1024) blocks = 1024;
   const dim3 g((unsigned)blocks);
-  if (shape == 2 && !mask)
+  if (shape == 4 && !mask)
+    hipLaunchKernelGGL((k_mlp_fused_t<16, 1, false>), g, dim3(64 * 16), 0, s, X, n, p, pk, P);
+  else if (shape == 4)
+    hipLaunchKernelGGL((k_mlp_fused_t<16, 1, true>), g, dim3(64 * 16), 0, s, X, n, p, pk, P);
+  else if (shape == 2 && !mask)
     hipLaunchKernelGGL((k_mlp_fused_t<8, 1, false>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
   else if (shape == 2)
     hipLaunchKernelGGL((k_mlp_fused_t<8, 1, true>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
