@@ -85,6 +85,13 @@ constexpr bool kDefer = HYQ_DEFER != 0;
 #define HYQ_OVERLAP 0
 #endif
 constexpr bool kOverlap = HYQ_OVERLAP != 0;
+// mixed_newton: the projection in its own kernel (k_pass1a), then k_pass1_fast from the
+// projected state without seeds, as the learned mode does, instead of one kernel for both.
+// Measured on the 120x120x60 workload (100 steps): pass 1 1.76 -> 1.47 ms.
+#ifndef HYQ_NEWTON_SPLIT
+#define HYQ_NEWTON_SPLIT 1
+#endif
+constexpr bool kNewtonSplit = HYQ_NEWTON_SPLIT != 0;
 
 constexpr int kNev = 5;  // timing events per step
 // dynamic LDS of the kernels that run the QR eigen solve (hyq_eig.h: qr_mat; 0 unless the
@@ -177,9 +184,18 @@ __global__ void k_dt(DevScalars* sc, double CFL, double hmin, double dtmax, doub
 #define HYQ_XCD_SWIZZLE 0
 #endif
 constexpr int kXcds = 8;
-HD inline unsigned update_grid(long long n) {
-  const unsigned g = (unsigned)((n + kBlock - 1) / kBlock);
-  return HYQ_XCD_SWIZZLE ? (g + kXcds - 1) / kXcds * kXcds : g;
+// HYQ_UPDATE_ROWS: a block takes one x-row (j, k) of the grid (ceil(Nx / 128) blocks per
+// row), with k varying fastest over consecutive blocks and those blocks kept on one XCD:
+// a cell's z neighbours are then read by blocks that run at the same time on the same L2.
+// Measured on the 120x120x60 workload: 0.419 against 0.423 ms (mixed), 0.503 against
+// 0.478 ms (FP64); off by default.
+#ifndef HYQ_UPDATE_ROWS
+#define HYQ_UPDATE_ROWS 0
+#endif
+HD inline unsigned update_grid(long long n, int Nx = 0, int Ny = 0, int Nz = 0) {
+  unsigned g = (unsigned)((n + kBlock - 1) / kBlock);
+  if (HYQ_UPDATE_ROWS) g = (unsigned)(((Nx + kBlock - 1) / kBlock) * (long long)Ny * Nz);
+  return (HYQ_XCD_SWIZZLE || HYQ_UPDATE_ROWS) ? (g + kXcds - 1) / kXcds * kXcds : g;
 }
 template <typename FT>
 __global__ __launch_bounds__(kBlock) void k_update(const double* __restrict__ Mr,
@@ -191,13 +207,28 @@ __global__ __launch_bounds__(kBlock) void k_update(const double* __restrict__ Mr
   if (sc->active == 0.0) return;
   const long long N = (long long)Nx * Ny * Nz;
   long long blk = blockIdx.x;
-  if (HYQ_XCD_SWIZZLE) {
+  if (HYQ_XCD_SWIZZLE || HYQ_UPDATE_ROWS) {
     const unsigned per = gridDim.x / kXcds;  // the grid is a multiple of 8 blocks
     blk = (long long)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
   }
-  const long long c = blk * blockDim.x + threadIdx.x;
-  if (c >= N) return;
-  const int i = (int)(c % Nx), j = (int)((c / Nx) % Ny), k = (int)(c / ((long long)Nx * Ny));
+  long long c;
+  int i, j, k;
+  if (HYQ_UPDATE_ROWS) {
+    const int rb = (Nx + kBlock - 1) / kBlock;  // blocks per x-row
+    const long long row = blk / rb;
+    if (row >= (long long)Ny * Nz) return;
+    i = (int)(blk % rb) * kBlock + (int)threadIdx.x;
+    if (i >= Nx) return;
+    k = (int)(row % Nz);
+    j = (int)(row / Nz);
+    c = i + (long long)Nx * (j + (long long)Ny * k);
+  } else {
+    c = blk * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    i = (int)(c % Nx);
+    j = (int)((c / Nx) % Ny);
+    k = (int)(c / ((long long)Nx * Ny));
+  }
   const double dt = sc->dt;
   update_cell<FT>(Mr, F, vp, Mout, c, i, j, k, Nx, Ny, Nz, N, dt / dx, dt / dy, dt / dz);
 }
@@ -481,7 +512,7 @@ __device__ __forceinline__ unsigned long long pass1_fast_one(
     fl = flags[c];
   }
   Warm w = no_warm();
-  if (!PROJECT) {
+  if (!PROJECT && P != nullptr) {
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
       w.w[2 * q] = P[((size_t)q * N + c) * 2];
@@ -998,6 +1029,12 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
       hipLaunchKernelGGL(k_pass1_fast<false>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
                          (float*)c->F, (float*)c->vp, c->flags, (const float*)c->P, c->list1,
                          c->sc, c->N, p.flag2D, p.Ma, p.positivity, pend);
+    } else if (kNewtonSplit) {
+      hipLaunchKernelGGL(k_pass1a<false>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr, c->X,
+                         c->flags, c->sc, c->N, p.flag2D, p.Ma, pend);
+      hipLaunchKernelGGL(k_pass1_fast<false>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
+                         (float*)c->F, (float*)c->vp, c->flags, (const float*)nullptr, c->list1,
+                         c->sc, c->N, p.flag2D, p.Ma, p.positivity, pend);
     } else {
       hipLaunchKernelGGL(k_pass1_fast<true>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
                          (float*)c->F, (float*)c->vp, c->flags, (const float*)nullptr, c->list1,
@@ -1020,11 +1057,11 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
   hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, c->stream, c->sc, p.CFL, hmin, p.dtmax, p.Kn);
   if (ev) hipEventRecord(ev[2], c->stream);
   if (p.precision == HYQ_PREC_FP64) {
-    hipLaunchKernelGGL(k_update<double>, dim3(update_grid(c->N)), dim3(kBlock), 0, c->stream, c->Mr,
+    hipLaunchKernelGGL(k_update<double>, dim3(update_grid(c->N, p.Nx, p.Ny, p.Nz)), dim3(kBlock), 0, c->stream, c->Mr,
                        (const double*)c->F, (const double*)c->vp, c->M, c->sc, p.Nx, p.Ny, p.Nz,
                        c->dx, c->dy, c->dz);
   } else {
-    hipLaunchKernelGGL(k_update<float>, dim3(update_grid(c->N)), dim3(kBlock), 0, c->stream, c->Mr,
+    hipLaunchKernelGGL(k_update<float>, dim3(update_grid(c->N, p.Nx, p.Ny, p.Nz)), dim3(kBlock), 0, c->stream, c->Mr,
                        (const float*)c->F, (const float*)c->vp, c->M, c->sc, p.Nx, p.Ny, p.Nz,
                        c->dx, c->dy, c->dz);
   }
