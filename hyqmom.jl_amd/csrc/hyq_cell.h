@@ -43,33 +43,11 @@ HD void s28_to_packed(const S28& s, double* c) {
 }
 
 // The closure's state between its two halves: the realized standardized 4th-order set, the
-// floored variances and the density and means (35 values, storable as a moment vector).
+// floored variances and the density and means.
 struct ClosureState {
   S28 s;
   double C200, C020, C002, M000, um, vm, wm;
 };
-HD void closure_state_store(const ClosureState& q, double* v) {
-  v[0] = q.s.S300; v[1] = q.s.S400; v[2] = q.s.S110; v[3] = q.s.S210;
-  v[4] = q.s.S310; v[5] = q.s.S120; v[6] = q.s.S220; v[7] = q.s.S030;
-  v[8] = q.s.S130; v[9] = q.s.S040; v[10] = q.s.S101; v[11] = q.s.S201;
-  v[12] = q.s.S301; v[13] = q.s.S102; v[14] = q.s.S202; v[15] = q.s.S003;
-  v[16] = q.s.S103; v[17] = q.s.S004; v[18] = q.s.S011; v[19] = q.s.S111;
-  v[20] = q.s.S211; v[21] = q.s.S021; v[22] = q.s.S121; v[23] = q.s.S031;
-  v[24] = q.s.S012; v[25] = q.s.S112; v[26] = q.s.S013; v[27] = q.s.S022;
-  v[28] = q.C200; v[29] = q.C020; v[30] = q.C002;
-  v[31] = q.M000; v[32] = q.um; v[33] = q.vm; v[34] = q.wm;
-}
-HD void closure_state_load(const double* v, ClosureState& q) {
-  q.s.S300 = v[0]; q.s.S400 = v[1]; q.s.S110 = v[2]; q.s.S210 = v[3];
-  q.s.S310 = v[4]; q.s.S120 = v[5]; q.s.S220 = v[6]; q.s.S030 = v[7];
-  q.s.S130 = v[8]; q.s.S040 = v[9]; q.s.S101 = v[10]; q.s.S201 = v[11];
-  q.s.S301 = v[12]; q.s.S102 = v[13]; q.s.S202 = v[14]; q.s.S003 = v[15];
-  q.s.S103 = v[16]; q.s.S004 = v[17]; q.s.S011 = v[18]; q.s.S111 = v[19];
-  q.s.S211 = v[20]; q.s.S021 = v[21]; q.s.S121 = v[22]; q.s.S031 = v[23];
-  q.s.S012 = v[24]; q.s.S112 = v[25]; q.s.S013 = v[26]; q.s.S022 = v[27];
-  q.C200 = v[28]; q.C020 = v[29]; q.C002 = v[30];
-  q.M000 = v[31]; q.um = v[32]; q.vm = v[33]; q.wm = v[34];
-}
 
 // Flux_closure35_and_realizable_3D (src/numerics/Flux_closure35_and_realizable_3D.jl:33-234),
 // first half: standardization and the realizability projection (:33-176).

@@ -60,16 +60,12 @@ constexpr int kBlock = 128;
 #define HYQ_FAST_WAVES 2
 #endif
 constexpr int kPassWaves = HYQ_PASS_WAVES, kFastWaves = HYQ_FAST_WAVES;
-// The fast modes' pass 1 and pass 2 as two kernels each (wave speeds | closure and fluxes;
-// checks | BGK) instead of one: less live state per kernel, fewer spills.
-#ifndef HYQ_SPLIT1
-#define HYQ_SPLIT1 0
-#endif
+// The fast modes' pass 2 as two kernels (checks | BGK) instead of one: less live state per
+// kernel, fewer spills.
 #ifndef HYQ_SPLIT2
 #define HYQ_SPLIT2 1
 #endif
-constexpr bool kSplit1 = HYQ_SPLIT1 != 0, kSplit2 = HYQ_SPLIT2 != 0;
-constexpr bool kSplitClosure = HYQ_SPLIT1 == 2;  // the closure's two halves in two kernels
+constexpr bool kSplit2 = HYQ_SPLIT2 != 0;
 // mixed_newton: 1 = common cells and deferred cells in separate kernels, 0 = one kernel per
 // pass (k_pass1<float, 1>, k_pass2<true, 1>)
 #ifndef HYQ_DEFER
@@ -560,127 +556,6 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_fast(
   if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
 }
 
-// Split form of k_pass1_fast: the wave speeds of the 6x6 Jacobians in one kernel
-// (V6 [6][N], FP64; V6[0] = NaN marks a deferred cell), closure and fluxes in the next.
-// Each half holds fewer live values than the fused kernel, so it spills less.
-template <bool LEARNED>
-__global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_speeds(
-    const double* __restrict__ A, uint32_t* __restrict__ flags, const float* __restrict__ P,
-    double* __restrict__ V6, int* __restrict__ list, DevScalars* sc, long long N, int positivity) {
-  if (sc->active == 0.0) return;
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= N) return;
-  double a[35];
-#pragma unroll
-  for (int t = 0; t < 35; ++t) a[t] = A[(size_t)t * N + c];
-  Warm w = no_warm();
-  if (LEARNED) {
-#pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      w.w[2 * q] = P[((size_t)q * N + c) * 2];
-      w.w[2 * q + 1] = P[((size_t)q * N + c) * 2 + 1];
-    }
-  }
-  uint32_t fl = flags[c];
-  double v6[6];
-  if (fast_cell_speeds(a, positivity != 0, true, w, v6, fl)) {
-#pragma unroll
-    for (int d = 0; d < 6; ++d) V6[(size_t)d * N + c] = v6[d];
-    flags[c] = fl;
-  } else {
-    V6[c] = NAN;
-    flags[c] = fl & 0xFFFFu;  // the full path counts its own iterations
-    list[atomicAdd(&sc->nhard1, 1u)] = (int)c;
-  }
-}
-
-__global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_finish(
-    double* __restrict__ AMr, float* __restrict__ F, float* __restrict__ vp,
-    uint32_t* __restrict__ flags, const double* __restrict__ V6, DevScalars* sc, long long N,
-    int flag2D, double Ma) {
-  if (sc->active == 0.0) return;
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long vb = 0;
-  const double v0 = c < N ? V6[c] : NAN;
-  if (v0 == v0) {
-    double a[35], mr[35], v6[6], v[6];
-#pragma unroll
-    for (int t = 0; t < 35; ++t) a[t] = AMr[(size_t)t * N + c];
-    v6[0] = v0;
-#pragma unroll
-    for (int d = 1; d < 6; ++d) v6[d] = V6[(size_t)d * N + c];
-    uint32_t fl = flags[c];
-    pass1_finish<float>(a, flag2D, Ma, mr, F + c, F + (size_t)35 * N + c, F + (size_t)70 * N + c,
-                        (size_t)N, v6, v, fl);
-#pragma unroll
-    for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = mr[t];
-#pragma unroll
-    for (int d = 0; d < 6; ++d) {
-      vp[(size_t)d * N + c] = (float)v[d];
-      const unsigned long long b = __double_as_longlong(fabs(v[d]));
-      vb = b > vb ? b : vb;
-    }
-    flags[c] = fl;
-  }
-  vb = wave_max_u64(vb);
-  if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
-}
-
-// The finishing half of the split pass 1 in two kernels (HYQ_SPLIT1 = 2): the closure's
-// projection, whose state (ClosureState, 35 values) overwrites A in Mr, then the 5th-order
-// closure, fluxes and the combined wave speeds.  Deferred cells (V6[0] = NaN) are skipped.
-__global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_project(
-    double* __restrict__ AMr, uint32_t* __restrict__ flags, const double* __restrict__ V6,
-    DevScalars* sc, long long N, int flag2D, double Ma) {
-  if (sc->active == 0.0) return;
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= N) return;
-  const double v0 = V6[c];
-  if (v0 != v0) return;
-  double a[35], v[35];
-#pragma unroll
-  for (int t = 0; t < 35; ++t) a[t] = AMr[(size_t)t * N + c];
-  uint32_t fl = flags[c];
-  ClosureState q;
-  closure_project(a, flag2D, Ma, q, fl);
-  closure_state_store(q, v);
-#pragma unroll
-  for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = v[t];
-  flags[c] = fl;
-}
-
-__global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_emit(
-    double* __restrict__ AMr, float* __restrict__ F, float* __restrict__ vp,
-    const double* __restrict__ V6, DevScalars* sc, long long N) {
-  if (sc->active == 0.0) return;
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long vb = 0;
-  const double v0 = c < N ? V6[c] : NAN;
-  if (v0 == v0) {
-    double w[35], mr[35], v6[6], v[6];
-#pragma unroll
-    for (int t = 0; t < 35; ++t) w[t] = AMr[(size_t)t * N + c];
-    ClosureState q;
-    closure_state_load(w, q);
-    closure_emit<true, float>(q, mr, F + c, F + (size_t)35 * N + c, F + (size_t)70 * N + c,
-                              (size_t)N);
-    v6[0] = v0;
-#pragma unroll
-    for (int d = 1; d < 6; ++d) v6[d] = V6[(size_t)d * N + c];
-    combine_speeds(mr, v6, v);
-#pragma unroll
-    for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = mr[t];
-#pragma unroll
-    for (int d = 0; d < 6; ++d) {
-      vp[(size_t)d * N + c] = (float)v[d];
-      const unsigned long long b = __double_as_longlong(fabs(v[d]));
-      vb = b > vb ? b : vb;
-    }
-  }
-  vb = wave_max_u64(vb);
-  if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
-}
-
 // Pass 1 of the cells that were still in the previous step's pass-2 list when the fast
 // pass 1 ran (the analytic start; failures go to the pass-1 list).  Then k_reset2 empties the
 // pass-2 list.
@@ -863,7 +738,6 @@ struct hyq_ctx {
   float* X;      // [5N][16] plane features
   float* P;      // [5N][2] predictions
   int *list1, *list2;  // cells deferred to the full pass 1 / pass 2 (fast modes)
-  double* V6;          // [6][N] 6x6 wave speeds between the split pass-1 kernels
   bool overlap;            // the pass-2 list kernel runs on stream2 (fast modes)
   hipStream_t stream2;
   hipEvent_t evA, evB;     // pass 2 checked (stream -> stream2) / list done (stream2 -> stream)
@@ -908,9 +782,8 @@ int hyq_create(const hyq_params* params, hyq_ctx** out) {
          hipMalloc(&c->P, sizeof(float) * 2 * 5 * c->N) == hipSuccess;
   if (ok && (params->precision == HYQ_PREC_MIXED_NEWTON || params->precision == HYQ_PREC_MIXED_LEARNED))
     ok = hipMalloc(&c->list1, sizeof(int) * c->N) == hipSuccess &&
-         hipMalloc(&c->list2, sizeof(int) * c->N) == hipSuccess &&
-         (!kSplit1 || hipMalloc(&c->V6, sizeof(double) * 6 * c->N) == hipSuccess);
-  c->overlap = ok && kOverlap && kDefer && kSplit2 && !kSplit1 &&
+         hipMalloc(&c->list2, sizeof(int) * c->N) == hipSuccess;
+  c->overlap = ok && kOverlap && kDefer && kSplit2 &&
                (p.precision == HYQ_PREC_MIXED_NEWTON || p.precision == HYQ_PREC_MIXED_LEARNED);
   if (c->overlap)
     ok = hipMalloc(&c->pend, c->N) == hipSuccess &&
@@ -945,7 +818,6 @@ int hyq_destroy(hyq_ctx* c) {
   hipFree(c->P);
   hipFree(c->list1);
   hipFree(c->list2);
-  hipFree(c->V6);
   for (int i = 0; i < kNev; ++i)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -991,36 +863,7 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
   } else {
     // fast modes: the common cells in one kernel, the deferred ones in a second
     const bool learned = p.precision == HYQ_PREC_MIXED_LEARNED && c->mlp != nullptr;
-    if (kSplit1) {
-      // projection (+ features), network, 6x6 wave speeds, closure and fluxes
-      if (learned) {
-        hipLaunchKernelGGL(k_pass1a<true>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
-                           c->X, c->flags, c->sc, c->N, p.flag2D, p.Ma,
-                           (const unsigned char*)nullptr);
-        const int rc = hyq_mlp_infer_masked_async(c->mlp, c->X, 5 * c->N, c->P, c->stream);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_pass1_speeds<true>, dim3(g), dim3(kBlock), 0, c->stream,
-                           (const double*)c->Mr, c->flags, (const float*)c->P, c->V6, c->list1,
-                           c->sc, c->N, p.positivity);
-      } else {
-        hipLaunchKernelGGL(k_pass1a<false>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
-                           c->X, c->flags, c->sc, c->N, p.flag2D, p.Ma,
-                           (const unsigned char*)nullptr);
-        hipLaunchKernelGGL(k_pass1_speeds<false>, dim3(g), dim3(kBlock), 0, c->stream,
-                           (const double*)c->Mr, c->flags, (const float*)nullptr, c->V6,
-                           c->list1, c->sc, c->N, p.positivity);
-      }
-      if (kSplitClosure) {
-        hipLaunchKernelGGL(k_pass1_project, dim3(g), dim3(kBlock), 0, c->stream, c->Mr, c->flags,
-                           (const double*)c->V6, c->sc, c->N, p.flag2D, p.Ma);
-        hipLaunchKernelGGL(k_pass1_emit, dim3(g), dim3(kBlock), 0, c->stream, c->Mr,
-                           (float*)c->F, (float*)c->vp, (const double*)c->V6, c->sc, c->N);
-      } else {
-        hipLaunchKernelGGL(k_pass1_finish, dim3(g), dim3(kBlock), 0, c->stream, c->Mr,
-                           (float*)c->F, (float*)c->vp, c->flags, (const double*)c->V6, c->sc,
-                           c->N, p.flag2D, p.Ma);
-      }
-    } else if (learned) {
+    if (learned) {
       // projection + features, then the network's warm starts
       hipLaunchKernelGGL(k_pass1a<true>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr, c->X,
                          c->flags, c->sc, c->N, p.flag2D, p.Ma, pend);

@@ -348,15 +348,10 @@ HD bool fast_cell_speeds(const double* A, bool positivity, bool need_vals, const
 
 // The rest of pass 1 for a cell that took none of the rare branches: closure and fluxes of
 // A and the 1D speeds, combined with the 6x6 speeds v6.
-HD void combine_speeds(const double* Mr, const double v6[6], double vp[6]);
 template <typename FT>
 HD void pass1_finish(const double* A, int flag2D, double Ma, double* Mr, FT* Fx, FT* Fy, FT* Fz,
                      size_t st, const double v6[6], double vp[6], uint32_t& fl) {
   flux_closure35<true, FT>(A, flag2D, Ma, Mr, Fx, Fy, Fz, st, fl);
-  combine_speeds(Mr, v6, vp);
-}
-// The 1D closure speeds of the final state Mr, combined with the 6x6 speeds v6.
-HD void combine_speeds(const double* Mr, const double v6[6], double vp[6]) {
   double a, b;
   closure1d_speeds(Mr[0], Mr[1], Mr[2], Mr[3], Mr[4], a, b);
   vp[0] = jmin(a, v6[0]);
