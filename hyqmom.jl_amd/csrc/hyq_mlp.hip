@@ -1,6 +1,7 @@
 // Learned warm start: a 15 -> 128 -> 128 -> 128 -> 2 MLP (tanh-form GELU) that predicts the extreme
 // eigenvalues of the standardized flux Jacobian from the 15 standardized moments of a plane.
-// Matrix products run on MFMA (v_mfma_f32_16x16x16_bf16: bf16 operands, FP32 accumulate);
+// Matrix products run on MFMA (v_mfma_f32_16x16x32_bf16, the double-rate bf16 instruction of
+// gfx950: bf16 operands, FP32 accumulate);
 // weights are kept as FP32 master copies and rounded to bf16 as they are loaded.  Training
 // (forward, backward, Adam) and inference use the kernels of this file only.
 //
@@ -41,8 +42,17 @@ constexpr size_t OFF_W1 = 0, OFF_B1 = OFF_W1 + IN * HID, OFF_W2 = OFF_B1 + HID,
 constexpr size_t OFF_XMU = NPARAM, OFF_XSD = OFF_XMU + 16, OFF_YMU = OFF_XSD + 16,
                  OFF_YSD = OFF_YMU + 2, NBLOB = OFF_YSD + 2;
 
-typedef short s4 __attribute__((ext_vector_type(4)));
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+
+// 16x16x32 bf16 MFMA: lane l gives A row l%16 and B column l%16 with eight k values each
+// (this file uses k = 8(l/16)..8(l/16)+7 of the 32-wide k step), and receives D rows
+// 4(l/16)..4(l/16)+3 of column l%16.
+__device__ __forceinline__ f4 mfma32(s8 a, s8 b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, a),
+                                                 __builtin_bit_cast(bf8v, b), c, 0, 0, 0);
+}
 
 __device__ __forceinline__ short f2bf(float f) {  // round to nearest even (v_cvt_pk_bf16_f32)
   return __builtin_bit_cast(short, (__bf16)f);
@@ -62,7 +72,7 @@ __device__ __forceinline__ float dgelu(float z) {
 
 // C[M x N] = A[M x K] . B[K x N] (+ bias), A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn].
 // One wave computes rows [16 r, 16 r + 16) x columns [16 NT c, 16 NT (c+1)); lane l feeds
-// A rows l%16 and B columns l%16 with the four k values 4(l/16)..+3 of each 16-wide k step,
+// A rows l%16 and B columns l%16 with the eight k values 8(l/16)..+7 of each 32-wide k step,
 // and receives D rows 4(l/16)..+3 of column l%16.
 // Epilogue: act 0 = none, 1 = GELU; Z gets the pre-activation, H the activation (either may
 // be null); stores only rows < M and columns < Nst.
@@ -80,25 +90,25 @@ __global__ __launch_bounds__(256) void k_gemm(const float* __restrict__ A, long 
   f4 acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    s4 a;
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    s8 a;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int k = k0 + 4 * q + e;
+    for (int e = 0; e < 8; ++e) {
+      const int k = k0 + 8 * q + e;
       const float v = (row < M && k < K) ? A[row * sam + (long)k * sak] : 0.f;
       a[e] = f2bf(v);
     }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       const long col = (long)(ct * NT + t) * 16 + r;
-      s4 b;
+      s8 b;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int k = k0 + 4 * q + e;
+      for (int e = 0; e < 8; ++e) {
+        const int k = k0 + 8 * q + e;
         const float v = (col < N && k < K) ? B[(long)k * sbk + col * sbn] : 0.f;
         b[e] = f2bf(v);
       }
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+      acc[t] = mfma32(a, b, acc[t]);
     }
   }
 #pragma unroll
@@ -200,15 +210,15 @@ __global__ void k_denorm(const float* __restrict__ Y, const float* __restrict__ 
 // per-wave LDS buffer between layers (bias, GELU and bf16 rounding applied on the way), and
 // writes only the two de-normalized outputs.  HBM traffic per sample: 64 B in, 8 B out.
 //
-// Packed weight layout: for k-step ks (16 rows of W) and n-tile t (16 columns), chunk
-// (ks * NT + t) holds, for lane l, the 4 bf16 W[16 ks + 4 (l / 16) + e][16 t + l % 16],
-// e = 0..3 -- one 8-byte LDS read per lane per MFMA.
-constexpr int PK_W1 = 0;                      // 1 k-step x 8 tiles
-constexpr int PK_W2 = PK_W1 + 1 * 8 * 256;    // 8 x 8 chunks of 256 bf16
-constexpr int PK_W3 = PK_W2 + 8 * 8 * 256;
-constexpr int PK_W4 = PK_W3 + 8 * 8 * 256;    // 8 k-steps x 1 tile
-constexpr int PK_N = PK_W4 + 8 * 1 * 256;     // bf16 elements
-constexpr int ACT_LD = HID + 4;               // bf16 row pitch of the activation tile
+// Packed weight layout: for k-step ks (32 rows of W) and n-tile t (16 columns), chunk
+// (ks * NT + t) holds, for lane l, the 8 bf16 W[32 ks + 8 (l / 16) + e][16 t + l % 16],
+// e = 0..7 (zero for rows past the layer's K) -- one 16-byte LDS read per lane per MFMA.
+constexpr int PK_W1 = 0;                      // 1 k-step (16 rows, padded) x 8 tiles
+constexpr int PK_W2 = PK_W1 + 1 * 8 * 512;    // 4 x 8 chunks of 512 bf16
+constexpr int PK_W3 = PK_W2 + 4 * 8 * 512;
+constexpr int PK_W4 = PK_W3 + 4 * 8 * 512;    // 4 k-steps x 1 tile
+constexpr int PK_N = PK_W4 + 4 * 1 * 512;     // bf16 elements
+constexpr int ACT_LD = HID + 8;               // bf16 row pitch of the activation tile (16 B aligned)
 
 __global__ void k_pack(const float* __restrict__ p, short* __restrict__ pk) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -219,12 +229,10 @@ __global__ void k_pack(const float* __restrict__ p, short* __restrict__ pk) {
   else if (i < PK_W3) { base = PK_W2; K = HID; NT = 8; W = p + OFF_W2; }
   else if (i < PK_W4) { base = PK_W3; K = HID; NT = 8; W = p + OFF_W3; }
   else { base = PK_W4; K = HID; NT = 1; W = p + OFF_W4; }
-  const int o = i - base, chunk = o / 256, r = o % 256, lane = r / 4, e = r % 4;
+  const int o = i - base, chunk = o / 512, r = o % 512, lane = r / 8, e = r % 8;
   const int ks = chunk / NT, t = chunk % NT;
-  const int row = 16 * ks + 4 * (lane / 16) + e, col = 16 * t + lane % 16;
-  const int ncol = NT * 16;
-  (void)K;
-  pk[i] = f2bf(W[row * ncol + col]);
+  const int row = 32 * ks + 8 * (lane / 16) + e, col = 16 * t + lane % 16;
+  pk[i] = row < K ? f2bf(W[row * (NT * 16) + col]) : (short)0;
 }
 
 
@@ -243,12 +251,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
                                                           const float* __restrict__ p,
                                                           const short* __restrict__ pk,
                                                           float* __restrict__ P) {
-  __shared__ s4 wts[PK_N / 4];
+  __shared__ s8 wts[PK_N / 8];
   __shared__ float bias[3 * HID + 2];
   __shared__ __align__(16) short act[WAVES][RT * 16 * ACT_LD];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const s4* pk4 = reinterpret_cast<const s4*>(pk);
-  for (int i = tid; i < PK_N / 4; i += 64 * WAVES) wts[i] = pk4[i];
+  const s8* pk8 = reinterpret_cast<const s8*>(pk);
+  for (int i = tid; i < PK_N / 8; i += 64 * WAVES) wts[i] = pk8[i];
   for (int i = tid; i < HID; i += 64 * WAVES) {
     bias[i] = p[OFF_B1 + i];
     bias[HID + i] = p[OFF_B2 + i];
@@ -279,22 +287,21 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
       }
     }
     f4 acc[RT][8];
-    // layer 1: normalized input (k = 4q..4q+3 of 16) times W1
+    // layer 1: normalized input (k = 8q..8q+7 of the 32-wide step; features 0..14) times W1
 #pragma unroll
     for (int u = 0; u < RT; ++u) {
       const long row = row0 + 16 * u + r;
-      s4 a;
+      s8 a;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int k = 4 * q + e;
+      for (int e = 0; e < 8; ++e) {
+        const int k = 8 * q + e;
         float v = 0.f;
         if (row < n && k < 15) v = (X[row * 16 + k] - p[OFF_XMU + k]) / p[OFF_XSD + k];
         a[e] = f2bf(v);
       }
 #pragma unroll
       for (int t = 0; t < 8; ++t)
-        acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wts[(PK_W1 / 4) + t * 64 + lane],
-                                                              f4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        acc[u][t] = mfma32(a, wts[(PK_W1 / 8) + t * 64 + lane], f4{0.f, 0.f, 0.f, 0.f});
     }
 #pragma unroll
     for (int u = 0; u < RT; ++u)
@@ -307,23 +314,22 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
     // layers 2 and 3
 #pragma unroll
     for (int L = 0; L < 2; ++L) {
-      const int wb = (L == 0 ? PK_W2 : PK_W3) / 4;
+      const int wb = (L == 0 ? PK_W2 : PK_W3) / 8;
 #pragma unroll
       for (int u = 0; u < RT; ++u)
 #pragma unroll
         for (int t = 0; t < 8; ++t) acc[u][t] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ks = 0; ks < 8; ++ks) {
-        s4 av[RT];
+      for (int ks = 0; ks < 4; ++ks) {
+        s8 av[RT];
 #pragma unroll
         for (int u = 0; u < RT; ++u)
-          av[u] = *reinterpret_cast<const s4*>(&A[(16 * u + r) * ACT_LD + 16 * ks + 4 * q]);
+          av[u] = *reinterpret_cast<const s8*>(&A[(16 * u + r) * ACT_LD + 32 * ks + 8 * q]);
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
-          const s4 bv = wts[wb + (ks * 8 + t) * 64 + lane];
+          const s8 bv = wts[wb + (ks * 8 + t) * 64 + lane];
 #pragma unroll
-          for (int u = 0; u < RT; ++u)
-            acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[u], bv, acc[u][t], 0, 0, 0);
+          for (int u = 0; u < RT; ++u) acc[u][t] = mfma32(av[u], bv, acc[u][t]);
         }
       }
       wave_lds_fence();  // every lane has read the tile before it is overwritten
@@ -341,9 +347,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
     for (int u = 0; u < RT; ++u) {
       f4 o = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ks = 0; ks < 8; ++ks) {
-        const s4 av = *reinterpret_cast<const s4*>(&A[(16 * u + r) * ACT_LD + 16 * ks + 4 * q]);
-        o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, wts[(PK_W4 / 4) + ks * 64 + lane], o, 0, 0, 0);
+      for (int ks = 0; ks < 4; ++ks) {
+        const s8 av = *reinterpret_cast<const s8*>(&A[(16 * u + r) * ACT_LD + 32 * ks + 8 * q]);
+        o = mfma32(av, wts[(PK_W4 / 8) + ks * 64 + lane], o);
       }
       if (r < 2) {
         const float mu = p[OFF_YMU + r], sd = p[OFF_YSD + r], b = bias[3 * HID + r];
@@ -358,144 +364,20 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
   }
 }
 
-// The same network computed transposed, Y^T = W^T X^T, with the packed weight tiles as the
-// A operand.  MFMA 16x16x16 lane layouts: the A operand of lane l holds row l%16 and
-// k = 4(l/16)+e, the B operand column l%16 and k = 4(l/16)+e, the result rows 4(l/16)+e of
-// column l%16.  A packed tile of W (k-step ks, column tile t) therefore is the A operand of
-// the W^T tile (t, ks) unchanged, and the result of column tile t (features 16t + 4(l/16)+e
-// of sample l%16) is, in bf16, exactly the B operand of k-step t of the next layer: the
-// activations stay in registers and the hidden layers need no LDS round trip.  Only the
-// weights sit in LDS, so two blocks fit on a CU.
-template <int WAVES, int RT, bool MASK>
-__global__ __launch_bounds__(64 * WAVES) void k_mlp_fused_t(const float* __restrict__ X, long n,
-                                                            const float* __restrict__ p,
-                                                            const short* __restrict__ pk,
-                                                            float* __restrict__ P) {
-  __shared__ s4 wts[PK_N / 4];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const s4* pk4 = reinterpret_cast<const s4*>(pk);
-  for (int i = tid; i < PK_N / 4; i += 64 * WAVES) wts[i] = pk4[i];
-  __syncthreads();
-  const int r = lane & 15, q = lane >> 4;
-  const long ngroups = (n + 16 * RT - 1) / (16 * RT);
-  for (long grp = (long)blockIdx.x * WAVES + w; grp < ngroups; grp += (long)gridDim.x * WAVES) {
-    const long row0 = grp * 16 * RT;
-    {
-      // (without MASK the test never skips: the group's first row exists; the branch is kept
-      // so that both instantiations get the same control flow)
-      const long row = row0 + lane;
-      const bool need = lane < 16 * RT && row < n && (!MASK || X[row * 16 + 15] != 0.f);
-      if (__ballot(need) == 0ull) {
-        if (lane < 16 * RT && row < n) {
-          P[row * 2] = NAN;
-          P[row * 2 + 1] = NAN;
-        }
-        continue;
-      }
-    }
-    // layer 1: B operand = normalized input of sample r, features 4q..4q+3
-    s4 h[RT][8];
-    {
-      s4 x[RT];
-#pragma unroll
-      for (int u = 0; u < RT; ++u) {
-        const long row = row0 + 16 * u + r;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int k = 4 * q + e;
-          float v = 0.f;
-          if (row < n && k < 15) v = (X[row * 16 + k] - p[OFF_XMU + k]) / p[OFF_XSD + k];
-          x[u][e] = f2bf(v);
-        }
-      }
-#pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        const s4 wv = wts[(PK_W1 / 4) + t * 64 + lane];
-        float b[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) b[e] = p[OFF_B1 + 16 * t + 4 * q + e];
-#pragma unroll
-        for (int u = 0; u < RT; ++u) {
-          const f4 y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wv, x[u], f4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-#pragma unroll
-          for (int e = 0; e < 4; ++e) h[u][t][e] = f2bf(gelu(y[e] + b[e]));
-        }
-      }
-    }
-    // layers 2 and 3
-#pragma unroll
-    for (int L = 0; L < 2; ++L) {
-      const int wb = (L == 0 ? PK_W2 : PK_W3) / 4;
-      const float* bl = p + (L == 0 ? OFF_B2 : OFF_B3);
-      s4 g[RT][8];
-#pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        f4 acc[RT];
-#pragma unroll
-        for (int u = 0; u < RT; ++u) acc[u] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int ks = 0; ks < 8; ++ks) {
-          const s4 wv = wts[wb + (ks * 8 + t) * 64 + lane];
-#pragma unroll
-          for (int u = 0; u < RT; ++u)
-            acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wv, h[u][ks], acc[u], 0, 0, 0);
-        }
-        float b[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) b[e] = bl[16 * t + 4 * q + e];
-#pragma unroll
-        for (int u = 0; u < RT; ++u)
-#pragma unroll
-          for (int e = 0; e < 4; ++e) g[u][t][e] = f2bf(gelu(acc[u][e] + b[e]));
-      }
-#pragma unroll
-      for (int u = 0; u < RT; ++u)
-#pragma unroll
-        for (int t = 0; t < 8; ++t) h[u][t] = g[u][t];
-    }
-    // layer 4: outputs 4q+e of sample r; outputs 0 and 1 are on the lanes with q = 0
-#pragma unroll
-    for (int u = 0; u < RT; ++u) {
-      f4 o = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int ks = 0; ks < 8; ++ks)
-        o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wts[(PK_W4 / 4) + ks * 64 + lane], h[u][ks], o, 0, 0, 0);
-      const long row = row0 + 16 * u + r;
-      if (q == 0 && row < n) {
-#pragma unroll
-        for (int e = 0; e < 2; ++e)
-          P[row * 2 + e] = (o[e] + p[OFF_B4 + e]) * p[OFF_YSD + e] + p[OFF_YMU + e];
-      }
-    }
-  }
-}
-
 // Launch shapes of the fused inference (waves per block, row tiles per wave), both within
-// the 160 KB of LDS per CU next to the 72 KB of packed weights: 16 x 1 (4 waves per SIMD)
+// the 160 KB of LDS per CU next to the 76 KB of packed weights: 16 x 1 (4 waves per SIMD)
 // and 8 x 2 (2 waves per SIMD, each B operand feeding two MFMAs).  They compute the same
 // values in the same order; the first large inference times both and keeps the faster.
-constexpr int kFusedShapes = 5;
+constexpr int kFusedShapes = 2;
 void launch_fused(int shape, bool mask, const float* X, long n, const float* p, const short* pk,
                   float* P, hipStream_t s) {
-  const int waves = (shape == 0 || shape == 4) ? 16 : 8;
-  const int rt = (shape == 0 || shape == 2 || shape == 4) ? 1 : 2;
+  const int waves = shape == 0 ? 16 : 8;
+  const int rt = shape == 0 ? 1 : 2;
   const long groups = (n + 16 * rt - 1) / (16 * rt);
   long blocks = (groups + waves - 1) / waves;
   if (blocks > 1024) blocks = 1024;
   const dim3 g((unsigned)blocks);
-  if (shape == 4 && !mask)
-    hipLaunchKernelGGL((k_mlp_fused_t<16, 1, false>), g, dim3(64 * 16), 0, s, X, n, p, pk, P);
-  else if (shape == 4)
-    hipLaunchKernelGGL((k_mlp_fused_t<16, 1, true>), g, dim3(64 * 16), 0, s, X, n, p, pk, P);
-  else if (shape == 2 && !mask)
-    hipLaunchKernelGGL((k_mlp_fused_t<8, 1, false>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
-  else if (shape == 2)
-    hipLaunchKernelGGL((k_mlp_fused_t<8, 1, true>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
-  else if (shape == 3 && !mask)
-    hipLaunchKernelGGL((k_mlp_fused_t<8, 2, false>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
-  else if (shape == 3)
-    hipLaunchKernelGGL((k_mlp_fused_t<8, 2, true>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
-  else if (shape == 0 && !mask)
+  if (shape == 0 && !mask)
     hipLaunchKernelGGL((k_mlp_fused<16, 1, false>), g, dim3(64 * 16), 0, s, X, n, p, pk, P);
   else if (shape == 0)
     hipLaunchKernelGGL((k_mlp_fused<16, 1, true>), g, dim3(64 * 16), 0, s, X, n, p, pk, P);

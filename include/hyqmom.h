@@ -175,9 +175,8 @@ int hyq_mlp_infer(hyq_mlp* m, const float* X, int64_t n, float* P);
 /* the same through the layer-by-layer training kernels (bitwise equal; a cross-check) */
 int hyq_mlp_infer_layers(hyq_mlp* m, const float* X, int64_t n, float* P);
 /* launch shape of the fused inference: -1 (the default) times the shapes on the first
- * inference of 65536 rows or more and keeps the fastest; 0..4 fixes one (0: 16 waves x 1
- * tile, 1: 8 x 2, activations through LDS; 2: 8 x 1, 3: 8 x 2, 4: 16 x 1, the transposed
- * form with the activations in registers).  All give the same values.  No reference
+ * inference of 65536 rows or more and keeps the faster; 0 or 1 fixes one (0: 16 waves x 1
+ * row tile, 1: 8 waves x 2 row tiles).  Both give the same values.  No reference
  * counterpart. */
 int hyq_mlp_set_shape(hyq_mlp* m, int32_t shape);
 /* loss and gradients (grad_host: hyq_mlp_param_count() floats, may be NULL); T [n][2]. */

@@ -10,7 +10,7 @@ replaces by the analytic bound.  This module restates, in numpy, what the device
   loss = 1/(2n) sum_b sum_{o<2} (Y[b, o] - (T[b, o] - ymu[o]) / ysd[o])^2
 
 with every matrix-product operand rounded to bfloat16 (round to nearest even) and products
-accumulated in high precision, as the MFMA v_mfma_f32_16x16x16_bf16 instructions do (FP32
+accumulated in high precision, as the MFMA v_mfma_f32_16x16x32_bf16 instructions do (FP32
 accumulate).  g is the tanh form of GELU, z sigmoid(2 sqrt(2/pi) (z + 0.044715 z^3)).
 With bf16=False everything stays in FP64, which the finite-difference check of the backward
 pass uses.

@@ -57,10 +57,10 @@ def test_fused_inference_equals_layers(n):
     assert not np.array_equal(a, c)
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("shape", [0, 1])
 def test_every_fused_shape_equals_layers(shape):
-    """Each launch shape of the fused inference, the transposed ones (activations kept in
-    registers) included, gives the layer-by-layer predictions bit for bit, masked rows aside:
+    """Each launch shape of the fused inference gives the layer-by-layer predictions bit for
+    bit, masked rows aside:
     with the mask a 16-row tile whose rows all have column 15 = 0 gets NaN."""
     from hyqmom_amd import _lib
     from hyqmom_amd.warmstart import WarmStartMLP
