@@ -1,14 +1,15 @@
-// Learned warm start: a 15 -> 128 -> 128 -> 128 -> 2 MLP (tanh-form GELU) that predicts the extreme
-// eigenvalues of the standardized flux Jacobian from the 15 standardized moments of a plane.
+// Learned warm start: a 15 -> H -> H -> H -> 2 MLP (tanh-form GELU; H = HYQ_MLP_HID, 32 by
+// default) that predicts the extreme eigenvalues of the standardized flux Jacobian from the 15
+// standardized moments of a plane.
 // Matrix products run on MFMA (v_mfma_f32_16x16x32_bf16, the double-rate bf16 instruction of
 // gfx950: bf16 operands, FP32 accumulate);
 // weights are kept as FP32 master copies and rounded to bf16 as they are loaded.  Training
 // (forward, backward, Adam) and inference use the kernels of this file only.
 //
-// Parameter block (FP32, this order): W1 [16][128] (input padded to 16), b1 [128],
-// W2 [128][128], b2 [128], W3 [128][128], b3 [128], W4 [128][16] (outputs padded to 16,
-// columns >= 2 held at zero), b4 [16], then input mean/std [16]+[16] and output mean/std
-// [2]+[2] used to normalize features and labels.
+// Parameter block (FP32, this order): W1 [16][H] (input padded to 16), b1 [H], W2 [H][H],
+// b2 [H], W3 [H][H], b3 [H], W4 [H][16] (outputs padded to 16, columns >= 2 held at zero),
+// b4 [16], then input mean/std [16]+[16] and output mean/std [2]+[2] used to normalize
+// features and labels.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -35,7 +36,13 @@ int mfail(int code, const char* what, hipError_t e = hipSuccess) {
     if (_e != hipSuccess) return mfail(HYQ_EHIP, msg, _e); \
   } while (0)
 
-constexpr int IN = 16, HID = 128, OUT = 16;
+#ifndef HYQ_MLP_HID
+#define HYQ_MLP_HID 32
+#endif
+constexpr int IN = 16, HID = HYQ_MLP_HID, OUT = 16;
+static_assert(HID % 32 == 0 && HID >= 32 && HID <= 128, "hidden width: 32, 64, 96 or 128");
+constexpr int NTH = HID / 16;  // 16-column tiles of a hidden layer
+constexpr int KSH = HID / 32;  // 32-deep k steps over a hidden layer
 constexpr size_t OFF_W1 = 0, OFF_B1 = OFF_W1 + IN * HID, OFF_W2 = OFF_B1 + HID,
                  OFF_B2 = OFF_W2 + HID * HID, OFF_W3 = OFF_B2 + HID, OFF_B3 = OFF_W3 + HID * HID,
                  OFF_W4 = OFF_B3 + HID, OFF_B4 = OFF_W4 + HID * OUT, NPARAM = OFF_B4 + OUT;
@@ -213,11 +220,11 @@ __global__ void k_denorm(const float* __restrict__ Y, const float* __restrict__ 
 // Packed weight layout: for k-step ks (32 rows of W) and n-tile t (16 columns), chunk
 // (ks * NT + t) holds, for lane l, the 8 bf16 W[32 ks + 8 (l / 16) + e][16 t + l % 16],
 // e = 0..7 (zero for rows past the layer's K) -- one 16-byte LDS read per lane per MFMA.
-constexpr int PK_W1 = 0;                      // 1 k-step (16 rows, padded) x 8 tiles
-constexpr int PK_W2 = PK_W1 + 1 * 8 * 512;    // 4 x 8 chunks of 512 bf16
-constexpr int PK_W3 = PK_W2 + 4 * 8 * 512;
-constexpr int PK_W4 = PK_W3 + 4 * 8 * 512;    // 4 k-steps x 1 tile
-constexpr int PK_N = PK_W4 + 4 * 1 * 512;     // bf16 elements
+constexpr int PK_W1 = 0;                        // 1 k-step (16 rows, padded) x NTH tiles
+constexpr int PK_W2 = PK_W1 + 1 * NTH * 512;    // KSH x NTH chunks of 512 bf16
+constexpr int PK_W3 = PK_W2 + KSH * NTH * 512;
+constexpr int PK_W4 = PK_W3 + KSH * NTH * 512;  // KSH k-steps x 1 tile
+constexpr int PK_N = PK_W4 + KSH * 1 * 512;     // bf16 elements
 constexpr int ACT_LD = HID + 8;               // bf16 row pitch of the activation tile (16 B aligned)
 
 __global__ void k_pack(const float* __restrict__ p, short* __restrict__ pk) {
@@ -225,9 +232,9 @@ __global__ void k_pack(const float* __restrict__ p, short* __restrict__ pk) {
   if (i >= PK_N) return;
   int base, K, NT;
   const float* W;
-  if (i < PK_W2) { base = PK_W1; K = IN; NT = 8; W = p + OFF_W1; }
-  else if (i < PK_W3) { base = PK_W2; K = HID; NT = 8; W = p + OFF_W2; }
-  else if (i < PK_W4) { base = PK_W3; K = HID; NT = 8; W = p + OFF_W3; }
+  if (i < PK_W2) { base = PK_W1; K = IN; NT = NTH; W = p + OFF_W1; }
+  else if (i < PK_W3) { base = PK_W2; K = HID; NT = NTH; W = p + OFF_W2; }
+  else if (i < PK_W4) { base = PK_W3; K = HID; NT = NTH; W = p + OFF_W3; }
   else { base = PK_W4; K = HID; NT = 1; W = p + OFF_W4; }
   const int o = i - base, chunk = o / 512, r = o % 512, lane = r / 8, e = r % 8;
   const int ks = chunk / NT, t = chunk % NT;
@@ -266,11 +273,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
   __syncthreads();
   const int r = lane & 15, q = lane >> 4;
   // this lane's bias values: column 16 t + r of each hidden layer
-  float bl[3][8];
+  float bl[3][NTH];
 #pragma unroll
   for (int L = 0; L < 3; ++L)
 #pragma unroll
-    for (int t = 0; t < 8; ++t) bl[L][t] = bias[HID * L + 16 * t + r];
+    for (int t = 0; t < NTH; ++t) bl[L][t] = bias[HID * L + 16 * t + r];
   const long ngroups = (n + 16 * RT - 1) / (16 * RT);  // RT-tile groups
   short* A = act[w];
   for (long grp = (long)blockIdx.x * WAVES + w; grp < ngroups; grp += (long)gridDim.x * WAVES) {
@@ -286,7 +293,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
         continue;
       }
     }
-    f4 acc[RT][8];
+    f4 acc[RT][NTH];
     // layer 1: normalized input (k = 8q..8q+7 of the 32-wide step; features 0..14) times W1
 #pragma unroll
     for (int u = 0; u < RT; ++u) {
@@ -300,13 +307,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
         a[e] = f2bf(v);
       }
 #pragma unroll
-      for (int t = 0; t < 8; ++t)
+      for (int t = 0; t < NTH; ++t)
         acc[u][t] = mfma32(a, wts[(PK_W1 / 8) + t * 64 + lane], f4{0.f, 0.f, 0.f, 0.f});
     }
 #pragma unroll
     for (int u = 0; u < RT; ++u)
 #pragma unroll
-      for (int t = 0; t < 8; ++t)
+      for (int t = 0; t < NTH; ++t)
 #pragma unroll
         for (int e = 0; e < 4; ++e)
           A[(16 * u + 4 * q + e) * ACT_LD + 16 * t + r] = f2bf(gelu(acc[u][t][e] + bl[0][t]));
@@ -318,16 +325,16 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
 #pragma unroll
       for (int u = 0; u < RT; ++u)
 #pragma unroll
-        for (int t = 0; t < 8; ++t) acc[u][t] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int t = 0; t < NTH; ++t) acc[u][t] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ks = 0; ks < 4; ++ks) {
+      for (int ks = 0; ks < KSH; ++ks) {
         s8 av[RT];
 #pragma unroll
         for (int u = 0; u < RT; ++u)
           av[u] = *reinterpret_cast<const s8*>(&A[(16 * u + r) * ACT_LD + 32 * ks + 8 * q]);
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          const s8 bv = wts[wb + (ks * 8 + t) * 64 + lane];
+        for (int t = 0; t < NTH; ++t) {
+          const s8 bv = wts[wb + (ks * NTH + t) * 64 + lane];
 #pragma unroll
           for (int u = 0; u < RT; ++u) acc[u][t] = mfma32(av[u], bv, acc[u][t]);
         }
@@ -336,7 +343,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
 #pragma unroll
       for (int u = 0; u < RT; ++u)
 #pragma unroll
-        for (int t = 0; t < 8; ++t)
+        for (int t = 0; t < NTH; ++t)
 #pragma unroll
           for (int e = 0; e < 4; ++e)
             A[(16 * u + 4 * q + e) * ACT_LD + 16 * t + r] = f2bf(gelu(acc[u][t][e] + bl[L + 1][t]));
@@ -347,7 +354,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
     for (int u = 0; u < RT; ++u) {
       f4 o = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ks = 0; ks < 4; ++ks) {
+      for (int ks = 0; ks < KSH; ++ks) {
         const s8 av = *reinterpret_cast<const s8*>(&A[(16 * u + r) * ACT_LD + 32 * ks + 8 * q]);
         o = mfma32(av, wts[(PK_W4 / 8) + ks * 64 + lane], o);
       }
@@ -365,7 +372,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
 }
 
 // Launch shapes of the fused inference (waves per block, row tiles per wave), both within
-// the 160 KB of LDS per CU next to the 76 KB of packed weights: 16 x 1 (4 waves per SIMD)
+// the 160 KB of LDS per CU next to the packed weights: 16 x 1 (4 waves per SIMD)
 // and 8 x 2 (2 waves per SIMD, each B operand feeding two MFMAs).  They compute the same
 // values in the same order; the first large inference times both and keeps the faster.
 constexpr int kFusedShapes = 2;
@@ -448,9 +455,9 @@ static int ensure_cap(hyq_mlp* m, long n) {
 // forward on normalized inputs m->Xn (n rows); fills Z*, H*, Y
 static void forward(hyq_mlp* m, long n) {
   const float* p = m->blob;
-  gemm<8>(m->stream, m->Xn, IN, 1, p + OFF_W1, HID, 1, n, HID, IN, p + OFF_B1, m->Z1, m->H1, HID, HID, 1);
-  gemm<8>(m->stream, m->H1, HID, 1, p + OFF_W2, HID, 1, n, HID, HID, p + OFF_B2, m->Z2, m->H2, HID, HID, 1);
-  gemm<8>(m->stream, m->H2, HID, 1, p + OFF_W3, HID, 1, n, HID, HID, p + OFF_B3, m->Z3, m->H3, HID, HID, 1);
+  gemm<NTH>(m->stream, m->Xn, IN, 1, p + OFF_W1, HID, 1, n, HID, IN, p + OFF_B1, m->Z1, m->H1, HID, HID, 1);
+  gemm<NTH>(m->stream, m->H1, HID, 1, p + OFF_W2, HID, 1, n, HID, HID, p + OFF_B2, m->Z2, m->H2, HID, HID, 1);
+  gemm<NTH>(m->stream, m->H2, HID, 1, p + OFF_W3, HID, 1, n, HID, HID, p + OFF_B3, m->Z3, m->H3, HID, HID, 1);
   gemm<1>(m->stream, m->H3, HID, 1, p + OFF_W4, OUT, 1, n, OUT, HID, p + OFF_B4, nullptr, m->Y, OUT, OUT, 0);
 }
 
@@ -462,20 +469,20 @@ static void backward(hyq_mlp* m, long n) {
   // layer 4
   gemm<1>(s, m->H3, 1, HID, m->dY, OUT, 1, HID, OUT, n, nullptr, nullptr, g + OFF_W4, OUT, OUT, 0);
   hipLaunchKernelGGL(k_colsum, dim3(OUT), dim3(256), 0, s, m->dY, n, OUT, g + OFF_B4);
-  gemm<8>(s, m->dY, OUT, 1, p + OFF_W4, 1, OUT, n, HID, OUT, nullptr, nullptr, m->dA, HID, HID, 0);
+  gemm<NTH>(s, m->dY, OUT, 1, p + OFF_W4, 1, OUT, n, HID, OUT, nullptr, nullptr, m->dA, HID, HID, 0);
   hipLaunchKernelGGL(k_gelu_bwd, dim3(g1(n * HID)), dim3(256), 0, s, m->Z3, m->dA, n * HID);
   // layer 3
-  gemm<8>(s, m->H2, 1, HID, m->dA, HID, 1, HID, HID, n, nullptr, nullptr, g + OFF_W3, HID, HID, 0);
+  gemm<NTH>(s, m->H2, 1, HID, m->dA, HID, 1, HID, HID, n, nullptr, nullptr, g + OFF_W3, HID, HID, 0);
   hipLaunchKernelGGL(k_colsum, dim3(HID), dim3(256), 0, s, m->dA, n, HID, g + OFF_B3);
-  gemm<8>(s, m->dA, HID, 1, p + OFF_W3, 1, HID, n, HID, HID, nullptr, nullptr, m->dB, HID, HID, 0);
+  gemm<NTH>(s, m->dA, HID, 1, p + OFF_W3, 1, HID, n, HID, HID, nullptr, nullptr, m->dB, HID, HID, 0);
   hipLaunchKernelGGL(k_gelu_bwd, dim3(g1(n * HID)), dim3(256), 0, s, m->Z2, m->dB, n * HID);
   // layer 2
-  gemm<8>(s, m->H1, 1, HID, m->dB, HID, 1, HID, HID, n, nullptr, nullptr, g + OFF_W2, HID, HID, 0);
+  gemm<NTH>(s, m->H1, 1, HID, m->dB, HID, 1, HID, HID, n, nullptr, nullptr, g + OFF_W2, HID, HID, 0);
   hipLaunchKernelGGL(k_colsum, dim3(HID), dim3(256), 0, s, m->dB, n, HID, g + OFF_B2);
-  gemm<8>(s, m->dB, HID, 1, p + OFF_W2, 1, HID, n, HID, HID, nullptr, nullptr, m->dA, HID, HID, 0);
+  gemm<NTH>(s, m->dB, HID, 1, p + OFF_W2, 1, HID, n, HID, HID, nullptr, nullptr, m->dA, HID, HID, 0);
   hipLaunchKernelGGL(k_gelu_bwd, dim3(g1(n * HID)), dim3(256), 0, s, m->Z1, m->dA, n * HID);
   // layer 1
-  gemm<8>(s, m->Xn, 1, IN, m->dA, HID, 1, IN, HID, n, nullptr, nullptr, g + OFF_W1, HID, HID, 0);
+  gemm<NTH>(s, m->Xn, 1, IN, m->dA, HID, 1, IN, HID, n, nullptr, nullptr, g + OFF_W1, HID, HID, 0);
   hipLaunchKernelGGL(k_colsum, dim3(HID), dim3(256), 0, s, m->dA, n, HID, g + OFF_B1);
 }
 
@@ -614,9 +621,9 @@ int hyq_mlp_infer_layers(hyq_mlp* m, const float* X, int64_t n, float* P) {
   const float* p = m->blob;
   hipStream_t stream = m->stream;
   hipLaunchKernelGGL(k_normalize, dim3(g1(n * 16)), dim3(256), 0, stream, X, p + OFF_XMU, m->iX, (long)n);
-  gemm<8>(stream, m->iX, IN, 1, p + OFF_W1, HID, 1, n, HID, IN, p + OFF_B1, nullptr, m->iA, HID, HID, 1);
-  gemm<8>(stream, m->iA, HID, 1, p + OFF_W2, HID, 1, n, HID, HID, p + OFF_B2, nullptr, m->iB, HID, HID, 1);
-  gemm<8>(stream, m->iB, HID, 1, p + OFF_W3, HID, 1, n, HID, HID, p + OFF_B3, nullptr, m->iA, HID, HID, 1);
+  gemm<NTH>(stream, m->iX, IN, 1, p + OFF_W1, HID, 1, n, HID, IN, p + OFF_B1, nullptr, m->iA, HID, HID, 1);
+  gemm<NTH>(stream, m->iA, HID, 1, p + OFF_W2, HID, 1, n, HID, HID, p + OFF_B2, nullptr, m->iB, HID, HID, 1);
+  gemm<NTH>(stream, m->iB, HID, 1, p + OFF_W3, HID, 1, n, HID, HID, p + OFF_B3, nullptr, m->iA, HID, HID, 1);
   gemm<1>(stream, m->iA, HID, 1, p + OFF_W4, OUT, 1, n, OUT, HID, p + OFF_B4, nullptr, m->iY, OUT, OUT, 0);
   hipLaunchKernelGGL(k_denorm, dim3(g1(n)), dim3(256), 0, stream, m->iY, p + OFF_YMU, P, (long)n);
   MCHK(hipGetLastError(), "mlp infer launch");

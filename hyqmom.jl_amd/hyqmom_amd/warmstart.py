@@ -1,6 +1,6 @@
-"""The learned warm start of the wave-speed solve: a 15 -> 128 -> 128 -> 128 -> 2 GELU MLP
-on bf16 MFMA that predicts the standardized extreme eigenvalues of each plane's flux
-Jacobian (no counterpart in HyQMOM.jl).  Everything runs in libhyqmom_hip.so; this module
+"""The learned warm start of the wave-speed solve: a 15 -> H -> H -> H -> 2 GELU MLP (H = 32
+as the library is built) on bf16 MFMA that predicts the standardized extreme eigenvalues of
+each plane's flux Jacobian (no counterpart in HyQMOM.jl).  Everything runs in libhyqmom_hip.so; this module
 holds device buffers (torch tensors, plumbing only) and the training loop.
 
 Precision "mixed_learned" uses the predictions as starting points of Laguerre's iteration,
@@ -160,10 +160,20 @@ def fit_normalization(X, L):
     return np.concatenate([xmu, xsd, ymu, ysd]).astype(np.float32)
 
 
+def hidden_width(nparam):
+    """The hidden width H of a network with nparam parameters, 2 H^2 + 35 H + 16 of them
+    (W1 16 x H, three biases of H, W2 and W3 H x H, W4 H x 16, b4 16)."""
+    H = int(round((-35 + np.sqrt(35 * 35 + 8 * (nparam - 16))) / 4))
+    if 2 * H * H + 35 * H + 16 != nparam:
+        raise ValueError("no hidden width has %d parameters" % nparam)
+    return H
+
+
 def init_blob(nparam, norm, seed=0):
     """He-initialized weights (output columns >= 2 zero) with the given normalization."""
     rng = np.random.default_rng(seed)
-    sizes = [(16, 128), (128,), (128, 128), (128,), (128, 128), (128,), (128, 16), (16,)]
+    H = hidden_width(nparam)
+    sizes = [(16, H), (H,), (H, H), (H,), (H, H), (H,), (H, 16), (16,)]
     parts = []
     for s in sizes:
         if len(s) == 2:

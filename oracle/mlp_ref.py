@@ -15,14 +15,14 @@ accumulate).  g is the tanh form of GELU, z sigmoid(2 sqrt(2/pi) (z + 0.044715 z
 With bf16=False everything stays in FP64, which the finite-difference check of the backward
 pass uses.
 
-Parameter blob (FP32): W1 [16][128], b1 [128], W2 [128][128], b2, W3 [128][128], b3,
-W4 [128][16], b4 [16], then xmu [16], xsd [16], ymu [2], ysd [2].
+Parameter blob (FP32), hidden width H = HID (32, as the library is built): W1 [16][H], b1 [H],
+W2 [H][H], b2, W3 [H][H], b3, W4 [H][16], b4 [16], then xmu [16], xsd [16], ymu [2], ysd [2].
 """
 import math
 
 import numpy as np
 
-IN, HID, OUT = 16, 128, 16
+IN, HID, OUT = 16, 32, 16
 SHAPES = [("W1", (IN, HID)), ("b1", (HID,)), ("W2", (HID, HID)), ("b2", (HID,)),
           ("W3", (HID, HID)), ("b3", (HID,)), ("W4", (HID, OUT)), ("b4", (OUT,))]
 NPARAM = sum(int(np.prod(s)) for _, s in SHAPES)

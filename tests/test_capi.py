@@ -37,3 +37,17 @@ def test_invalid_params_rejected():
     p = _lib.HyqParams(Nx=4, Ny=4, Nz=4, Nmom=34)
     h = ctypes.c_void_p()
     assert L.hyq_create(ctypes.byref(p), ctypes.byref(h)) == -1
+
+
+def test_network_width_matches_cpu_reference():
+    """The library's parameter blob has the layout of the CPU reference network (same hidden
+    width), so blobs move between them (a host-only query, no GPU)."""
+    import mlp_ref as R
+    from hyqmom_amd import _lib
+    from hyqmom_amd.warmstart import hidden_width
+    L = _lib.lib()
+    assert L.hyq_mlp_param_count() == R.NPARAM
+    assert L.hyq_mlp_blob_size() == R.NBLOB
+    assert hidden_width(R.NPARAM) == R.HID
+    for H in (32, 64, 96, 128):
+        assert hidden_width(2 * H * H + 35 * H + 16) == H

@@ -19,7 +19,8 @@ def test_bf16_rounding_known_values():
 
 
 def test_blob_layout_sizes():
-    assert R.NPARAM == 16 * 128 + 128 + 2 * (128 * 128 + 128) + 128 * 16 + 16
+    assert R.HID == 32
+    assert R.NPARAM == 16 * 32 + 32 + 2 * (32 * 32 + 32) + 32 * 16 + 16
     assert R.NBLOB == R.NPARAM + 36
     p = R.unpack(R.random_blob(1))
     assert np.all(p["W4"][:, 2:] == 0) and np.all(p["b4"][2:] == 0)
