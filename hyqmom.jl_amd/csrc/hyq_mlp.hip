@@ -279,16 +279,51 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
 #pragma unroll
     for (int t = 0; t < NTH; ++t) bl[L][t] = bias[HID * L + 16 * t + r];
   const long ngroups = (n + 16 * RT - 1) / (16 * RT);  // RT-tile groups
+  const long gstride = (long)gridDim.x * WAVES;
+  // normalization of this lane's input columns k = 8q + e (lanes q < 2; column 15 is the
+  // need flag and enters as 0)
+  float xmu[8], xsd[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int k = 8 * q + e;
+    xmu[e] = k < 15 ? p[OFF_XMU + k] : 0.f;
+    xsd[e] = k < 15 ? p[OFF_XSD + k] : 1.f;
+  }
+  // raw feature rows of the next group, loaded while the current one is computed: lane
+  // (r, q < 2) holds columns 8q..8q+7 of row 16u + r of the group, column 15 included
+  float xn[RT][8];
+  auto load_group = [&](long g) {
+#pragma unroll
+    for (int u = 0; u < RT; ++u) {
+      const long row = g * 16 * RT + 16 * u + r;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) xn[u][e] = (q < 2 && row < n) ? X[row * 16 + 8 * q + e] : 0.f;
+    }
+  };
   short* A = act[w];
-  for (long grp = (long)blockIdx.x * WAVES + w; grp < ngroups; grp += (long)gridDim.x * WAVES) {
+  long grp = (long)blockIdx.x * WAVES + w;
+  if (grp < ngroups) load_group(grp);
+  for (; grp < ngroups; grp += gstride) {
     const long row0 = grp * 16 * RT;
+    float xv[RT][8];
+#pragma unroll
+    for (int u = 0; u < RT; ++u)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) xv[u][e] = xn[u][e];
+    if (grp + gstride < ngroups) load_group(grp + gstride);
     if (MASK) {
-      const long row = row0 + lane;
-      const bool need = lane < 16 * RT && row < n && X[row * 16 + 15] != 0.f;
+      // column 15 of row 16u + r sits in lane (r, q = 1), element 7
+      bool need = false;
+#pragma unroll
+      for (int u = 0; u < RT; ++u) need = need || (q == 1 && row0 + 16 * u + r < n && xv[u][7] != 0.f);
       if (__ballot(need) == 0ull) {
-        if (lane < 16 * RT && row < n) {
-          P[row * 2] = NAN;
-          P[row * 2 + 1] = NAN;
+#pragma unroll
+        for (int u = 0; u < RT; ++u) {
+          const long row = row0 + 16 * u + r;
+          if (q == 0 && row < n) {
+            P[row * 2] = NAN;
+            P[row * 2 + 1] = NAN;
+          }
         }
         continue;
       }
@@ -303,7 +338,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
       for (int e = 0; e < 8; ++e) {
         const int k = 8 * q + e;
         float v = 0.f;
-        if (row < n && k < 15) v = (X[row * 16 + k] - p[OFF_XMU + k]) / p[OFF_XSD + k];
+        if (row < n && k < 15) v = (xv[u][e] - xmu[e]) / xsd[e];
         a[e] = f2bf(v);
       }
 #pragma unroll
@@ -371,27 +406,31 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
   }
 }
 
-// Launch shapes of the fused inference (waves per block, row tiles per wave), both within
-// the 160 KB of LDS per CU next to the packed weights: 16 x 1 (4 waves per SIMD)
-// and 8 x 2 (2 waves per SIMD, each B operand feeding two MFMAs).  They compute the same
-// values in the same order; the first large inference times both and keeps the faster.
-constexpr int kFusedShapes = 2;
+// Launch shapes of the fused inference (waves per block, row tiles per wave): 16 x 1, 8 x 2
+// (each B operand feeding two MFMAs) and 8 x 1 (smaller blocks, so that more of them share a
+// CU).  They compute the same values in the same order; the first large inference times them
+// and keeps the fastest.
+constexpr int kFusedShapes = 3;
 void launch_fused(int shape, bool mask, const float* X, long n, const float* p, const short* pk,
                   float* P, hipStream_t s) {
   const int waves = shape == 0 ? 16 : 8;
-  const int rt = shape == 0 ? 1 : 2;
+  const int rt = shape == 1 ? 2 : 1;
   const long groups = (n + 16 * rt - 1) / (16 * rt);
   long blocks = (groups + waves - 1) / waves;
-  if (blocks > 1024) blocks = 1024;
+  if (blocks > 2048) blocks = 2048;
   const dim3 g((unsigned)blocks);
   if (shape == 0 && !mask)
     hipLaunchKernelGGL((k_mlp_fused<16, 1, false>), g, dim3(64 * 16), 0, s, X, n, p, pk, P);
   else if (shape == 0)
     hipLaunchKernelGGL((k_mlp_fused<16, 1, true>), g, dim3(64 * 16), 0, s, X, n, p, pk, P);
-  else if (!mask)
+  else if (shape == 1 && !mask)
     hipLaunchKernelGGL((k_mlp_fused<8, 2, false>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
-  else
+  else if (shape == 1)
     hipLaunchKernelGGL((k_mlp_fused<8, 2, true>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
+  else if (!mask)
+    hipLaunchKernelGGL((k_mlp_fused<8, 1, false>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
+  else
+    hipLaunchKernelGGL((k_mlp_fused<8, 1, true>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
 }
 
 unsigned g1(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
@@ -568,7 +607,7 @@ static int infer_fused(hyq_mlp* m, const float* X, int64_t n, float* P, hipStrea
     m->packed_version = m->version;
   }
   if (m->shape < 0 && n >= 65536) {
-    // pick the faster launch shape: every row computed (no mask), best of three runs each,
+    // pick the fastest launch shape: every row computed (no mask), best of three runs each,
     // on this call's own rows (the real launch below overwrites the trial predictions)
     float best[kFusedShapes];
     for (int sh = 0; sh < kFusedShapes; ++sh) best[sh] = 1e30f;

@@ -87,7 +87,7 @@ class WarmStartMLP:
         return P
 
     def set_shape(self, shape):
-        """Fused inference launch shape: -1 picks the faster by a timed trial, 0 (16 waves x 1 tile) or 1 (8 waves x 2 tiles) fixes one."""
+        """Fused inference launch shape: -1 picks the fastest by a timed trial, 0 (16 waves x 1 tile), 1 (8 x 2) or 2 (8 x 1) fixes one."""
         self._check(_lib.lib().hyq_mlp_set_shape(self.handle, int(shape)), "hyq_mlp_set_shape")
 
     def infer_layers(self, X):

@@ -175,9 +175,9 @@ int hyq_mlp_infer(hyq_mlp* m, const float* X, int64_t n, float* P);
 /* the same through the layer-by-layer training kernels (bitwise equal; a cross-check) */
 int hyq_mlp_infer_layers(hyq_mlp* m, const float* X, int64_t n, float* P);
 /* launch shape of the fused inference: -1 (the default) times the shapes on the first
- * inference of 65536 rows or more and keeps the faster; 0 or 1 fixes one (0: 16 waves x 1
- * row tile, 1: 8 waves x 2 row tiles).  Both give the same values.  No reference
- * counterpart. */
+ * inference of 65536 rows or more and keeps the fastest; 0..2 fixes one (0: 16 waves x 1
+ * row tile, 1: 8 waves x 2 row tiles, 2: 8 waves x 1 row tile).  All give the same values.
+ * No reference counterpart. */
 int hyq_mlp_set_shape(hyq_mlp* m, int32_t shape);
 /* loss and gradients (grad_host: hyq_mlp_param_count() floats, may be NULL); T [n][2]. */
 int hyq_mlp_grad(hyq_mlp* m, const float* X, const float* T, int64_t n, float* loss_out,

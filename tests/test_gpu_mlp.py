@@ -57,7 +57,7 @@ def test_fused_inference_equals_layers(n):
     assert not np.array_equal(a, c)
 
 
-@pytest.mark.parametrize("shape", [0, 1])
+@pytest.mark.parametrize("shape", [0, 1, 2])
 def test_every_fused_shape_equals_layers(shape):
     """Each launch shape of the fused inference gives the layer-by-layer predictions bit for
     bit, masked rows aside:
