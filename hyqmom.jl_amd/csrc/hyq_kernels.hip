@@ -714,6 +714,114 @@ __global__ __launch_bounds__(kBlock, kFullWaves) void k_pass2_full(double* __res
   }
 }
 
+// ---- runner diagnostics -------------------------------------------------------------
+// Symmetry test of simulation_runner.jl:565-628 on the k = 1 slice: the diagonal cells
+// (d, d), d < Nd = min(Nx, Ny), against their mirrors (Nd-1-d, Nd-1-d) for the first five
+// moments, differences for moments 1, 3, 5 and sums for 2, 4 (as the reference pairs them);
+// per moment max(Diff) / (|Diff|_2 + 1), and the largest magnitude of the five.  One block.
+__global__ __launch_bounds__(256) void k_symmetry(const double* __restrict__ M, long long N,
+                                                  int Nx, int Ny, double* __restrict__ out) {
+  __shared__ double smax[5][256], ssq[5][256];
+  const int tid = threadIdx.x;
+  const int nd = Nx < Ny ? Nx : Ny;
+  double mx[5], sq[5];
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    mx[n] = -INFINITY;
+    sq[n] = 0.0;
+  }
+  for (int d = tid; d < nd; d += 256) {
+    const long long a = d + (long long)Nx * d, b = (nd - 1 - d) + (long long)Nx * (nd - 1 - d);
+#pragma unroll
+    for (int n = 0; n < 5; ++n) {
+      const double va = M[(size_t)n * N + a], vb = M[(size_t)n * N + b];
+      const double df = (n & 1) ? va + vb : va - vb;
+      mx[n] = df > mx[n] ? df : mx[n];
+      sq[n] += df * df;
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < 5; ++n) {
+    smax[n][tid] = mx[n];
+    ssq[n][tid] = sq[n];
+  }
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h) {
+#pragma unroll
+      for (int n = 0; n < 5; ++n) {
+        smax[n][tid] = smax[n][tid + h] > smax[n][tid] ? smax[n][tid + h] : smax[n][tid];
+        ssq[n][tid] += ssq[n][tid + h];
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    double r = 0.0;
+    for (int n = 0; n < 5; ++n) {
+      const double v = smax[n][0] / (sqrt(ssq[n][0]) + 1.0);
+      out[n] = v;
+      r = fabs(v) > r ? fabs(v) : r;
+    }
+    out[5] = r;
+  }
+}
+
+// First-step diagnostics of simulation_runner.jl:404-494 on the last step's wave speeds
+// vp [6][N] (x min, x max, y min, y max, z min, z max): bits[0..2] the largest |speed| per
+// direction (as ordered bits of non-negative doubles; a NaN speed ranks above every number,
+// as the reference's maximum propagates it), bits[4] the largest |speed| over the cells
+// without a NaN speed, then the cell that reaches it first in the reference's search order
+// (k, then i, then j; its `>` test never takes a NaN cell): bits[3] its key
+// (k Nx + i) Ny + j, minimum over the cells that reach bits[4].
+template <typename FT>
+__global__ __launch_bounds__(kBlock) void k_vmax_dirs(const FT* __restrict__ vp, long long N,
+                                                      unsigned long long* __restrict__ bits) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long b[3] = {0ull, 0ull, 0ull}, fin = 0ull;
+  if (c < N) {
+    bool nan = false;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double lo = fabs((double)vp[(size_t)(2 * d) * N + c]);
+      const double hi = fabs((double)vp[(size_t)(2 * d + 1) * N + c]);
+      const bool dn = lo != lo || hi != hi;
+      nan = nan || dn;
+      b[d] = dn ? 0x7FF8000000000000ull : __double_as_longlong(hi > lo ? hi : lo);
+    }
+    if (!nan) {
+      fin = b[0] > b[1] ? b[0] : b[1];
+      fin = b[2] > fin ? b[2] : fin;
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const unsigned long long w = wave_max_u64(b[d]);
+    if ((threadIdx.x & 63) == 0) atomicMax(&bits[d], w);
+  }
+  const unsigned long long wf = wave_max_u64(fin);
+  if ((threadIdx.x & 63) == 0) atomicMax(&bits[4], wf);
+}
+template <typename FT>
+__global__ __launch_bounds__(kBlock) void k_vmax_worst(const FT* __restrict__ vp, long long N,
+                                                       int Nx, int Ny,
+                                                       unsigned long long* __restrict__ bits) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const unsigned long long g = bits[4];
+  unsigned long long m = 0ull;
+#pragma unroll
+  for (int t = 0; t < 6; ++t) {
+    const double v = fabs((double)vp[(size_t)t * N + c]);
+    if (v != v) return;
+    const unsigned long long vb = __double_as_longlong(v);
+    m = vb > m ? vb : m;
+  }
+  if (m != g) return;
+  const long long i = c % Nx, j = (c / Nx) % Ny, k = c / ((long long)Nx * Ny);
+  atomicMin(&bits[3], (unsigned long long)((k * Nx + i) * Ny + j));
+}
+
 unsigned grid_for(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -742,6 +850,7 @@ struct hyq_ctx {
   hipStream_t stream2;
   hipEvent_t evA, evB;     // pass 2 checked (stream -> stream2) / list done (stream2 -> stream)
   unsigned char* pend;     // [N] cells in the pass-2 list still running on stream2
+  double* diag;            // [8] results of the diagnostics
 };
 
 extern "C" {
@@ -776,7 +885,8 @@ int hyq_create(const hyq_params* params, hyq_ctx** out) {
             hipMalloc(&c->F, fsz * 105 * c->N) == hipSuccess &&
             hipMalloc(&c->vp, fsz * 6 * c->N) == hipSuccess &&
             hipMalloc(&c->flags, sizeof(uint32_t) * c->N) == hipSuccess &&
-            hipMalloc(&c->sc, sizeof(DevScalars)) == hipSuccess;
+            hipMalloc(&c->sc, sizeof(DevScalars)) == hipSuccess &&
+            hipMalloc(&c->diag, sizeof(double) * 8) == hipSuccess;
   if (ok && p.precision == HYQ_PREC_MIXED_LEARNED)
     ok = hipMalloc(&c->X, sizeof(float) * 16 * 5 * c->N) == hipSuccess &&
          hipMalloc(&c->P, sizeof(float) * 2 * 5 * c->N) == hipSuccess;
@@ -814,6 +924,7 @@ int hyq_destroy(hyq_ctx* c) {
   hipFree(c->vp);
   hipFree(c->flags);
   hipFree(c->sc);
+  hipFree(c->diag);
   hipFree(c->X);
   hipFree(c->P);
   hipFree(c->list1);
@@ -1044,6 +1155,56 @@ int hyq_get_wave_speeds(hyq_ctx* c, double* vp) {
       for (long long i = 0; i < 6 * c->N; ++i) vp[i] = tmp[i];
     delete[] tmp;
     HIPCHK(e, "vp copy");
+  }
+  return HYQ_OK;
+}
+
+int hyq_symmetry_maxdiff(hyq_ctx* c, double* maxdiff, double* per_moment) {
+  if (!c || !maxdiff) return fail(HYQ_EINVAL, "null argument");
+  double h[6];
+  hipLaunchKernelGGL(k_symmetry, dim3(1), dim3(256), 0, c->stream, (const double*)c->M, c->N,
+                     c->p.Nx, c->p.Ny, c->diag);
+  HIPCHK(hipGetLastError(), "symmetry launch");
+  HIPCHK(hipMemcpyAsync(h, c->diag, sizeof(h), hipMemcpyDeviceToHost, c->stream), "symmetry copy");
+  HIPCHK(hipStreamSynchronize(c->stream), "symmetry sync");
+  *maxdiff = h[5];
+  if (per_moment)
+    for (int n = 0; n < 5; ++n) per_moment[n] = h[n];
+  return HYQ_OK;
+}
+
+int hyq_vmax_breakdown(hyq_ctx* c, double* vdir, int64_t* worst) {
+  if (!c || !vdir) return fail(HYQ_EINVAL, "null argument");
+  unsigned long long h[5] = {0ull, 0ull, 0ull, ~0ull, 0ull};
+  unsigned long long* d = reinterpret_cast<unsigned long long*>(c->diag);
+  HIPCHK(hipMemcpyAsync(d, h, sizeof(h), hipMemcpyHostToDevice, c->stream), "vmax init");
+  const unsigned g = grid_for(c->N);
+  if (c->p.precision == HYQ_PREC_FP64) {
+    hipLaunchKernelGGL(k_vmax_dirs<double>, dim3(g), dim3(kBlock), 0, c->stream,
+                       (const double*)c->vp, c->N, d);
+    hipLaunchKernelGGL(k_vmax_worst<double>, dim3(g), dim3(kBlock), 0, c->stream,
+                       (const double*)c->vp, c->N, c->p.Nx, c->p.Ny, d);
+  } else {
+    hipLaunchKernelGGL(k_vmax_dirs<float>, dim3(g), dim3(kBlock), 0, c->stream,
+                       (const float*)c->vp, c->N, d);
+    hipLaunchKernelGGL(k_vmax_worst<float>, dim3(g), dim3(kBlock), 0, c->stream,
+                       (const float*)c->vp, c->N, c->p.Nx, c->p.Ny, d);
+  }
+  HIPCHK(hipGetLastError(), "vmax launch");
+  HIPCHK(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream), "vmax copy");
+  HIPCHK(hipStreamSynchronize(c->stream), "vmax sync");
+  for (int k = 0; k < 3; ++k) {
+    double v;
+    std::memcpy(&v, &h[k], sizeof(v));
+    vdir[k] = v;
+  }
+  if (worst && h[3] == ~0ull) {  // every cell has a NaN speed
+    worst[0] = worst[1] = worst[2] = -1;
+  } else if (worst) {
+    const long long key = (long long)h[3], Nx = c->p.Nx, Ny = c->p.Ny;
+    worst[0] = (key / Ny) % Nx;      // i
+    worst[1] = key % Ny;             // j
+    worst[2] = key / (Ny * Nx);      // k
   }
   return HYQ_OK;
 }

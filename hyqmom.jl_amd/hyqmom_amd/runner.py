@@ -186,6 +186,23 @@ class Solver:
         _lib.check(_lib.lib().hyq_deferred_cells(self.h, ctypes.byref(a), ctypes.byref(b)), "deferred cells")
         return a.value, b.value
 
+    def symmetry_maxdiff(self):
+        """(MaxDiff, per-moment values) of the runner's symmetry check
+        (simulation_runner.jl:565-628) on the current state, computed on the device."""
+        m = ctypes.c_double()
+        per = np.zeros(5)
+        _lib.check(_lib.lib().hyq_symmetry_maxdiff(self.h, ctypes.byref(m), per.ctypes.data_as(_lib.D)),
+                   "symmetry")
+        return m.value, per
+
+    def vmax_breakdown(self):
+        """((vx_max, vy_max, vz_max), (i, j, k) 0-based of the fastest cell) of the last step's
+        wave speeds, as the reference's first-step diagnostics (simulation_runner.jl:404-494)."""
+        v = np.zeros(3)
+        w = (_lib.I64 * 3)()
+        _lib.check(_lib.lib().hyq_vmax_breakdown(self.h, v.ctypes.data_as(_lib.D), w), "vmax breakdown")
+        return tuple(v), (w[0], w[1], w[2])
+
 
 def grid(params):
     """grid_out of simulation_runner.jl:720-730."""
@@ -220,16 +237,33 @@ def simulation_runner(params, verbose=False):
             w.close(grid(params))
             return fname, grid(params)
         t0 = time.time()
+        sym = int(_get(params, "symmetry_check_interval", 0) or 0)
         while solver.t < tmax and solver.nn < nnmax:
+            t_prev = solver.t
             solver.step()
             if verbose:
                 n = params_cells(params)
                 el = time.time() - t0
-                print("step %d t=%.6e wall=%.3fs s/pt=%.3e" % (solver.nn, solver.t, el, el / n))
+                if solver.nn == 1:
+                    _print_first_step(solver, params)
+                line = "Step %4d: t = %.4f, dt = %.4e, wall = %.4f s, s/pt = %.4e s" % (
+                    solver.nn, solver.t, solver.t - t_prev, el, el / n)
+                if sym > 0 and (solver.nn % sym == 0 or solver.nn == 1):
+                    line += ", MaxDiff = %.3e" % solver.symmetry_maxdiff()[0]
+                print(line)
                 t0 = time.time()
         return solver.get_state(), solver.t, solver.nn, grid(params)
     finally:
         solver.close()
+
+
+def _print_first_step(solver, params):
+    """The vmax breakdown the reference prints after its first step (simulation_runner.jl:
+    404-427), from the device reduction."""
+    (vx, vy, vz), (i, j, k) = solver.vmax_breakdown()
+    vmax = max(vx, vy, vz)
+    print("  first step: vx_max = %.6e, vy_max = %.6e, vz_max = %.6e, vmax = %.6e, "
+          "fastest cell (%d,%d,%d)" % (vx, vy, vz, vmax, i + 1, j + 1, k + 1))
 
 
 def params_cells(params):

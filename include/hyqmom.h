@@ -124,6 +124,21 @@ int hyq_reset_kernel_times(hyq_ctx* ctx);
  * the other modes.  No reference counterpart (the reference has a single path). */
 int hyq_deferred_cells(hyq_ctx* ctx, int64_t* pass1, int64_t* pass2);
 
+/* The runner's symmetry diagnostic (simulation_runner.jl:565-628) on the current state:
+ * the diagonal cells (i, i, 1), i <= Nd = min(Nx, Ny), against (Nd+1-i, Nd+1-i, 1) for the
+ * first five moments (differences for moments 1, 3, 5, sums for 2 and 4); per moment
+ * max(Diff) / (norm(Diff) + 1) into per_moment[5] (may be NULL), and the largest magnitude
+ * of the five into *maxdiff.  A device reduction. */
+int hyq_symmetry_maxdiff(hyq_ctx* ctx, double* maxdiff, double* per_moment);
+
+/* The first-step diagnostics of simulation_runner.jl:404-494 on the last step's wave
+ * speeds: vdir[3] = max |speed| in x, y and z (vx_max, vy_max, vz_max; NaN if any speed is
+ * NaN), and worst[3] (may be NULL) = the 0-based (i, j, k) of the cell with the largest
+ * |speed|, the first in the reference's search order (k, then i, then j), cells with a NaN
+ * speed skipped as the reference's `>` test skips them (-1s if every cell has one).  Device
+ * reductions. */
+int hyq_vmax_breakdown(hyq_ctx* ctx, double* vdir, int64_t* worst);
+
 /* Batched per-cell kernels on device SoA planes ([35][n] doubles). */
 
 /* Flux_closure35_and_realizable_3D (src/numerics/Flux_closure35_and_realizable_3D.jl:33-234)
