@@ -98,6 +98,23 @@ function run!(s::AmdSolver, tmax::Real, nnmax::Integer)
     return s
 end
 
+# The runner's symmetry check MaxDiff (simulation_runner.jl:565-628), a device reduction.
+function symmetry_maxdiff(s::AmdSolver)
+    m = Ref(0.0); per = zeros(5)
+    check(ccall((:hyq_symmetry_maxdiff, libhyqmom), Cint, (Ptr{Cvoid}, Ref{Float64}, Ptr{Float64}),
+                s.ptr, m, per), "hyq_symmetry_maxdiff")
+    return m[], per
+end
+
+# The first-step vmax breakdown (simulation_runner.jl:404-461): (vx_max, vy_max, vz_max) and
+# the 1-based (i, j, k) of the fastest cell in the reference's search order.
+function vmax_breakdown(s::AmdSolver)
+    v = zeros(3); w = zeros(Int64, 3)
+    check(ccall((:hyq_vmax_breakdown, libhyqmom), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Int64}),
+                s.ptr, v, w), "hyq_vmax_breakdown")
+    return (v[1], v[2], v[3]), (w[1] + 1, w[2] + 1, w[3] + 1)
+end
+
 # grid_out of simulation_runner.jl:720-730
 function grid_out(params)
     xmin, xmax = getp(params, :xmin, -0.5), getp(params, :xmax, 0.5)
