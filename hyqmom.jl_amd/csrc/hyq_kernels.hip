@@ -714,6 +714,47 @@ __global__ __launch_bounds__(kBlock, kFullWaves) void k_pass2_full(double* __res
   }
 }
 
+// ---- snapshot fields ----------------------------------------------------------------
+// compute_central_field / compute_standardized_field (src/utils/compute_standardized_field.jl:
+// 16-62): M2CS4_35 (src/moments/M2CS4_35.jl) per cell.  STD = 0: the central moments C4
+// (c000 = 1, first order 0); STD = 1: the standardized S4, C4[ijk] / (s_x^i s_y^j s_z^k)
+// with s = sqrt(max(C2, eps)) per axis, 1 for the order-0 and pure second-order entries, 0
+// for the first order.  Out: [35][N] planes.
+template <int STD>
+__global__ __launch_bounds__(kBlock) void k_snapshot_field(const double* __restrict__ M,
+                                                           long long N, double* __restrict__ out) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  double m[35], cm[35];
+#pragma unroll
+  for (int n = 0; n < 35; ++n) m[n] = M[(size_t)n * N + c];
+  double M000, u, v, w;
+  raw_to_central4(m, cm, M000, u, v, w);
+  const double sx = sqrt(jmax(cm[tpos(2, 0, 0)], EPS));
+  const double sy = sqrt(jmax(cm[tpos(0, 2, 0)], EPS));
+  const double sz = sqrt(jmax(cm[tpos(0, 0, 2)], EPS));
+#pragma unroll
+  for (int n = 0; n < 35; ++n) {
+    const int i = e35().e[n][0], j = e35().e[n][1], k = e35().e[n][2];
+    const double cv = cm[T35(n)];
+    double r = cv;
+    if (STD) {
+      const int o = i + j + k;
+      if (o == 0) r = 1.0;
+      else if (o == 1) r = 0.0;
+      else if (o == 2 && (i == 2 || j == 2 || k == 2)) r = 1.0;
+      else {
+        double d = 1.0;
+        for (int t = 0; t < i; ++t) d *= sx;
+        for (int t = 0; t < j; ++t) d *= sy;
+        for (int t = 0; t < k; ++t) d *= sz;
+        r = cv / d;
+      }
+    }
+    out[(size_t)n * N + c] = r;
+  }
+}
+
 // ---- runner diagnostics -------------------------------------------------------------
 // Symmetry test of simulation_runner.jl:565-628 on the k = 1 slice: the diagonal cells
 // (d, d), d < Nd = min(Nx, Ny), against their mirrors (Nd-1-d, Nd-1-d) for the first five
@@ -1155,6 +1196,27 @@ int hyq_get_wave_speeds(hyq_ctx* c, double* vp) {
       for (long long i = 0; i < 6 * c->N; ++i) vp[i] = tmp[i];
     delete[] tmp;
     HIPCHK(e, "vp copy");
+  }
+  return HYQ_OK;
+}
+
+int hyq_snapshot_fields(hyq_ctx* c, double* C, double* S) {
+  if (!c) return fail(HYQ_EINVAL, "null argument");
+  // Mr is free between steps (each step's pass 1 rewrites it before reading it)
+  const unsigned g = grid_for(c->N);
+  for (int which = 0; which < 2; ++which) {
+    double* dst = which == 0 ? C : S;
+    if (!dst) continue;
+    if (which == 0)
+      hipLaunchKernelGGL(k_snapshot_field<0>, dim3(g), dim3(kBlock), 0, c->stream,
+                         (const double*)c->M, c->N, c->Mr);
+    else
+      hipLaunchKernelGGL(k_snapshot_field<1>, dim3(g), dim3(kBlock), 0, c->stream,
+                         (const double*)c->M, c->N, c->Mr);
+    HIPCHK(hipGetLastError(), "snapshot field launch");
+    HIPCHK(hipMemcpyAsync(dst, c->Mr, sizeof(double) * 35 * c->N, hipMemcpyDeviceToHost, c->stream),
+           "snapshot field copy");
+    HIPCHK(hipStreamSynchronize(c->stream), "snapshot field sync");
   }
   return HYQ_OK;
 }

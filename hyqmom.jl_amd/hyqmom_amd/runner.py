@@ -186,6 +186,16 @@ class Solver:
         _lib.check(_lib.lib().hyq_deferred_cells(self.h, ctypes.byref(a), ctypes.byref(b)), "deferred cells")
         return a.value, b.value
 
+    def snapshot_fields(self):
+        """(C, S): the central and standardized moment fields of the current state
+        (compute_central_field / compute_standardized_field), computed on the device, in the
+        state's (Nx, Ny, Nz, 35) layout."""
+        C = np.empty(int(np.prod(self.shape)), dtype=np.float64)
+        S = np.empty_like(C)
+        _lib.check(_lib.lib().hyq_snapshot_fields(self.h, C.ctypes.data_as(_lib.D), S.ctypes.data_as(_lib.D)),
+                   "snapshot fields")
+        return C.reshape(self.shape, order="F"), S.reshape(self.shape, order="F")
+
     def symmetry_maxdiff(self):
         """(MaxDiff, per-moment values) of the runner's symmetry check
         (simulation_runner.jl:565-628) on the current state, computed on the device."""
@@ -230,10 +240,10 @@ def simulation_runner(params, verbose=False):
             from .snapshots import SnapshotWriter
             fname = _get(params, "snapshot_filename") or "snapshots.npz"
             w = SnapshotWriter(fname, params, snap)
-            w.write(solver.get_state(), solver.t, solver.nn)
+            w.write(solver.get_state(), solver.t, solver.nn, *solver.snapshot_fields())
             while solver.t < tmax and solver.nn < nnmax:
                 solver.run(tmax, min(nnmax, solver.nn + snap))
-                w.write(solver.get_state(), solver.t, solver.nn)
+                w.write(solver.get_state(), solver.t, solver.nn, *solver.snapshot_fields())
             w.close(grid(params))
             return fname, grid(params)
         t0 = time.time()

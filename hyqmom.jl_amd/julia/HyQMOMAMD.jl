@@ -98,6 +98,16 @@ function run!(s::AmdSolver, tmax::Real, nnmax::Integer)
     return s
 end
 
+# Central and standardized moment fields of the current state (compute_central_field,
+# compute_standardized_field), computed on the device.
+function snapshot_fields(s::AmdSolver)
+    C = Array{Float64,4}(undef, s.p.Nx, s.p.Ny, s.p.Nz, 35); S = similar(C)
+    GC.@preserve C S check(ccall((:hyq_snapshot_fields, libhyqmom), Cint,
+                                 (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), s.ptr, C, S),
+                           "hyq_snapshot_fields")
+    return C, S
+end
+
 # The runner's symmetry check MaxDiff (simulation_runner.jl:565-628), a device reduction.
 function symmetry_maxdiff(s::AmdSolver)
     m = Ref(0.0); per = zeros(5)

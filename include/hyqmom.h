@@ -124,6 +124,13 @@ int hyq_reset_kernel_times(hyq_ctx* ctx);
  * the other modes.  No reference counterpart (the reference has a single path). */
 int hyq_deferred_cells(hyq_ctx* ctx, int64_t* pass1, int64_t* pass2);
 
+/* The snapshot fields of the current state on the device: the central moments C
+ * (compute_central_field) and the standardized moments S (compute_standardized_field,
+ * src/utils/compute_standardized_field.jl:16-62; M2CS4_35 per cell), host outputs in the
+ * state's (Nx,Ny,Nz,35) layout; either may be NULL.  Call between steps (the device
+ * scratch it uses is rewritten by every step). */
+int hyq_snapshot_fields(hyq_ctx* ctx, double* C, double* S);
+
 /* The runner's symmetry diagnostic (simulation_runner.jl:565-628) on the current state:
  * the diagonal cells (i, i, 1), i <= Nd = min(Nx, Ny), against (Nd+1-i, Nd+1-i, 1) for the
  * first five moments (differences for moments 1, 3, 5, sums for 2 and 4); per moment

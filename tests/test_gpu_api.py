@@ -50,3 +50,36 @@ def test_run_simulation_with_snapshots(tmp_path):
     S = z["snapshots/%06d/S" % n]
     assert np.allclose(S[..., 2], 1.0) and np.allclose(S[..., 1], 0.0)
     assert int(z["meta/snapshot_interval"]) == 2 and "grid/xm" in z.files
+
+
+def test_snapshot_fields_match_reference(tmp_path):
+    """hyq_snapshot_fields (device M2CS4_35 per cell) against the CPU reference of
+    compute_central_field / compute_standardized_field, on the initial jets and after a few
+    steps; the snapshot file holds the device fields of its state."""
+    import hyqmom_amd as H
+    from snapshot_ref import central_standardized
+    p = _params()
+    s = H.Solver(p)
+    try:
+        s.set_state(H.initial_state(p))
+        for steps in (0, 3):
+            if steps:
+                s.run(10.0, steps)
+            M = s.get_state()
+            C, S = s.snapshot_fields()
+            Cr, Sr = central_standardized(M)
+            for got, want in ((C, Cr), (S, Sr)):
+                # per moment, relative to its largest magnitude or to 1 (moments that vanish
+                # exactly in one restatement carry rounding-level values in the other)
+                scale = np.maximum(np.abs(want).max(axis=(0, 1, 2)), 1.0)
+                err = (np.abs(got - want).max(axis=(0, 1, 2)) / scale).max()
+                assert err < 1e-11, (steps, err)
+    finally:
+        s.close()
+    fname = str(tmp_path / "snap.npz")
+    H.run_simulation_with_snapshots(_params(snapshot_filename=fname, nnmax=2), snapshot_interval=2)
+    z = np.load(fname)
+    M = z["snapshots/000002/M"]
+    Cr, Sr = central_standardized(M)
+    assert np.allclose(z["snapshots/000002/C"], Cr, rtol=1e-10, atol=1e-12)
+    assert np.allclose(z["snapshots/000002/S"], Sr, rtol=1e-10, atol=1e-12)

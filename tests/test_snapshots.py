@@ -1,9 +1,9 @@
-"""Central and standardized moment fields of the snapshot file (compute_central_field /
-compute_standardized_field), pinned by Gaussian closed forms."""
+"""The CPU reference of the snapshot file's central and standardized moment fields
+(compute_central_field / compute_standardized_field), pinned by Gaussian closed forms."""
 import numpy as np
 
 import hyqmom_amd as H
-from hyqmom_amd.snapshots import central_standardized
+from snapshot_ref import central_standardized
 
 
 def test_gaussian_cells_standardize_to_isserlis_values():
