@@ -280,6 +280,34 @@ def params_cells(params):
     return int(_get(params, "Nx")) * int(_get(params, "Ny")) * int(_get(params, "Nz"))
 
 
+def run_simulation(Np=None, Nx=20, Ny=20, Nz=1, tmax=0.1, num_workers=1, verbose=True,
+                   save_output=False, enable_plots=False, save_figures=False, output_dir=".",
+                   Kn=1.0, Ma=0.0, flag2D=0, CFL=0.5, homogeneous_z=True, debug_output=False,
+                   precision="fp64"):
+    """src/simulation_runner.jl:816-944: the original crossing jets (rhol = 1, rhor = 0.01,
+    T = 1, no correlations) on the unit cube, dtmax = CFL min(dx, dy, dz), run on the device.
+    Returns the reference's dictionary (M, final_time, time_steps, xm, ym, zm, Nx, Ny, Nz,
+    tmax; string keys for its symbols).  num_workers, plots and output files have no role on
+    one device; they are accepted and ignored."""
+    if Np is not None:
+        Nx = Ny = Np
+    dx, dy, dz = 1.0 / Nx, 1.0 / Ny, 1.0 / Nz
+    dtmax = CFL * min(dx, dy, dz)
+    params = dict(Nx=Nx, Ny=Ny, Nz=Nz, tmax=tmax, Kn=Kn, Ma=Ma, flag2D=flag2D, CFL=CFL, dx=dx,
+                  dy=dy, dz=dz, Nmom=35, nnmax=math.ceil(tmax / dtmax) + 100000, dtmax=dtmax,
+                  rhol=1.0, rhor=0.01, T=1.0, r110=0.0, r101=0.0, r011=0.0,
+                  symmetry_check_interval=10, homogeneous_z=homogeneous_z,
+                  enable_memory_tracking=False, debug_output=debug_output, precision=precision)
+    if verbose:
+        print("HyQMOM simulation: grid %dx%dx%d, tmax %g, Kn %g, Ma %g, CFL %g, dtmax %g"
+              % (Nx, Ny, Nz, tmax, Kn, Ma, CFL, dtmax))
+    M, t, nn, g = simulation_runner(params, verbose=verbose)
+    if verbose:
+        print("Simulation complete: final time %g, %d time steps" % (t, nn))
+    return {"M": M, "final_time": t, "time_steps": nn, "xm": g["xm"], "ym": g["ym"],
+            "zm": g["zm"], "Nx": Nx, "Ny": Ny, "Nz": Nz, "tmax": tmax}
+
+
 def run_simulation_with_snapshots(params, snapshot_interval=2):
     """src/simulation_runner.jl:994-1011"""
     p = dict(params) if isinstance(params, dict) else dict(vars(params))
