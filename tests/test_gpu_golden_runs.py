@@ -28,3 +28,24 @@ def test_run_simulation_matches_golden_run(name, precision):
         assert absmax < 1e-13 and err < 1e-12, (absmax, err)
     else:
         assert absmax < 1e-6, absmax
+
+
+@pytest.mark.parametrize("Ma", [0.5, 1.0])
+def test_run_simulation_2d_flag_matches_oracle(Ma):
+    """flag2D = 1 (the 2D closure and realizability of the reference's 2D mode) at Nz = 1:
+    run_simulation on the device against the oracle's step loop, 7 steps to t = 0.05."""
+    import math
+
+    import oracle as O
+    import hyqmom_amd as H
+    from hyqmom_amd.runner import hardcoded_jets
+    Np, tmax = 20, 0.05
+    r = H.run_simulation(Np=Np, tmax=tmax, Kn=1.0, Ma=Ma, flag2D=1, CFL=0.5, verbose=False)
+    M0 = hardcoded_jets(Np, Np, 1, 1.0, 0.01, 1.0, Ma)
+    dx = 1.0 / Np
+    Mo, to, nno, _, rc = O.run(M0, Np, Np, 1, 1.0, Ma, 1, 0.5, 0.5 * dx, dx, dx, 1.0, True, tmax,
+                               math.ceil(tmax / (0.5 * dx)) + 100000)
+    assert rc == 0 and r["time_steps"] == nno and abs(r["final_time"] - to) <= 1e-15
+    scale = np.maximum(np.abs(Mo).max(axis=(0, 1, 2)), 1e-300)
+    err = (np.abs(r["M"] - Mo).max(axis=(0, 1, 2)) / scale).max()
+    assert err < 1e-11, err
