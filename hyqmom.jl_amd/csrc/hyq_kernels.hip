@@ -28,6 +28,9 @@ extern "C" int hyq_mlp_infer_async(hyq_mlp* m, const float* X, int64_t n, float*
                                    hipStream_t stream);
 extern "C" int hyq_mlp_infer_masked_async(hyq_mlp* m, const float* X, int64_t n, float* P,
                                           hipStream_t stream);
+extern "C" int hyq_mlp_infer_bf16_masked_async(hyq_mlp* m, const short* Xb, int64_t n, float* P,
+                                               hipStream_t stream);
+extern "C" const float* hyq_mlp_input_norm(hyq_mlp* m);
 
 namespace {
 
@@ -422,11 +425,17 @@ __global__ __launch_bounds__(kBlock) void k_plane_dataset(const double* __restri
   }
 }
 // Pass 1, first half: projection (state written to A) and, with FEAT, the plane features.
+// FEAT: also the network's input rows of the cell's five planes, normalized with `norm`
+// (16 means, 16 standard deviations) and rounded to bf16 as the network's first layer takes
+// them (the same FP32 operations and rounding as its own normalization), column 15 the need
+// flag; row q N + c of Xb, 32 bytes.
+__device__ __forceinline__ short to_bf16(float f) { return __builtin_bit_cast(short, (__bf16)f); }
 template <bool FEAT>
 __global__ __launch_bounds__(kBlock, kPassWaves) void k_pass1a(const double* __restrict__ M, double* __restrict__ A,
-                                                   float* __restrict__ X, uint32_t* __restrict__ flags,
+                                                   short* __restrict__ Xb, uint32_t* __restrict__ flags,
                                                    DevScalars* sc, long long N, int flag2D, double Ma,
-                                                   const unsigned char* __restrict__ pend) {
+                                                   const unsigned char* __restrict__ pend,
+                                                   const float* __restrict__ norm) {
   if (sc->active == 0.0) return;
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N || (pend && pend[c])) return;
@@ -444,9 +453,16 @@ __global__ __launch_bounds__(kBlock, kPassWaves) void k_pass1a(const double* __r
   for (int q = 0; q < 5; ++q) {
     float x[16];
     plane_feature(pl[q], x);
-    float4* dst = reinterpret_cast<float4*>(X + ((size_t)q * N + c) * 16);
+    typedef short s8v __attribute__((ext_vector_type(8)));
+    s8v h[2];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) dst[t] = make_float4(x[4 * t], x[4 * t + 1], x[4 * t + 2], x[4 * t + 3]);
+    for (int t = 0; t < 16; ++t) {
+      const float v = t < 15 ? (x[t] - norm[t]) / norm[16 + t] : x[15];
+      h[t / 8][t % 8] = to_bf16(v);
+    }
+    s8v* dst = reinterpret_cast<s8v*>(Xb + ((size_t)q * N + c) * 16);
+    dst[0] = h[0];
+    dst[1] = h[1];
   }
 }
 
@@ -1017,16 +1033,19 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
     const bool learned = p.precision == HYQ_PREC_MIXED_LEARNED && c->mlp != nullptr;
     if (learned) {
       // projection + features, then the network's warm starts
-      hipLaunchKernelGGL(k_pass1a<true>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr, c->X,
-                         c->flags, c->sc, c->N, p.flag2D, p.Ma, pend);
-      const int rc = hyq_mlp_infer_masked_async(c->mlp, c->X, 5 * c->N, c->P, c->stream);
+      hipLaunchKernelGGL(k_pass1a<true>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
+                         reinterpret_cast<short*>(c->X), c->flags, c->sc, c->N, p.flag2D, p.Ma, pend,
+                         hyq_mlp_input_norm(c->mlp));
+      const int rc = hyq_mlp_infer_bf16_masked_async(c->mlp, reinterpret_cast<const short*>(c->X),
+                                                     5 * c->N, c->P, c->stream);
       if (rc) return rc;
       hipLaunchKernelGGL(k_pass1_fast<false>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
                          (float*)c->F, (float*)c->vp, c->flags, (const float*)c->P, c->list1,
                          c->sc, c->N, p.flag2D, p.Ma, p.positivity, pend);
     } else if (kNewtonSplit) {
-      hipLaunchKernelGGL(k_pass1a<false>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr, c->X,
-                         c->flags, c->sc, c->N, p.flag2D, p.Ma, pend);
+      hipLaunchKernelGGL(k_pass1a<false>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
+                         reinterpret_cast<short*>(c->X), c->flags, c->sc, c->N, p.flag2D, p.Ma, pend,
+                         (const float*)nullptr);
       hipLaunchKernelGGL(k_pass1_fast<false>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
                          (float*)c->F, (float*)c->vp, c->flags, (const float*)nullptr, c->list1,
                          c->sc, c->N, p.flag2D, p.Ma, p.positivity, pend);

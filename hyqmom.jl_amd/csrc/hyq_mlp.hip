@@ -248,16 +248,20 @@ __global__ void k_pack(const float* __restrict__ p, short* __restrict__ pk) {
 // tile, so after the weights are loaded the waves run independently (no block barrier in the
 // loop; a wavefront fence orders a wave's activation stores before its loads).  MASK: rows
 // whose column 15 is 0 need no prediction; a wave whose rows all are such writes NaN
-// predictions and skips the work.
+// predictions and skips the work.  BIN: the rows are already normalized and rounded to bf16
+// (16 bf16 per row, column 15 the need flag), as the step's projection kernel writes them;
+// the kernel then feeds them to the first MFMA as loaded.  Both forms give the same bits.
 __device__ __forceinline__ void wave_lds_fence() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
-template <int WAVES, int RT, bool MASK>
-__global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restrict__ X, long n,
+template <int WAVES, int RT, bool MASK, bool BIN>
+__global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const void* __restrict__ Xv, long n,
                                                           const float* __restrict__ p,
                                                           const short* __restrict__ pk,
                                                           float* __restrict__ P) {
+  const float* X = reinterpret_cast<const float*>(Xv);
+  const s8* Xb = reinterpret_cast<const s8*>(Xv);  // BIN: row b is Xb[2 b], Xb[2 b + 1]
   __shared__ s8 wts[PK_N / 8];
   __shared__ float bias[3 * HID + 2];
   __shared__ __align__(16) short act[WAVES][RT * 16 * ACT_LD];
@@ -292,12 +296,17 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
   // raw feature rows of the next group, loaded while the current one is computed: lane
   // (r, q < 2) holds columns 8q..8q+7 of row 16u + r of the group, column 15 included
   float xn[RT][8];
+  s8 bn[RT];
   auto load_group = [&](long g) {
 #pragma unroll
     for (int u = 0; u < RT; ++u) {
       const long row = g * 16 * RT + 16 * u + r;
+      if (BIN) {
+        bn[u] = (q < 2 && row < n) ? Xb[row * 2 + q] : s8{0, 0, 0, 0, 0, 0, 0, 0};
+      } else {
 #pragma unroll
-      for (int e = 0; e < 8; ++e) xn[u][e] = (q < 2 && row < n) ? X[row * 16 + 8 * q + e] : 0.f;
+        for (int e = 0; e < 8; ++e) xn[u][e] = (q < 2 && row < n) ? X[row * 16 + 8 * q + e] : 0.f;
+      }
     }
   };
   short* A = act[w];
@@ -306,16 +315,25 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
   for (; grp < ngroups; grp += gstride) {
     const long row0 = grp * 16 * RT;
     float xv[RT][8];
+    s8 bv8[RT];
 #pragma unroll
-    for (int u = 0; u < RT; ++u)
+    for (int u = 0; u < RT; ++u) {
+      if (BIN) {
+        bv8[u] = bn[u];
+      } else {
 #pragma unroll
-      for (int e = 0; e < 8; ++e) xv[u][e] = xn[u][e];
+        for (int e = 0; e < 8; ++e) xv[u][e] = xn[u][e];
+      }
+    }
     if (grp + gstride < ngroups) load_group(grp + gstride);
     if (MASK) {
       // column 15 of row 16u + r sits in lane (r, q = 1), element 7
       bool need = false;
 #pragma unroll
-      for (int u = 0; u < RT; ++u) need = need || (q == 1 && row0 + 16 * u + r < n && xv[u][7] != 0.f);
+      for (int u = 0; u < RT; ++u) {
+        const bool flag = BIN ? bv8[u][7] != 0 : xv[u][7] != 0.f;
+        need = need || (q == 1 && row0 + 16 * u + r < n && flag);
+      }
       if (__ballot(need) == 0ull) {
 #pragma unroll
         for (int u = 0; u < RT; ++u) {
@@ -334,12 +352,17 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
     for (int u = 0; u < RT; ++u) {
       const long row = row0 + 16 * u + r;
       s8 a;
+      if (BIN) {
+        a = bv8[u];
+        if (q == 1) a[7] = 0;  // the need flag does not enter the network
+      } else {
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int k = 8 * q + e;
-        float v = 0.f;
-        if (row < n && k < 15) v = (xv[u][e] - xmu[e]) / xsd[e];
-        a[e] = f2bf(v);
+        for (int e = 0; e < 8; ++e) {
+          const int k = 8 * q + e;
+          float v = 0.f;
+          if (row < n && k < 15) v = (xv[u][e] - xmu[e]) / xsd[e];
+          a[e] = f2bf(v);
+        }
       }
 #pragma unroll
       for (int t = 0; t < NTH; ++t)
@@ -411,26 +434,29 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const float* __restric
 // CU).  They compute the same values in the same order; the first large inference times them
 // and keeps the fastest.
 constexpr int kFusedShapes = 3;
-void launch_fused(int shape, bool mask, const float* X, long n, const float* p, const short* pk,
-                  float* P, hipStream_t s) {
+template <bool MASK, bool BIN>
+void launch_fused_t(int shape, const void* X, long n, const float* p, const short* pk, float* P,
+                    hipStream_t s) {
   const int waves = shape == 0 ? 16 : 8;
   const int rt = shape == 1 ? 2 : 1;
   const long groups = (n + 16 * rt - 1) / (16 * rt);
   long blocks = (groups + waves - 1) / waves;
   if (blocks > 2048) blocks = 2048;
   const dim3 g((unsigned)blocks);
-  if (shape == 0 && !mask)
-    hipLaunchKernelGGL((k_mlp_fused<16, 1, false>), g, dim3(64 * 16), 0, s, X, n, p, pk, P);
-  else if (shape == 0)
-    hipLaunchKernelGGL((k_mlp_fused<16, 1, true>), g, dim3(64 * 16), 0, s, X, n, p, pk, P);
-  else if (shape == 1 && !mask)
-    hipLaunchKernelGGL((k_mlp_fused<8, 2, false>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
+  if (shape == 0)
+    hipLaunchKernelGGL((k_mlp_fused<16, 1, MASK, BIN>), g, dim3(64 * 16), 0, s, X, n, p, pk, P);
   else if (shape == 1)
-    hipLaunchKernelGGL((k_mlp_fused<8, 2, true>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
-  else if (!mask)
-    hipLaunchKernelGGL((k_mlp_fused<8, 1, false>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
+    hipLaunchKernelGGL((k_mlp_fused<8, 2, MASK, BIN>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
   else
-    hipLaunchKernelGGL((k_mlp_fused<8, 1, true>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
+    hipLaunchKernelGGL((k_mlp_fused<8, 1, MASK, BIN>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
+}
+// X: float rows (bin = false) or normalized bf16 rows (bin = true), 16 values each.
+void launch_fused(int shape, bool mask, bool bin, const void* X, long n, const float* p,
+                  const short* pk, float* P, hipStream_t s) {
+  if (mask && bin) launch_fused_t<true, true>(shape, X, n, p, pk, P, s);
+  else if (mask) launch_fused_t<true, false>(shape, X, n, p, pk, P, s);
+  else if (bin) launch_fused_t<false, true>(shape, X, n, p, pk, P, s);
+  else launch_fused_t<false, false>(shape, X, n, p, pk, P, s);
 }
 
 unsigned g1(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
@@ -596,8 +622,8 @@ int hyq_mlp_set_blob(hyq_mlp* m, const float* blob_host) {
   return HYQ_OK;
 }
 
-static int infer_fused(hyq_mlp* m, const float* X, int64_t n, float* P, hipStream_t stream,
-                       bool mask) {
+static int infer_fused(hyq_mlp* m, const void* X, int64_t n, float* P, hipStream_t stream,
+                       bool mask, bool bin = false) {
   if (!m || !X || !P || n < 0) return mfail(HYQ_EINVAL, "bad argument");
   if (n == 0) return HYQ_OK;
   if (m->packed_version != m->version) {
@@ -615,7 +641,7 @@ static int infer_fused(hyq_mlp* m, const float* X, int64_t n, float* P, hipStrea
       for (int sh = 0; sh < kFusedShapes; ++sh) {
         float ms = 0.f;
         MCHK(hipEventRecord(m->ev[0], stream), "mlp event");
-        launch_fused(sh, false, X, (long)n, m->blob, m->packed, P, stream);
+        launch_fused(sh, false, bin, X, (long)n, m->blob, m->packed, P, stream);
         MCHK(hipEventRecord(m->ev[1], stream), "mlp event");
         MCHK(hipEventSynchronize(m->ev[1]), "mlp event");
         MCHK(hipEventElapsedTime(&ms, m->ev[0], m->ev[1]), "mlp event");
@@ -626,7 +652,7 @@ static int infer_fused(hyq_mlp* m, const float* X, int64_t n, float* P, hipStrea
       if (best[sh] < best[pick]) pick = sh;
     m->shape = pick;
   }
-  launch_fused(m->shape < 0 ? 0 : m->shape, mask, X, (long)n, m->blob, m->packed, P, stream);
+  launch_fused(m->shape < 0 ? 0 : m->shape, mask, bin, X, (long)n, m->blob, m->packed, P, stream);
   MCHK(hipGetLastError(), "mlp infer launch");
   return HYQ_OK;
 }
@@ -643,6 +669,16 @@ int hyq_mlp_infer_masked_async(hyq_mlp* m, const float* X, int64_t n, float* P,
                                hipStream_t stream) {
   return infer_fused(m, X, n, P, stream, true);
 }
+
+// The same from normalized bf16 rows Xb [n][16] (column 15 the need flag, 0 or 1), as the
+// step's projection kernel writes them with the network's input normalization
+// (hyq_mlp_input_norm); the predictions are the same bits as from the float rows.
+int hyq_mlp_infer_bf16_masked_async(hyq_mlp* m, const short* Xb, int64_t n, float* P,
+                                    hipStream_t stream) {
+  return infer_fused(m, Xb, n, P, stream, true, true);
+}
+// Device pointer to the input normalization: 16 means, then 16 standard deviations.
+const float* hyq_mlp_input_norm(hyq_mlp* m) { return m ? m->blob + OFF_XMU : nullptr; }
 
 int hyq_mlp_set_shape(hyq_mlp* m, int32_t shape) {
   if (!m || shape < -1 || shape >= kFusedShapes) return mfail(HYQ_EINVAL, "bad argument");

@@ -242,3 +242,44 @@ def test_trained_network_reduces_iterations():
     assert (np.abs(v_l - v_n) / scale).max() < 2e-7
     sc = np.abs(M_n).max(axis=(0, 1, 2))
     assert (np.abs(M_l - M_n) / sc).max() < 1e-6
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2])
+def test_bf16_rows_give_the_same_predictions(shape):
+    """The step's projection kernel hands the network its input rows already normalized and
+    rounded to bf16 (the same FP32 subtraction and division, then round to nearest even);
+    the fused kernel fed those rows gives the float rows' predictions bit for bit, masked
+    tiles aside.  The rows here are made on the host with numpy's IEEE FP32 arithmetic."""
+    import ctypes
+    import torch
+    from hyqmom_amd import _lib
+    from hyqmom_amd.warmstart import WarmStartMLP
+    n = 5000
+    X = np.random.default_rng(23).standard_normal((n, 16)).astype(np.float32)
+    X[:, 15] = 1.0
+    X[1600:1632, 15] = 0.0
+    blob = R.random_blob(24)
+    m = WarmStartMLP(blob)
+    m.set_shape(shape)
+    mu = blob[R.NPARAM:R.NPARAM + 16].astype(np.float32)
+    sd = blob[R.NPARAM + 16:R.NPARAM + 32].astype(np.float32)
+    Xn = X.copy()
+    Xn[:, :15] = (X[:, :15] - mu[:15]) / sd[:15]
+    bits = R.bf16(Xn).view(np.uint32) >> 16
+    Xb = torch.from_numpy(bits.astype(np.uint16).view(np.int16).copy()).cuda()
+    Xd = torch.from_numpy(X).cuda()
+    Pf = torch.empty((n, 2), dtype=torch.float32, device="cuda")
+    Pb = torch.empty((n, 2), dtype=torch.float32, device="cuda")
+    L = _lib.lib()
+    for name, src, dst in (("hyq_mlp_infer_masked_async", Xd, Pf),
+                           ("hyq_mlp_infer_bf16_masked_async", Xb, Pb)):
+        fn = getattr(L, name)
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+        fn.restype = ctypes.c_int
+        assert fn(m.handle, src.data_ptr(), n, dst.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    a, b = Pf.cpu().numpy(), Pb.cpu().numpy()
+    skip = np.zeros(n, bool)
+    skip[1600:1632] = True
+    assert np.isnan(b[skip]).all()
+    assert np.array_equal(a[~skip], b[~skip]), np.abs(a - b)[~skip].max()
