@@ -35,8 +35,8 @@ sys.path.insert(0, os.path.join(ROOT, "hyqmom.jl_amd"))
 # HBM bytes per step of pass 1 in the mixed+learned mode (k_pass1a + k_mlp_fused +
 # k_pass1_fast + k_pass1_full), measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE in
 # separate runs, each scaled by what the counter reads for a 1 GiB device copy (bench.py
-# --copy-calibration 1): 1.048 + 0.311 + 1.719 + 0.459 GB, averaged over the first 40 steps
-MEASURED_PASS1_TRAFFIC = {"mixed_learned": 3.54e9}
+# --copy-calibration 1): 0.907 + 0.173 + 1.720 + 0.460 GB, averaged over the first 40 steps
+MEASURED_PASS1_TRAFFIC = {"mixed_learned": 3.26e9}
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 GRID = (120, 120, 60)
