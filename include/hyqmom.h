@@ -179,12 +179,12 @@ int hyq_hll_lines(const double* M, const double* F, const double* vmin, const do
 int hyq_collision_batch(const double* M, int64_t n, double dt, double Kn, int32_t positivity,
                         double* Mout);
 
-/* ---- learned warm start (a 15 -> 128 -> 128 -> 128 -> 2 GELU MLP on bf16 MFMA) ----------
+/* ---- learned warm start (a 15 -> H -> H -> H -> 2 GELU MLP on bf16 MFMA, H = 32) --------
  * No counterpart in the reference: the network predicts the extreme eigenvalues of the
  * standardized flux Jacobian of a plane from its 15 standardized moments; the predictions
  * only seed a Laguerre iteration whose result is exact to FP64 or falls back to the full solve.
  * Parameters travel as one FP32 blob of hyq_mlp_blob_size() floats (weights, biases, input
- * and label normalization). */
+ * and label normalization); hyq_mlp_param_count() = 2 H^2 + 35 H + 16 gives H. */
 typedef struct hyq_mlp hyq_mlp;
 int64_t hyq_mlp_blob_size(void);
 int64_t hyq_mlp_param_count(void);
