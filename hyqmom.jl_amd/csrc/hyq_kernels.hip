@@ -191,13 +191,20 @@ constexpr int kXcds = 8;
 #ifndef HYQ_UPDATE_ROWS
 #define HYQ_UPDATE_ROWS 0
 #endif
+// Threads per block of the update.  Larger blocks (more x-rows per block, so fewer y
+// neighbours fetched by another XCD's blocks) measured slower on the 120x120x60 workload:
+// 0.502 (512) and 0.523 ms (1024) against 0.424 ms (128) for the mixed update.
+#ifndef HYQ_UPDATE_BLOCK
+#define HYQ_UPDATE_BLOCK 128
+#endif
+constexpr int kUpdBlock = HYQ_UPDATE_BLOCK;
 HD inline unsigned update_grid(long long n, int Nx = 0, int Ny = 0, int Nz = 0) {
-  unsigned g = (unsigned)((n + kBlock - 1) / kBlock);
-  if (HYQ_UPDATE_ROWS) g = (unsigned)(((Nx + kBlock - 1) / kBlock) * (long long)Ny * Nz);
+  unsigned g = (unsigned)((n + kUpdBlock - 1) / kUpdBlock);
+  if (HYQ_UPDATE_ROWS) g = (unsigned)(((Nx + kUpdBlock - 1) / kUpdBlock) * (long long)Ny * Nz);
   return (HYQ_XCD_SWIZZLE || HYQ_UPDATE_ROWS) ? (g + kXcds - 1) / kXcds * kXcds : g;
 }
 template <typename FT>
-__global__ __launch_bounds__(kBlock) void k_update(const double* __restrict__ Mr,
+__global__ __launch_bounds__(kUpdBlock) void k_update(const double* __restrict__ Mr,
                                                    const FT* __restrict__ F,
                                                    const FT* __restrict__ vp,
                                                    double* __restrict__ Mout, const DevScalars* sc,
@@ -213,10 +220,10 @@ __global__ __launch_bounds__(kBlock) void k_update(const double* __restrict__ Mr
   long long c;
   int i, j, k;
   if (HYQ_UPDATE_ROWS) {
-    const int rb = (Nx + kBlock - 1) / kBlock;  // blocks per x-row
+    const int rb = (Nx + kUpdBlock - 1) / kUpdBlock;  // blocks per x-row
     const long long row = blk / rb;
     if (row >= (long long)Ny * Nz) return;
-    i = (int)(blk % rb) * kBlock + (int)threadIdx.x;
+    i = (int)(blk % rb) * kUpdBlock + (int)threadIdx.x;
     if (i >= Nx) return;
     k = (int)(row % Nz);
     j = (int)(row / Nz);
@@ -1071,11 +1078,11 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
   hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, c->stream, c->sc, p.CFL, hmin, p.dtmax, p.Kn);
   if (ev) hipEventRecord(ev[2], c->stream);
   if (p.precision == HYQ_PREC_FP64) {
-    hipLaunchKernelGGL(k_update<double>, dim3(update_grid(c->N, p.Nx, p.Ny, p.Nz)), dim3(kBlock), 0, c->stream, c->Mr,
+    hipLaunchKernelGGL(k_update<double>, dim3(update_grid(c->N, p.Nx, p.Ny, p.Nz)), dim3(kUpdBlock), 0, c->stream, c->Mr,
                        (const double*)c->F, (const double*)c->vp, c->M, c->sc, p.Nx, p.Ny, p.Nz,
                        c->dx, c->dy, c->dz);
   } else {
-    hipLaunchKernelGGL(k_update<float>, dim3(update_grid(c->N, p.Nx, p.Ny, p.Nz)), dim3(kBlock), 0, c->stream, c->Mr,
+    hipLaunchKernelGGL(k_update<float>, dim3(update_grid(c->N, p.Nx, p.Ny, p.Nz)), dim3(kUpdBlock), 0, c->stream, c->Mr,
                        (const float*)c->F, (const float*)c->vp, c->M, c->sc, p.Nx, p.Ny, p.Nz,
                        c->dx, c->dy, c->dz);
   }
