@@ -1,5 +1,6 @@
 """Benchmark: cell-steps/s of the HyQMOM 3D step at 120x120x60 (crossing jets, Ma=1.5,
-Kn=0.01, CFL=0.7, 200 steps) on one MI355X, through the C ABI of libhyqmom_hip.so.
+Kn=0.01, CFL=0.7; --steps steps from the initial state, 200 by default) on one MI355X,
+through the C ABI of libhyqmom_hip.so.
 
 Prints one JSON line.  `value` = Nx*Ny*Nz*steps / wall time of the timed steps of the
 headline mode (default: mixed precision with the learned warm start), state resident in
@@ -321,8 +322,7 @@ def main():
     roof = {"kernel": dom, "bound": d["bound"], "achieved": d.get("achieved"), "peak": d["peak"],
             "unit": d["unit"], "frac": d.get("frac"), "traffic": traffic}
     out = {
-        "metric": "cell-steps/s at 120x120x60 (crossing jets, Ma=1.5, Kn=0.01, 200 steps); "
-                  "mixed+learned vs plain FP64",
+        "metric": "cell-steps/s at 120\u00b2\u00d760; mixed+learned vs plain FP64 at max rel err \u22641e-6",
         "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": run["seconds"] * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None,

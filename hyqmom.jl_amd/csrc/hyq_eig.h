@@ -387,34 +387,6 @@ HD bool hqr_n(double* H, double* wr, double* wi) {
   return hqr_core<N>(m, s);
 }
 
-// Where plane_eigs keeps the 6x6 matrix during the QR iteration: registers (hqr6_reg, the
-// default) or, with HYQ_QR_LDS, the LDS slot below.
-#ifndef HYQ_QR_LDS
-#define HYQ_QR_LDS 0
-#endif
-// The LDS form: hqr_core's data-dependent indices would put the matrix in per-lane scratch
-// memory, so on the device it takes a slot of LDS per lane (element-major, lane-minor:
-// conflict-free), provided by the launch as dynamic shared memory of qr_lds_bytes(block)
-// bytes (0 for the register form); on the host a local array.
-HD constexpr unsigned qr_lds_bytes(unsigned block) { return HYQ_QR_LDS ? 36u * 8u * block : 0u; }
-#if defined(__HIP_DEVICE_COMPILE__)
-struct QrMat {
-  double* b;
-  int s;
-  __device__ double& operator()(int i, int j) { return b[(i * 6 + j) * s]; }
-};
-__device__ __forceinline__ QrMat qr_mat() {
-  extern __shared__ double hyq_qr_lds[];
-  return QrMat{hyq_qr_lds + threadIdx.x, (int)blockDim.x};
-}
-#else
-struct QrMat {
-  double a[36];
-  double& operator()(int i, int j) { return a[i * 6 + j]; }
-};
-inline QrMat qr_mat() { return QrMat{}; }
-#endif
-
 struct EigRes {
   double vmin, vmax;  // extreme real parts (raw scale), NaN if not finite
   bool cplx;          // some |imag| > 1000 eps (raw scale)
@@ -437,16 +409,7 @@ HD EigRes plane_eigs(const Plane& p, uint32_t& fl) {
   balance_n<6>(H);
   hessenberg_n<6>(H);
   ExtremeSink ex{0.0, 0.0, 0.0, true};
-#if HYQ_QR_LDS
-  QrMat Q = qr_mat();
-#pragma unroll
-  for (int i = 0; i < 6; ++i)
-#pragma unroll
-    for (int j = 0; j < 6; ++j) Q(i, j) = H[i * 6 + j];
-  const bool conv = hqr_core<6>(Q, ex);
-#else
   const bool conv = hqr6_reg(H, ex);
-#endif
   if (!conv) {
     fl |= F_NOCONV;
     r.vmin = r.vmax = NAN;

@@ -46,7 +46,8 @@ __global__ void ke_begin(RunDev* R, int nr) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nr) return;
   RunDev& q = R[r];
-  const bool act = q.t < q.tmax && q.nn < q.nnmax;
+  // a run whose z wave-speed solve met a non-finite Jacobian stops there (the reference throws)
+  const bool act = q.t < q.tmax && q.nn < q.nnmax && q.err == 0ull;
   q.active = act ? 1.0 : 0.0;
   if (act) q.nn += 1;
   q.vmax_bits = 0ull;
@@ -225,6 +226,7 @@ int hyq_ensemble_set_state(hyq_ensemble* e, int32_t r, const double* M) {
   h.t = 0.0;
   h.nn = 0;
   h.active = 0.0;
+  h.err = 0ull;
   ECHK(hipMemcpy(e->R + r, &h, sizeof(RunDev), hipMemcpyHostToDevice), "run scalars");
   return HYQ_OK;
 }
@@ -270,6 +272,15 @@ int hyq_ensemble_run(hyq_ensemble* e, int64_t nsteps, int32_t use_graph) {
   }
   ECHK(hipGetLastError(), "ensemble launch");
   ECHK(hipStreamSynchronize(e->stream), "ensemble sync");
+  std::vector<RunDev> h(e->nr);
+  ECHK(hipMemcpy(h.data(), e->R, sizeof(RunDev) * e->nr, hipMemcpyDeviceToHost), "run scalars");
+  std::string bad;
+  for (int r = 0; r < e->nr; ++r)
+    if (h[r].err) bad += (bad.empty() ? "" : ", ") + std::to_string(r);
+  if (!bad.empty())
+    return efail(HYQ_ENONFINITE, ("non-finite Jacobian in the z wave-speed solve of run(s) " + bad +
+                                  " (eigenvalues6z_hyperbolic_3D.jl:108,121 throws); they stopped there")
+                                     .c_str());
   return HYQ_OK;
 }
 

@@ -17,6 +17,7 @@ struct RunDev {
   double t, dt, tmax, active;
   unsigned long long vmax_bits;
   long long nn, nnmax;
+  unsigned long long err;  // bit 0: a non-finite z Jacobian (where the reference throws)
 };
 struct Blk {
   int run, pad;
@@ -60,6 +61,7 @@ __global__ __launch_bounds__(B, 2) void ke_pass1(const double* __restrict__ M, d
       vb = bits > vb ? bits : vb;
     }
     flags[c] = fl;
+    if (fl & F_NONFIN) atomicOr(&q->err, 1ull);
   }
   vb = ewave_max(vb);
   if ((threadIdx.x & 63) == 0) atomicMax(&q->vmax_bits, vb);
@@ -115,13 +117,13 @@ void launch_cells(int which, int b, unsigned nblk, const Blk* T, const EnsPtrs& 
 #define HYQ_EL(B)                                                                                    \
   do {                                                                                               \
     if (which == 0)                                                                                  \
-      hipLaunchKernelGGL((ke_pass1<B, FT, EM>), dim3(nblk), dim3(B), qr_lds_bytes(B), s, p.M, p.Mr, F, vp, p.flags, \
+      hipLaunchKernelGGL((ke_pass1<B, FT, EM>), dim3(nblk), dim3(B), 0, s, p.M, p.Mr, F, vp, p.flags, \
                          p.R, T, p.P);                                                               \
     else if (which == 1)                                                                             \
       hipLaunchKernelGGL((ke_update<B, FT>), dim3(nblk), dim3(B), 0, s, p.Mr, F, vp, p.M, p.R, T,     \
                          p.P);                                                                       \
     else                                                                                             \
-      hipLaunchKernelGGL((ke_pass2<B, MIXED, EM>), dim3(nblk), dim3(B), qr_lds_bytes(B), s, p.M, p.R, T, p.P);      \
+      hipLaunchKernelGGL((ke_pass2<B, MIXED, EM>), dim3(nblk), dim3(B), 0, s, p.M, p.R, T, p.P);      \
   } while (0)
   if (b == 64) HYQ_EL(64);
   else if (b == 128) HYQ_EL(128);

@@ -51,51 +51,12 @@ int fail(int code, const char* what, hipError_t e = hipSuccess) {
   } while (0)
 
 constexpr int kBlock = 128;
-// Minimum waves per SIMD the register allocator must allow for the pass-1/pass-2 kernels:
-// more waves hide more latency at the price of spills to scratch.  The QR solve's matrix
-// in LDS (288 B per lane) caps occupancy at 2 anyway; measured on the 120x120x60 workload,
-// 2 beats 1 for every pass kernel, and the fast kernels (EM = 1) are slower at 3 or 4 with
-// their QR fallback in lane-private memory.
-#ifndef HYQ_PASS_WAVES
-#define HYQ_PASS_WAVES 2
-#endif
-#ifndef HYQ_FAST_WAVES
-#define HYQ_FAST_WAVES 2
-#endif
-constexpr int kPassWaves = HYQ_PASS_WAVES, kFastWaves = HYQ_FAST_WAVES;
-// The fast modes' pass 2 as two kernels (checks | BGK) instead of one: less live state per
-// kernel, fewer spills.
-#ifndef HYQ_SPLIT2
-#define HYQ_SPLIT2 1
-#endif
-constexpr bool kSplit2 = HYQ_SPLIT2 != 0;
-// mixed_newton: 1 = common cells and deferred cells in separate kernels, 0 = one kernel per
-// pass (k_pass1<float, 1>, k_pass2<true, 1>)
-#ifndef HYQ_DEFER
-#define HYQ_DEFER 1
-#endif
-constexpr bool kDefer = HYQ_DEFER != 0;
-// Fast modes: run the pass-2 list kernel of step n on a second stream, beside the pass 1 of
-// step n+1 (a cell's pass 1 needs only its own pass-2 result; the listed cells take their
-// pass 1 after the list kernel is done).  Measured on the 120x120x60 workload (100 steps) it
-// hid the list kernel (pass 2 1.03 -> 0.73 ms) but slowed pass 1 by more (1.76 -> 2.09 ms),
-// the list kernel's waves holding registers the fast kernel needs, so it is off by default.
-#ifndef HYQ_OVERLAP
-#define HYQ_OVERLAP 0
-#endif
-constexpr bool kOverlap = HYQ_OVERLAP != 0;
-// mixed_newton: the projection in its own kernel (k_pass1a), then k_pass1_fast from the
-// projected state without seeds, as the learned mode does, instead of one kernel for both.
-// Measured on the 120x120x60 workload (100 steps): pass 1 1.76 -> 1.47 ms.
-#ifndef HYQ_NEWTON_SPLIT
-#define HYQ_NEWTON_SPLIT 1
-#endif
-constexpr bool kNewtonSplit = HYQ_NEWTON_SPLIT != 0;
+// Minimum waves per SIMD the register allocator must allow for the pass kernels: more waves
+// hide more latency at the price of spills to scratch.  Measured on the 120x120x60 workload,
+// 2 beats 1 for every pass kernel, and the fast kernels are slower at 3 or 4.
+constexpr int kPassWaves = 2, kFastWaves = 2;
 
 constexpr int kNev = 5;  // timing events per step
-// dynamic LDS of the kernels that run the QR eigen solve (hyq_eig.h: qr_mat; 0 unless the
-// matrix is kept in LDS)
-constexpr unsigned kQrLds = qr_lds_bytes(kBlock);
 constexpr const char* kNonfiniteMsg =
     "non-finite Jacobian in the z wave-speed solve (eigenvalues6z_hyperbolic_3D.jl:108,121 throws)";
 
@@ -151,15 +112,13 @@ __global__ __launch_bounds__(kBlock, EM == 0 ? kPassWaves : kFastWaves) void k_p
 }
 
 // ---- dt (simulation_runner.jl:402-433, :496) -------------------------------------------
-// reset2 = 0 when the previous step's pass-2 list may still be in use on the second stream
-// (k_reset2 empties it once that is done).
-__global__ void k_step_begin(DevScalars* sc, int reset2) {
+__global__ void k_step_begin(DevScalars* sc) {
   const bool act = (sc->t < sc->tmax) && (sc->nn < sc->nnmax);
   sc->active = act ? 1.0 : 0.0;
   if (act) sc->nn += 1;
   sc->vmax_bits = 0ull;
   sc->nhard1 = 0u;
-  if (reset2) sc->nhard2 = 0u;
+  sc->nhard2 = 0u;
 }
 __global__ void k_dt(DevScalars* sc, double CFL, double hmin, double dtmax, double Kn) {
   if (sc->active == 0.0) return;
@@ -172,37 +131,8 @@ __global__ void k_dt(DevScalars* sc, double CFL, double hmin, double dtmax, doub
 
 // ---- HLL update (the line logic is in hyq_update.h) ----------------------------------
 // FT = double: FP64 fluxes and arithmetic; FT = float: the mixed-precision difference form.
-// The update reads each cell's y and z neighbours (Nx and Nx*Ny cells away).  Workgroups are
-// dealt to the 8 XCDs round-robin, each XCD with its own L2, so with the plain block order a
-// cell's neighbours are mostly fetched by another XCD.  With HYQ_XCD_SWIZZLE the grid is
-// padded to 8 * per blocks and block b takes the cells of block (b % 8) * per + b / 8: each
-// XCD then sweeps one contiguous eighth of the grid, where neighbour rows and planes are its
-// own recent reads.  Measured on the 120x120x60 workload it made the update 2-5% slower
-// (0.432 against 0.423 ms mixed, 0.502 against 0.478 ms FP64), so it is off by default.
-#ifndef HYQ_XCD_SWIZZLE
-#define HYQ_XCD_SWIZZLE 0
-#endif
-constexpr int kXcds = 8;
-// HYQ_UPDATE_ROWS: a block takes one x-row (j, k) of the grid (ceil(Nx / 128) blocks per
-// row), with k varying fastest over consecutive blocks and those blocks kept on one XCD:
-// a cell's z neighbours are then read by blocks that run at the same time on the same L2.
-// Measured on the 120x120x60 workload: 0.419 against 0.423 ms (mixed), 0.503 against
-// 0.478 ms (FP64); off by default.
-#ifndef HYQ_UPDATE_ROWS
-#define HYQ_UPDATE_ROWS 0
-#endif
-// Threads per block of the update.  Larger blocks (more x-rows per block, so fewer y
-// neighbours fetched by another XCD's blocks) measured slower on the 120x120x60 workload:
-// 0.502 (512) and 0.523 ms (1024) against 0.424 ms (128) for the mixed update.
-#ifndef HYQ_UPDATE_BLOCK
-#define HYQ_UPDATE_BLOCK 128
-#endif
-constexpr int kUpdBlock = HYQ_UPDATE_BLOCK;
-HD inline unsigned update_grid(long long n, int Nx = 0, int Ny = 0, int Nz = 0) {
-  unsigned g = (unsigned)((n + kUpdBlock - 1) / kUpdBlock);
-  if (HYQ_UPDATE_ROWS) g = (unsigned)(((Nx + kUpdBlock - 1) / kUpdBlock) * (long long)Ny * Nz);
-  return (HYQ_XCD_SWIZZLE || HYQ_UPDATE_ROWS) ? (g + kXcds - 1) / kXcds * kXcds : g;
-}
+constexpr int kUpdBlock = 128;
+HD inline unsigned update_grid(long long n) { return (unsigned)((n + kUpdBlock - 1) / kUpdBlock); }
 template <typename FT>
 __global__ __launch_bounds__(kUpdBlock) void k_update(const double* __restrict__ Mr,
                                                    const FT* __restrict__ F,
@@ -212,29 +142,11 @@ __global__ __launch_bounds__(kUpdBlock) void k_update(const double* __restrict__
                                                    double dz) {
   if (sc->active == 0.0) return;
   const long long N = (long long)Nx * Ny * Nz;
-  long long blk = blockIdx.x;
-  if (HYQ_XCD_SWIZZLE || HYQ_UPDATE_ROWS) {
-    const unsigned per = gridDim.x / kXcds;  // the grid is a multiple of 8 blocks
-    blk = (long long)(blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
-  }
-  long long c;
-  int i, j, k;
-  if (HYQ_UPDATE_ROWS) {
-    const int rb = (Nx + kUpdBlock - 1) / kUpdBlock;  // blocks per x-row
-    const long long row = blk / rb;
-    if (row >= (long long)Ny * Nz) return;
-    i = (int)(blk % rb) * kUpdBlock + (int)threadIdx.x;
-    if (i >= Nx) return;
-    k = (int)(row % Nz);
-    j = (int)(row / Nz);
-    c = i + (long long)Nx * (j + (long long)Ny * k);
-  } else {
-    c = blk * blockDim.x + threadIdx.x;
-    if (c >= N) return;
-    i = (int)(c % Nx);
-    j = (int)((c / Nx) % Ny);
-    k = (int)(c / ((long long)Nx * Ny));
-  }
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const int i = (int)(c % Nx);
+  const int j = (int)((c / Nx) % Ny);
+  const int k = (int)(c / ((long long)Nx * Ny));
   const double dt = sc->dt;
   update_cell<FT>(Mr, F, vp, Mout, c, i, j, k, Nx, Ny, Nz, N, dt / dx, dt / dy, dt / dz);
 }
@@ -441,11 +353,10 @@ template <bool FEAT>
 __global__ __launch_bounds__(kBlock, kPassWaves) void k_pass1a(const double* __restrict__ M, double* __restrict__ A,
                                                    short* __restrict__ Xb, uint32_t* __restrict__ flags,
                                                    DevScalars* sc, long long N, int flag2D, double Ma,
-                                                   const unsigned char* __restrict__ pend,
                                                    const float* __restrict__ norm) {
   if (sc->active == 0.0) return;
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= N || (pend && pend[c])) return;
+  if (c >= N) return;
   double m[35], a[35];
 #pragma unroll
   for (int t = 0; t < 35; ++t) m[t] = M[(size_t)t * N + c];
@@ -477,19 +388,12 @@ __global__ __launch_bounds__(kBlock, kPassWaves) void k_pass1a(const double* __r
 // `per` consecutive entries (per = ceil(n / waves), at most 64) on its first `per` lanes, and
 // the grid strides by waves * per.  Few cells per wave keep the union of their branch paths,
 // which a wave executes, short; the deferred cells are the rare, branch-heavy ones.
-// compact: 64 entries per wave, so that the list occupies as few wave slots as it can (for a
-// list kernel that runs beside other work).
 struct ListSpread {
   unsigned n, per, wave, lane, stride;
-  __device__ explicit ListSpread(unsigned n_, bool compact = false) : n(n_) {
+  __device__ explicit ListSpread(unsigned n_) : n(n_) {
     const unsigned waves = gridDim.x * (blockDim.x / 64);
-#ifdef HYQ_LIST_PER
-    per = HYQ_LIST_PER;
-#else
     per = (n + waves - 1) / waves;
     per = per < 1 ? 1 : (per > 64 ? 64 : per);
-#endif
-    if (compact) per = 64;
     wave = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
     lane = threadIdx.x & 63;
     stride = waves * per;
@@ -502,104 +406,55 @@ struct ListSpread {
 // without spilling, but on the 120x120x60 workload that was 15-20% slower than 2 (their
 // time is one wave's latency through the full path, and two waves per SIMD share the cells).
 // Grid: that many waves per SIMD on every CU, 128-thread blocks.
-#ifndef HYQ_FULL_WAVES
-#define HYQ_FULL_WAVES 2
-#endif
-constexpr int kFullWaves = HYQ_FULL_WAVES;
+constexpr int kFullWaves = 2;
 constexpr unsigned kListBlocks = 256 * 4 * kFullWaves / 2;
 
 // ---- the fast step with deferred cells (mixed_newton / mixed_learned modes) ----------------
-// Pass 1 of the cells that take no rare branch; the others keep their projected state in Mr
-// and go to the list for k_pass1_full.  PROJECT: the projection runs here (mixed_newton);
-// otherwise k_pass1a already left it in Mr and the network's seeds are in P (learned).
-template <bool PROJECT>
-__device__ __forceinline__ unsigned long long pass1_fast_one(
-    long long c, const double* __restrict__ M, double* __restrict__ AMr, float* __restrict__ F,
-    float* __restrict__ vp, uint32_t* __restrict__ flags, const float* __restrict__ P,
-    int* __restrict__ list, DevScalars* sc, long long N, int flag2D, double Ma, int positivity) {
-  unsigned long long vb = 0;
-  double a[35];
-  uint32_t fl = 0;
-  if (PROJECT) {
-    double m[35];
-#pragma unroll
-    for (int t = 0; t < 35; ++t) m[t] = M[(size_t)t * N + c];
-    pass1a_cell(m, flag2D, Ma, a, fl);
-  } else {
-#pragma unroll
-    for (int t = 0; t < 35; ++t) a[t] = AMr[(size_t)t * N + c];
-    fl = flags[c];
-  }
-  Warm w = no_warm();
-  if (!PROJECT && P != nullptr) {
-#pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      w.w[2 * q] = P[((size_t)q * N + c) * 2];
-      w.w[2 * q + 1] = P[((size_t)q * N + c) * 2 + 1];
-    }
-  }
-  double v6[6];
-  if (fast_cell_speeds(a, positivity != 0, true, w, v6, fl)) {
-    double mr[35], v[6];
-    pass1_finish<float>(a, flag2D, Ma, mr, F + c, F + (size_t)35 * N + c, F + (size_t)70 * N + c,
-                        (size_t)N, v6, v, fl);
-#pragma unroll
-    for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = mr[t];
-#pragma unroll
-    for (int d = 0; d < 6; ++d) {
-      vp[(size_t)d * N + c] = (float)v[d];
-      const unsigned long long b = __double_as_longlong(fabs(v[d]));
-      vb = b > vb ? b : vb;
-    }
-    flags[c] = fl;
-  } else {
-    if (PROJECT) {
-#pragma unroll
-      for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = a[t];
-    }
-    flags[c] = fl & 0xFFFFu;  // the full path counts its own iterations
-    list[atomicAdd(&sc->nhard1, 1u)] = (int)c;
-  }
-  return vb;
-}
-
-// pend (or null): cells whose pass 2 of the previous step is still running on the second
-// stream; they are skipped here and take their pass 1 in k_pass1_pend.
-template <bool PROJECT>
+// Pass 1 of the cells that take no rare branch, from the projected state that k_pass1a left
+// in AMr (and, in the learned mode, the network's seeds in P); the others keep that state and
+// go to the list for k_pass1_full.
 __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_fast(
-    const double* __restrict__ M, double* __restrict__ AMr, float* __restrict__ F, float* __restrict__ vp,
+    double* __restrict__ AMr, float* __restrict__ F, float* __restrict__ vp,
     uint32_t* __restrict__ flags, const float* __restrict__ P, int* __restrict__ list, DevScalars* sc,
-    long long N, int flag2D, double Ma, int positivity, const unsigned char* __restrict__ pend) {
+    long long N, int flag2D, double Ma, int positivity) {
   if (sc->active == 0.0) return;
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long vb = 0;
-  if (c < N && !(pend && pend[c]))
-    vb = pass1_fast_one<PROJECT>(c, M, AMr, F, vp, flags, P, list, sc, N, flag2D, Ma, positivity);
-  vb = wave_max_u64(vb);
-  if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
-}
-
-// Pass 1 of the cells that were still in the previous step's pass-2 list when the fast
-// pass 1 ran (the analytic start; failures go to the pass-1 list).  Then k_reset2 empties the
-// pass-2 list.
-__global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_pend(
-    const double* __restrict__ M, double* __restrict__ AMr, float* __restrict__ F, float* __restrict__ vp,
-    uint32_t* __restrict__ flags, const int* __restrict__ list2, int* __restrict__ list1,
-    DevScalars* sc, long long N, int flag2D, double Ma, int positivity) {
-  if (sc->active == 0.0) return;
-  const ListSpread ls(sc->nhard2);
-  if (ls.idle()) return;
-  unsigned long long vb = 0;
-  for (unsigned i = ls.first(); i < ls.n; i += ls.stride) {
-    if (!ls.mine(i)) continue;
-    const unsigned long long b = pass1_fast_one<true>(list2[i], M, AMr, F, vp, flags, nullptr, list1,
-                                                      sc, N, flag2D, Ma, positivity);
-    vb = b > vb ? b : vb;
+  if (c < N) {
+    double a[35];
+#pragma unroll
+    for (int t = 0; t < 35; ++t) a[t] = AMr[(size_t)t * N + c];
+    uint32_t fl = flags[c];
+    Warm w = no_warm();
+    if (P != nullptr) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        w.w[2 * q] = P[((size_t)q * N + c) * 2];
+        w.w[2 * q + 1] = P[((size_t)q * N + c) * 2 + 1];
+      }
+    }
+    double v6[6];
+    if (fast_cell_speeds(a, positivity != 0, true, w, v6, fl)) {
+      double mr[35], v[6];
+      pass1_finish<float>(a, flag2D, Ma, mr, F + c, F + (size_t)35 * N + c, F + (size_t)70 * N + c,
+                          (size_t)N, v6, v, fl);
+#pragma unroll
+      for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = mr[t];
+#pragma unroll
+      for (int d = 0; d < 6; ++d) {
+        vp[(size_t)d * N + c] = (float)v[d];
+        const unsigned long long b = __double_as_longlong(fabs(v[d]));
+        vb = b > vb ? b : vb;
+      }
+      flags[c] = fl;
+    } else {
+      flags[c] = fl & 0xFFFFu;  // the full path counts its own iterations
+      list[atomicAdd(&sc->nhard1, 1u)] = (int)c;
+    }
   }
   vb = wave_max_u64(vb);
   if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
 }
-__global__ void k_reset2(DevScalars* sc) { sc->nhard2 = 0u; }
 
 // The full pass 1 (QR fallbacks, corrections, z clamp / re-projection) from the projected
 // state in Mr, for the listed cells.  P: the network's seeds (learned mode) or null.
@@ -643,36 +498,14 @@ __global__ __launch_bounds__(kBlock, kFullWaves) void k_pass1_full(
   if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
 }
 
-// Pass 2 of the cells whose x/y planes need no correction; the others go to the list.
-__global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_fast(double* __restrict__ M,
-                                                                  int* __restrict__ list,
-                                                                  DevScalars* sc, long long N,
-                                                                  int flag2D, double Ma,
-                                                                  int positivity, double Kn) {
-  if (sc->active == 0.0) return;
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= N) return;
-  double m[35], o[35];
-#pragma unroll
-  for (int n = 0; n < 35; ++n) m[n] = M[(size_t)n * N + c];
-  uint32_t fl = 0;
-  if (pass2_fast<true>(m, flag2D, Ma, positivity != 0, sc->dt, Kn, o, fl)) {
-#pragma unroll
-    for (int n = 0; n < 35; ++n) M[(size_t)n * N + c] = o[n];
-  } else {
-    list[atomicAdd(&sc->nhard2, 1u)] = (int)c;
-  }
-}
-
-// Split form of k_pass2_fast: projection and the x/y hyperbolicity checks (projected state
-// to A; A[0] = NaN marks a deferred cell), then projection and BGK.
-// pend (or null): marks the listed cells for the next step's pass 1.
+// The fast modes' pass 2 in two kernels: projection and the x/y hyperbolicity checks
+// (projected state to A; A[0] = NaN marks a deferred cell, which goes to the list), then
+// projection and BGK of the other cells.
 __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_check(const double* __restrict__ M,
                                                                    double* __restrict__ A,
                                                                    int* __restrict__ list,
                                                                    DevScalars* sc, long long N,
-                                                                   int flag2D, double Ma,
-                                                                   unsigned char* __restrict__ pend) {
+                                                                   int flag2D, double Ma) {
   if (sc->active == 0.0) return;
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
@@ -688,7 +521,6 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_check(const double
     A[c] = NAN;
     list[atomicAdd(&sc->nhard2, 1u)] = (int)c;
   }
-  if (pend) pend[c] = ok ? 0 : 1;
 }
 
 __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_relax(double* __restrict__ M,
@@ -712,17 +544,13 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_relax(double* __re
 }
 
 // The full pass 2 for the listed cells.
-// On the second stream (overlapped, check_active = 0) the next step may already have changed
-// `active`; the list count alone then says whether there is work (0 after an inactive step),
-// and the list is packed into as few waves as possible.
 __global__ __launch_bounds__(kBlock, kFullWaves) void k_pass2_full(double* __restrict__ M,
                                                                   const int* __restrict__ list,
                                                                   DevScalars* sc, long long N,
                                                                   int flag2D, double Ma,
-                                                                  int positivity, double Kn,
-                                                                  int check_active) {
-  if (check_active && sc->active == 0.0) return;
-  const ListSpread ls(sc->nhard2, check_active == 0);
+                                                                  int positivity, double Kn) {
+  if (sc->active == 0.0) return;
+  const ListSpread ls(sc->nhard2);
   if (ls.idle()) return;
   for (unsigned i = ls.first(); i < ls.n; i += ls.stride) {
     if (!ls.mine(i)) continue;
@@ -892,6 +720,10 @@ unsigned grid_for(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 // ---------------------------------------------------------------------------------------
 // Context
+// Steps that hyq_run queues before it reads the device scalars back: a run stops within one
+// chunk of reaching tmax, however large nnmax is.
+constexpr int kChunk = 64;
+
 struct hyq_ctx {
   hyq_params p;
   long long N;
@@ -903,30 +735,28 @@ struct hyq_ctx {
   void* vp;        // [6][N]
   uint32_t* flags; // [N]
   DevScalars* sc;
-  hipEvent_t ev[5];  // per step: pass 1 | dt | update | pass 2
+  hipEvent_t ev[kNev * kChunk];  // per step: pass 1 | dt | update | pass 2 (a ring of kChunk steps)
   double ms_pass1, ms_update, ms_pass2;
   long long timed_steps;
   hyq_mlp* mlp;  // learned mode: warm-start network (not owned)
   float* X;      // [5N][16] plane features
   float* P;      // [5N][2] predictions
   int *list1, *list2;  // cells deferred to the full pass 1 / pass 2 (fast modes)
-  bool overlap;            // the pass-2 list kernel runs on stream2 (fast modes)
-  hipStream_t stream2;
-  hipEvent_t evA, evB;     // pass 2 checked (stream -> stream2) / list done (stream2 -> stream)
-  unsigned char* pend;     // [N] cells in the pass-2 list still running on stream2
   double* diag;            // [8] results of the diagnostics
 };
 
 extern "C" {
 
 const char* hyq_last_error(void) { return g_err.c_str(); }
-const char* hyq_version(void) { return "hyqmom-mi355x 0.1 (gfx950)"; }
+const char* hyq_version(void) { return "hyqmom-mi355x 0.2 (gfx950)"; }
 
 int hyq_create(const hyq_params* params, hyq_ctx** out) {
   if (!params || !out) return fail(HYQ_EINVAL, "null argument");
   const hyq_params& p = *params;
   if (p.Nmom != 35) return fail(HYQ_EINVAL, "Nmom must be 35");
   if (p.Nx < 1 || p.Ny < 1 || p.Nz < 1) return fail(HYQ_EINVAL, "grid sizes must be >= 1");
+  if ((long long)p.Nx * p.Ny * p.Nz > 0x7FFFFFFFLL)
+    return fail(HYQ_EINVAL, "more than 2^31 - 1 cells");
   if (p.precision < HYQ_PREC_FP64 || p.precision > HYQ_PREC_MIXED_NEWTON)
     return fail(HYQ_EINVAL, "unknown precision mode");
   int ndev = 0;
@@ -957,31 +787,23 @@ int hyq_create(const hyq_params* params, hyq_ctx** out) {
   if (ok && (params->precision == HYQ_PREC_MIXED_NEWTON || params->precision == HYQ_PREC_MIXED_LEARNED))
     ok = hipMalloc(&c->list1, sizeof(int) * c->N) == hipSuccess &&
          hipMalloc(&c->list2, sizeof(int) * c->N) == hipSuccess;
-  c->overlap = ok && kOverlap && kDefer && kSplit2 &&
-               (p.precision == HYQ_PREC_MIXED_NEWTON || p.precision == HYQ_PREC_MIXED_LEARNED);
-  if (c->overlap)
-    ok = hipMalloc(&c->pend, c->N) == hipSuccess &&
-         hipMemset(c->pend, 0, c->N) == hipSuccess &&
-         hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) == hipSuccess &&
-         hipEventCreateWithFlags(&c->evA, hipEventDisableTiming) == hipSuccess &&
-         hipEventCreateWithFlags(&c->evB, hipEventDisableTiming) == hipSuccess;
   if (!ok) {
     hyq_destroy(c);
     return fail(HYQ_ENOMEM, "device allocation failed");
   }
-  for (int i = 0; i < kNev; ++i) hipEventCreate(&c->ev[i]);
+  for (int i = 0; i < kNev * kChunk; ++i) {
+    if ((e = hipEventCreate(&c->ev[i])) != hipSuccess) {
+      hyq_destroy(c);
+      return fail(HYQ_EHIP, "hipEventCreate", e);
+    }
+  }
   *out = c;
   return HYQ_OK;
 }
 
 int hyq_destroy(hyq_ctx* c) {
   if (!c) return HYQ_OK;
-  if (c->stream2) hipStreamSynchronize(c->stream2);
   if (c->stream) hipStreamSynchronize(c->stream);
-  if (c->stream2) hipStreamDestroy(c->stream2);
-  if (c->evA) hipEventDestroy(c->evA);
-  if (c->evB) hipEventDestroy(c->evB);
-  hipFree(c->pend);
   hipFree(c->M);
   hipFree(c->Mr);
   hipFree(c->F);
@@ -993,7 +815,7 @@ int hyq_destroy(hyq_ctx* c) {
   hipFree(c->P);
   hipFree(c->list1);
   hipFree(c->list2);
-  for (int i = 0; i < kNev; ++i)
+  for (int i = 0; i < kNev * kChunk; ++i)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -1016,60 +838,41 @@ int hyq_get_state(hyq_ctx* c, double* M) {
   return HYQ_OK;
 }
 
+// One step on the context stream; ev (or null): kNev events bracketing its parts.
 static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
   const hyq_params& p = c->p;
   const unsigned g = grid_for(c->N);
   const double hmin = jmin(jmin(c->dx, c->dy), c->dz);
-  const unsigned char* pend = c->overlap ? c->pend : nullptr;
-  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, c->stream, c->sc, c->overlap ? 0 : 1);
+  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, c->stream, c->sc);
   if (ev) hipEventRecord(ev[0], c->stream);
+  const bool fast = p.precision == HYQ_PREC_MIXED_NEWTON || p.precision == HYQ_PREC_MIXED_LEARNED;
   if (p.precision == HYQ_PREC_FP64) {
-    hipLaunchKernelGGL((k_pass1<double, 0>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->Mr,
+    hipLaunchKernelGGL((k_pass1<double, 0>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
                        (double*)c->F, (double*)c->vp, c->flags, c->sc, c->N, p.flag2D, p.Ma,
                        p.positivity);
   } else if (p.precision == HYQ_PREC_MIXED) {
-    hipLaunchKernelGGL((k_pass1<float, 0>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->Mr,
-                       (float*)c->F, (float*)c->vp, c->flags, c->sc, c->N, p.flag2D, p.Ma,
-                       p.positivity);
-  } else if (!kDefer && !(p.precision == HYQ_PREC_MIXED_LEARNED && c->mlp != nullptr)) {
-    hipLaunchKernelGGL((k_pass1<float, 1>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->Mr,
+    hipLaunchKernelGGL((k_pass1<float, 0>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
                        (float*)c->F, (float*)c->vp, c->flags, c->sc, c->N, p.flag2D, p.Ma,
                        p.positivity);
   } else {
-    // fast modes: the common cells in one kernel, the deferred ones in a second
+    // fast modes: projection (+ the network's input rows and warm starts in the learned
+    // mode), the common cells, then the deferred ones
     const bool learned = p.precision == HYQ_PREC_MIXED_LEARNED && c->mlp != nullptr;
     if (learned) {
-      // projection + features, then the network's warm starts
       hipLaunchKernelGGL(k_pass1a<true>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
-                         reinterpret_cast<short*>(c->X), c->flags, c->sc, c->N, p.flag2D, p.Ma, pend,
+                         reinterpret_cast<short*>(c->X), c->flags, c->sc, c->N, p.flag2D, p.Ma,
                          hyq_mlp_input_norm(c->mlp));
       const int rc = hyq_mlp_infer_bf16_masked_async(c->mlp, reinterpret_cast<const short*>(c->X),
                                                      5 * c->N, c->P, c->stream);
       if (rc) return rc;
-      hipLaunchKernelGGL(k_pass1_fast<false>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
-                         (float*)c->F, (float*)c->vp, c->flags, (const float*)c->P, c->list1,
-                         c->sc, c->N, p.flag2D, p.Ma, p.positivity, pend);
-    } else if (kNewtonSplit) {
-      hipLaunchKernelGGL(k_pass1a<false>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
-                         reinterpret_cast<short*>(c->X), c->flags, c->sc, c->N, p.flag2D, p.Ma, pend,
-                         (const float*)nullptr);
-      hipLaunchKernelGGL(k_pass1_fast<false>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
-                         (float*)c->F, (float*)c->vp, c->flags, (const float*)nullptr, c->list1,
-                         c->sc, c->N, p.flag2D, p.Ma, p.positivity, pend);
     } else {
-      hipLaunchKernelGGL(k_pass1_fast<true>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
-                         (float*)c->F, (float*)c->vp, c->flags, (const float*)nullptr, c->list1,
-                         c->sc, c->N, p.flag2D, p.Ma, p.positivity, pend);
+      hipLaunchKernelGGL(k_pass1a<false>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
+                         (short*)nullptr, c->flags, c->sc, c->N, p.flag2D, p.Ma, (const float*)nullptr);
     }
-    if (c->overlap) {
-      // the previous step's pass-2 list is done: pass 1 of its cells, then empty the list
-      hipStreamWaitEvent(c->stream, c->evB, 0);
-      hipLaunchKernelGGL(k_pass1_pend, dim3(kListBlocks), dim3(kBlock), 0, c->stream,
-                         (const double*)c->M, c->Mr, (float*)c->F, (float*)c->vp, c->flags,
-                         (const int*)c->list2, c->list1, c->sc, c->N, p.flag2D, p.Ma, p.positivity);
-      hipLaunchKernelGGL(k_reset2, dim3(1), dim3(1), 0, c->stream, c->sc);
-    }
-    hipLaunchKernelGGL(k_pass1_full, dim3(kListBlocks), dim3(kBlock), kQrLds, c->stream, c->Mr,
+    hipLaunchKernelGGL(k_pass1_fast, dim3(g), dim3(kBlock), 0, c->stream, c->Mr, (float*)c->F,
+                       (float*)c->vp, c->flags, learned ? (const float*)c->P : (const float*)nullptr,
+                       c->list1, c->sc, c->N, p.flag2D, p.Ma, p.positivity);
+    hipLaunchKernelGGL(k_pass1_full, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->Mr,
                        (float*)c->F, (float*)c->vp, c->flags,
                        learned ? (const float*)c->P : (const float*)nullptr, (const int*)c->list1,
                        c->sc, c->N, p.flag2D, p.Ma, p.positivity);
@@ -1078,48 +881,28 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
   hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, c->stream, c->sc, p.CFL, hmin, p.dtmax, p.Kn);
   if (ev) hipEventRecord(ev[2], c->stream);
   if (p.precision == HYQ_PREC_FP64) {
-    hipLaunchKernelGGL(k_update<double>, dim3(update_grid(c->N, p.Nx, p.Ny, p.Nz)), dim3(kUpdBlock), 0, c->stream, c->Mr,
+    hipLaunchKernelGGL(k_update<double>, dim3(update_grid(c->N)), dim3(kUpdBlock), 0, c->stream, c->Mr,
                        (const double*)c->F, (const double*)c->vp, c->M, c->sc, p.Nx, p.Ny, p.Nz,
                        c->dx, c->dy, c->dz);
   } else {
-    hipLaunchKernelGGL(k_update<float>, dim3(update_grid(c->N, p.Nx, p.Ny, p.Nz)), dim3(kUpdBlock), 0, c->stream, c->Mr,
+    hipLaunchKernelGGL(k_update<float>, dim3(update_grid(c->N)), dim3(kUpdBlock), 0, c->stream, c->Mr,
                        (const float*)c->F, (const float*)c->vp, c->M, c->sc, p.Nx, p.Ny, p.Nz,
                        c->dx, c->dy, c->dz);
   }
   if (ev) hipEventRecord(ev[3], c->stream);
   if (p.precision == HYQ_PREC_FP64) {
-    hipLaunchKernelGGL((k_pass2<false, 0>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->sc,
+    hipLaunchKernelGGL((k_pass2<false, 0>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->sc,
                        c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
-  } else if (p.precision == HYQ_PREC_MIXED) {
-    hipLaunchKernelGGL((k_pass2<true, 0>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->sc,
-                       c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
-  } else if (!kDefer) {
-    hipLaunchKernelGGL((k_pass2<true, 1>), dim3(g), dim3(kBlock), kQrLds, c->stream, c->M, c->sc,
+  } else if (!fast) {
+    hipLaunchKernelGGL((k_pass2<true, 0>), dim3(g), dim3(kBlock), 0, c->stream, c->M, c->sc,
                        c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
   } else {
-    if (kSplit2) {
-      hipLaunchKernelGGL(k_pass2_check, dim3(g), dim3(kBlock), 0, c->stream, (const double*)c->M,
-                         c->Mr, c->list2, c->sc, c->N, p.flag2D, p.Ma,
-                         c->overlap ? c->pend : (unsigned char*)nullptr);
-      if (c->overlap) {
-        // the listed cells' full pass 2 on stream2, beside this step's BGK and the next
-        // step's pass 1 of the other cells
-        hipEventRecord(c->evA, c->stream);
-        hipStreamWaitEvent(c->stream2, c->evA, 0);
-        hipLaunchKernelGGL(k_pass2_full, dim3(kListBlocks), dim3(kBlock), kQrLds, c->stream2,
-                           c->M, (const int*)c->list2, c->sc, c->N, p.flag2D, p.Ma, p.positivity,
-                           p.Kn, 0);
-        hipEventRecord(c->evB, c->stream2);
-      }
-      hipLaunchKernelGGL(k_pass2_relax, dim3(g), dim3(kBlock), 0, c->stream, c->M,
-                         (const double*)c->Mr, c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
-    } else {
-      hipLaunchKernelGGL(k_pass2_fast, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->list2,
-                         c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
-    }
-    if (!c->overlap)
-      hipLaunchKernelGGL(k_pass2_full, dim3(kListBlocks), dim3(kBlock), kQrLds, c->stream, c->M,
-                         (const int*)c->list2, c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn, 1);
+    hipLaunchKernelGGL(k_pass2_check, dim3(g), dim3(kBlock), 0, c->stream, (const double*)c->M,
+                       c->Mr, c->list2, c->sc, c->N, p.flag2D, p.Ma);
+    hipLaunchKernelGGL(k_pass2_relax, dim3(g), dim3(kBlock), 0, c->stream, c->M,
+                       (const double*)c->Mr, c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
+    hipLaunchKernelGGL(k_pass2_full, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->M,
+                       (const int*)c->list2, c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
   }
   if (ev) hipEventRecord(ev[4], c->stream);
   const hipError_t e = hipGetLastError();
@@ -1135,13 +918,7 @@ static int load_scalars(hyq_ctx* c, double t, long long nn, double tmax, long lo
   s.nn = nn;
   s.nnmax = nnmax;
   HIPCHK(hipMemcpyAsync(c->sc, &s, sizeof(s), hipMemcpyHostToDevice, c->stream), "scalars");
-  // no pass-2 list is in flight between calls (each call joins stream2 before it returns)
-  if (c->overlap) HIPCHK(hipMemsetAsync(c->pend, 0, c->N, c->stream), "pending marks");
   return HYQ_OK;
-}
-// The last step's pass-2 list kernel on stream2 completes before the context stream goes on.
-static void join_stream2(hyq_ctx* c) {
-  if (c->overlap) hipStreamWaitEvent(c->stream, c->evB, 0);
 }
 
 static int accumulate_times(hyq_ctx* c, hipEvent_t* ev) {
@@ -1162,7 +939,6 @@ int hyq_step(hyq_ctx* c, double* t_inout, double* dt_out, int64_t* nn_inout) {
   int rc = load_scalars(c, *t_inout, *nn_inout, c->p.tmax, (long long)*nn_inout + 1);
   if (rc) return rc;
   rc = enqueue_step(c, c->ev);
-  join_stream2(c);
   if (rc) {
     hipStreamSynchronize(c->stream);
     return rc;
@@ -1185,26 +961,33 @@ int hyq_run(hyq_ctx* c, double tmax, int64_t nnmax, double* t_inout, int64_t* nn
   if (!c || !t_inout || !nn_inout) return fail(HYQ_EINVAL, "null argument");
   int rc = load_scalars(c, *t_inout, *nn_inout, tmax, nnmax);
   if (rc) return rc;
-  // launch nnmax - nn steps back to back; steps after t >= tmax are no-ops on the device.
-  // HIP events bracket pass 1, the update and pass 2 of every step.
-  const long long todo = (long long)nnmax - *nn_inout;
-  std::vector<hipEvent_t> ev(kNev * (size_t)(todo > 0 ? todo : 0));
-  for (auto& e : ev) HIPCHK(hipEventCreate(&e), "event create");
-  for (long long s = 0; s < todo; ++s) {
-    rc = enqueue_step(c, &ev[kNev * s]);
-    if (rc) break;
-  }
-  join_stream2(c);
-  if (rc) hipStreamSynchronize(c->stream);
+  // Steps are queued in chunks of kChunk; a step after t >= tmax (or nn >= nnmax) is a no-op
+  // on the device.  After each chunk the scalars come back, and the run ends once a chunk
+  // left it inactive, or on an error.  HIP events from a ring of kChunk steps bracket pass 1,
+  // the update and pass 2 of every step.
+  long long nn0 = *nn_inout;
   DevScalars s;
-  if (rc == HYQ_OK) {
+  std::memset(&s, 0, sizeof(s));
+  s.t = *t_inout;
+  s.nn = nn0;
+  bool more = s.t < tmax && s.nn < (long long)nnmax;
+  while (more) {
+    const long long left = (long long)nnmax - s.nn;
+    const int todo = (int)(left < kChunk ? left : kChunk);
+    for (int q = 0; q < todo && rc == HYQ_OK; ++q) rc = enqueue_step(c, &c->ev[kNev * q]);
+    if (rc) {
+      hipStreamSynchronize(c->stream);
+      return rc;
+    }
     HIPCHK(hipMemcpyAsync(&s, c->sc, sizeof(s), hipMemcpyDeviceToHost, c->stream), "scalars back");
     HIPCHK(hipStreamSynchronize(c->stream), "run sync");
-    const long long ran = s.nn - *nn_inout;
-    for (long long q = 0; q < ran && rc == HYQ_OK; ++q) rc = accumulate_times(c, &ev[kNev * q]);
+    const long long ran = s.nn - nn0;  // active steps of this chunk (they come first)
+    for (long long q = 0; q < ran && rc == HYQ_OK; ++q) rc = accumulate_times(c, &c->ev[kNev * q]);
+    if (rc) return rc;
+    nn0 = s.nn;
+    if (s.err) break;
+    more = s.active != 0.0 && s.t < tmax && s.nn < (long long)nnmax;
   }
-  for (auto& e : ev) hipEventDestroy(e);
-  if (rc) return rc;
   *t_inout = s.t;
   *nn_inout = s.nn;
   if (s.err) return fail(HYQ_ENONFINITE, kNonfiniteMsg);
@@ -1351,7 +1134,7 @@ int hyq_wavespeed_batch(const double* M, int64_t n, double Ma, int32_t flag2D,
   // the three flux outputs must be one [105][n] block for the pass-1 kernel
   if (Fy != Fx + (size_t)35 * n || Fz != Fx + (size_t)70 * n)
     return fail(HYQ_EINVAL, "Fx, Fy, Fz must be consecutive [35][n] blocks");
-  hipLaunchKernelGGL((k_pass1<double, 0>), dim3(grid_for(n)), dim3(kBlock), kQrLds, 0, M, Mr, Fx, vp,
+  hipLaunchKernelGGL((k_pass1<double, 0>), dim3(grid_for(n)), dim3(kBlock), 0, 0, M, Mr, Fx, vp,
                      flags,
                      (DevScalars*)nullptr, (long long)n, flag2D, Ma, positivity);
   HIPCHK(hipGetLastError(), "pass1 launch");
@@ -1380,7 +1163,7 @@ int hyq_plane_dataset(const double* M, int64_t n, float* X, float* L, uint8_t* v
   if (!M || !X || n < 0 || ((L == nullptr) != (valid == nullptr)))
     return fail(HYQ_EINVAL, "bad argument");
   if (n == 0) return HYQ_OK;
-  hipLaunchKernelGGL(k_plane_dataset, dim3(grid_for(n)), dim3(kBlock), kQrLds, 0, M, (long long)n, X, L,
+  hipLaunchKernelGGL(k_plane_dataset, dim3(grid_for(n)), dim3(kBlock), 0, 0, M, (long long)n, X, L,
                      valid);
   HIPCHK(hipGetLastError(), "dataset launch");
   HIPCHK(hipDeviceSynchronize(), "dataset sync");

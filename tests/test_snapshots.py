@@ -23,3 +23,25 @@ def test_gaussian_cells_standardize_to_isserlis_values():
     assert np.isclose(s[names["M400"]], 3.0) and np.isclose(s[names["M110"]], r)
     assert np.isclose(s[names["M220"]], 1 + 2 * r * r)
     np.testing.assert_allclose(S[1, 0, 0], H.maxwellian35(1.0, (0, 0, 0), np.eye(3)), atol=1e-14)
+
+
+def test_writer_streams_each_snapshot(tmp_path):
+    """Each snapshot is in the file as soon as it is written (the archive is complete after
+    every write), with the reference's key names; close() adds n_snapshots and the grid."""
+    from hyqmom_amd.snapshots import SnapshotWriter
+    fname = str(tmp_path / "snap.npz")
+    w = SnapshotWriter(fname, {"Nx": 2, "Ma": 1.5, "name": "jets"}, 3)
+    M = np.arange(2 * 1 * 1 * 35, dtype=float).reshape(2, 1, 1, 35)
+    w.write(M, 0.0, 0, M + 1, M + 2)
+    with np.load(fname) as f:
+        assert set(f.files) == {"meta/snapshot_interval", "meta/params/Nx", "meta/params/Ma",
+                                "meta/params/name", "snapshots/000001/M", "snapshots/000001/t",
+                                "snapshots/000001/step", "snapshots/000001/S", "snapshots/000001/C"}
+        np.testing.assert_array_equal(f["snapshots/000001/C"], M + 1)
+    w.write(2 * M, 0.5, 3, M, M)
+    w.close({"x": np.arange(3.0), "dx": 0.5})
+    with np.load(fname) as f:
+        assert int(f["meta/n_snapshots"]) == 2 and int(f["meta/snapshot_interval"]) == 3
+        np.testing.assert_array_equal(f["snapshots/000002/M"], 2 * M)
+        assert float(f["snapshots/000002/t"]) == 0.5 and int(f["snapshots/000002/step"]) == 3
+        np.testing.assert_array_equal(f["grid/x"], np.arange(3.0))
