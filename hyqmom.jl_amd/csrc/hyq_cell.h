@@ -97,9 +97,14 @@ HD void closure_project(const double* M4, int flag2D, double Ma, ClosureState& q
     const P10 p2 = check2D(P10{s.S300, s.S400, s.S101, s.S201, s.S301, s.S102, s.S202, s.S003, s.S103, s.S004}, fl);
     const P10 p3 = check2D(P10{s.S030, s.S040, s.S011, s.S021, s.S031, s.S012, s.S022, s.S003, s.S013, s.S004}, fl);
     edge_corner(R110, R101, R011, s, p1, p2, p3);
+    // :107-109: realizable_2D returns (S21, S12, S31, S22, S13), which the xz and yz calls
+    // destructure into (S201, S301, S102, S202, S103) and (S021, S031, S012, S022, S013):
+    // the corrected S102 lands in S301 and S301's in S102 (likewise S012 and S031)
     realizable_2D(s.S300, s.S400, s.S110, s.S210, s.S310, s.S120, s.S220, s.S030, s.S130, s.S040, fl);
     realizable_2D(s.S300, s.S400, s.S101, s.S201, s.S301, s.S102, s.S202, s.S003, s.S103, s.S004, fl);
+    { const double t = s.S301; s.S301 = s.S102; s.S102 = t; }
     realizable_2D(s.S030, s.S040, s.S011, s.S021, s.S031, s.S012, s.S022, s.S003, s.S013, s.S004, fl);
+    { const double t = s.S031; s.S031 = s.S012; s.S012 = t; }
   } else {
     const int f220 = realizable_3D(s, fl);
     if (f220) {

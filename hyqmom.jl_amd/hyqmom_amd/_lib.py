@@ -10,6 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.environ.get("HYQMOM_AMD_LIB") or os.path.join(os.path.dirname(HERE), "build", "libhyqmom_hip.so")
 
 HYQ_OK = 0
+HYQ_ENONFINITE = -4
 ERRORS = {-1: "HYQ_EINVAL", -2: "HYQ_ENOMEM", -3: "HYQ_EHIP", -4: "HYQ_ENONFINITE",
           -5: "HYQ_ENOTCONV", -6: "HYQ_ENODEV"}
 PREC = {"fp64": 0, "mixed": 1, "mixed_learned": 2, "mixed_newton": 3}

@@ -1254,12 +1254,17 @@ static void flux_closure35(const double* M4, int flag2D, double Ma, double* Fx, 
     S28 o = s;
     edge_corner_correction(q, o);
     s = o;
+    // :107-109.  realizable_2D returns (S21, S12, S31, S22, S13); the xz and yz calls
+    // destructure that into (S201, S301, S102, S202, S103) and (S021, S031, S012, S022,
+    // S013), so the corrected S102 lands in S301 and vice versa (likewise S012 / S031).
     realizable_2D(s.S300, s.S400, s.S110, s.S210, s.S310, s.S120, s.S220, s.S030, s.S130, s.S040,
                   flags);
     realizable_2D(s.S300, s.S400, s.S101, s.S201, s.S301, s.S102, s.S202, s.S003, s.S103, s.S004,
                   flags);
+    std::swap(s.S301, s.S102);
     realizable_2D(s.S030, s.S040, s.S011, s.S021, s.S031, s.S012, s.S022, s.S003, s.S013, s.S004,
                   flags);
+    std::swap(s.S031, s.S012);
   } else {
     int flag220 = realizable_3D(s, flags);
     int it = 0;
@@ -1304,6 +1309,40 @@ static void flux_closure35(const double* M4, int flag2D, double Ma, double* Fx, 
     if (Fz) Fz[n] = M5(i, j, k + 1);
     if (M4r) M4r[n] = M5(i, j, k);
   }
+}
+
+// Moments5_3D (src/moments/Moments5_3D.jl:16-138): the HyQMOM closure of the 5th-order
+// moments without any realizability projection.  Outputs in the reference's order
+// 500 410 320 230 140 401 302 203 104 311 221 131 212 113 122 050 041 032 023 014 005.
+static const int E21[21][3] = {{5, 0, 0}, {4, 1, 0}, {3, 2, 0}, {2, 3, 0}, {1, 4, 0}, {4, 0, 1},
+                               {3, 0, 2}, {2, 0, 3}, {1, 0, 4}, {3, 1, 1}, {2, 2, 1}, {1, 3, 1},
+                               {2, 1, 2}, {1, 1, 3}, {1, 2, 2}, {0, 5, 0}, {0, 4, 1}, {0, 3, 2},
+                               {0, 2, 3}, {0, 1, 4}, {0, 0, 5}};
+static void moments5_3d(const double* M4, double* M5o, double* C5o, double* S5o) {
+  double C4[35], S4[35];
+  m2cs4(M4, C4, S4);
+  const S28 s{S4[3],  S4[4],  S4[6],  S4[7],  S4[8],  S4[10], S4[11], S4[12], S4[13], S4[14],
+              S4[16], S4[17], S4[18], S4[20], S4[21], S4[22], S4[23], S4[24], S4[25], S4[26],
+              S4[27], S4[28], S4[29], S4[30], S4[31], S4[32], S4[33], S4[34]};
+  const S21 f = hyqmom_3D(s);
+  const double S5[21] = {f.S500, f.S410, f.S320, f.S230, f.S140, f.S401, f.S302,
+                         f.S203, f.S104, f.S311, f.S221, f.S131, f.S212, f.S113,
+                         f.S122, f.S050, f.S041, f.S032, f.S023, f.S014, f.S005};
+  const double sx = std::sqrt(jmax(C4[2], EPS)), sy = std::sqrt(jmax(C4[9], EPS)),
+               sz = std::sqrt(jmax(C4[19], EPS));
+  Cube Cc;
+  std::memset(&Cc, 0, sizeof(Cc));
+  for (int n = 0; n < 35; ++n) Cc(E35[n][0], E35[n][1], E35[n][2]) = C4[n];
+  for (int n = 0; n < 21; ++n) {
+    const int i = E21[n][0], j = E21[n][1], k = E21[n][2];
+    const double c = S5[n] * ipow(sx, i) * ipow(sy, j) * ipow(sz, k);
+    Cc(i, j, k) = c;
+    S5o[n] = S5[n];
+    C5o[n] = c;
+  }
+  Cube M5;
+  central_to_raw(M4[0], M4[1] / M4[0], M4[5] / M4[0], M4[15] / M4[0], Cc, 5, M5);
+  for (int n = 0; n < 21; ++n) M5o[n] = M5(E21[n][0], E21[n][1], E21[n][2]);
 }
 
 // Eigenvalues of a real n x n matrix (n <= 6): balancing, Householder Hessenberg
@@ -2126,6 +2165,23 @@ void orc_eigenvalues6(const double* M, int axis, int flag2D, double Ma, double* 
   if (axis == 3) orc::eigenvalues6z_hyperbolic(M, flag2D, Ma, true, *vmin, *vmax, Mr, &f);
   else orc::eigenvalues6_hyperbolic(M, axis, flag2D, Ma, *vmin, *vmax, Mr, &f);
   if (flags) *flags = f;
+}
+void orc_eigenvalues6z(const double* M, int flag2D, double Ma, int positivity, double* vmin,
+                       double* vmax, double* Mr, uint32_t* flags) {
+  uint32_t f = 0;
+  orc::eigenvalues6z_hyperbolic(M, flag2D, Ma, positivity != 0, *vmin, *vmax, Mr, &f);
+  if (flags) *flags = f;
+}
+void orc_realizable_3D_flags(const double* s28, double* out28, int* flag220, uint32_t* flags) {
+  orc::S28 s;
+  std::memcpy(&s, s28, sizeof(s));
+  uint32_t f = 0;
+  *flag220 = orc::realizable_3D(s, &f);
+  std::memcpy(out28, &s, sizeof(s));
+  *flags = f;
+}
+void orc_moments5_3d(const double* M4, double* M5, double* C5, double* S5) {
+  orc::moments5_3d(M4, M5, C5, S5);
 }
 void orc_closure_and_eigenvalues(const double* mom5, double* out3) {
   orc::closure_and_eigenvalues(mom5, out3[0], out3[1], out3[2]);

@@ -125,6 +125,28 @@ def eigenvalues6(M, axis, flag2D=0, Ma=0.0):
     return vmin.value, vmax.value, Mr, fl.value
 
 
+def eigenvalues6z(M, flag2D=0, Ma=0.0, positivity=True):
+    M = _f(M); Mr = np.zeros(35)
+    vmin = ctypes.c_double(); vmax = ctypes.c_double(); fl = ctypes.c_uint32()
+    lib().orc_eigenvalues6z(_p(M), ctypes.c_int(flag2D), ctypes.c_double(Ma),
+                            ctypes.c_int(1 if positivity else 0), ctypes.byref(vmin),
+                            ctypes.byref(vmax), _p(Mr), ctypes.byref(fl))
+    return vmin.value, vmax.value, Mr, fl.value
+
+
+def realizable_3D_flags(s28):
+    i = _f(s28); o = np.zeros(28); f = ctypes.c_int(0); fl = ctypes.c_uint32(0)
+    lib().orc_realizable_3D_flags(_p(i), _p(o), ctypes.byref(f), ctypes.byref(fl))
+    return o, f.value, fl.value
+
+
+def moments5_3d(M):
+    """Moments5_3D: (M5, C5, S5), 21 each, in the reference's order."""
+    M = _f(M); a, b, c = np.zeros(21), np.zeros(21), np.zeros(21)
+    lib().orc_moments5_3d(_p(M), _p(a), _p(b), _p(c))
+    return a, b, c
+
+
 def closure_and_eigenvalues(mom5):
     m = _f(mom5); o = np.zeros(3)
     lib().orc_closure_and_eigenvalues(_p(m), _p(o))
