@@ -128,12 +128,14 @@ def timed_run(N, precision, M0, steps, warmup, model=None):
     attached = s._mlp is not None
     fl = s.flags()  # pass-1 flags of the last step
     d1, d2 = s.deferred_cells()
+    why = s.fast_path_counts() if precision in ("mixed_newton", "mixed_learned") else None
     s.close()
     from hyqmom_amd._lib import newton_iterations
     return {"seconds": el, "pass1_ms": p1, "update_ms": up, "pass2_ms": p2, "network": attached,
             "last_step": {"fallback_cells_frac": float(((fl >> 12) & 1).mean()),
                           "iterations_per_cell": float(newton_iterations(fl).mean()),
-                          "deferred_cells_frac": [d1 / fl.size, d2 / fl.size]}}
+                          "deferred_cells_frac": [d1 / fl.size, d2 / fl.size],
+                          "fast_path_counts": why}}
 
 
 def surviving_run(N, precision, M0, steps, warmup, model=None, tries=24):

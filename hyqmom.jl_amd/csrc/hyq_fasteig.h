@@ -38,6 +38,8 @@ struct Sturm {
   double p0[7], p1[6], p2[5], p3[4], p4[3], p5[2], p6[1];
   int n;    // members present, 2..7; the last one is gcd(p, p') when n < 7
   bool ok;
+  bool pair;  // not ok only because the last remainder (the constant p6) is ill-conditioned:
+              // p0..p5 are resolved, and p has a nearly double root near the root of p5
 };
 
 // -(a mod b) for deg a = DA, deg b = DA - 1 (two synthetic-division steps), normalized into
@@ -79,6 +81,7 @@ HD void normalize_poly(double* c, bool& ok) {
 
 HD void sturm_build(const double c[7], Sturm& S) {
   S.ok = true;
+  S.pair = false;
 #pragma unroll
   for (int t = 0; t < 7; ++t) S.p0[t] = c[t];
 #pragma unroll
@@ -94,6 +97,7 @@ HD void sturm_build(const double c[7], Sturm& S) {
   if (st == 0) { S.n = 6; st = sturm_rem<2>(S.p4, S.p5, S.p6); }
   if (st == 0) S.n = 7;
   S.ok = st != 2;
+  S.pair = st == 2 && S.n == 6;
 }
 
 // degree of the last member (0 unless the sequence ended at a nontrivial gcd)
@@ -136,6 +140,13 @@ HD int sturm_changes(const Sturm& S, double x, int at_inf) {
 HD int sturm_real_roots(const Sturm& S) {  // distinct real roots on the whole line
   return sturm_changes(S, 0.0, -1) - sturm_changes(S, 0.0, 1);
 }
+// The same count for a chain whose last member is the constant s6 (S.pair chains).
+HD int sturm_real_roots_p6(const Sturm& S, double s6) {
+  Sturm T = S;
+  T.p6[0] = s6;
+  T.n = 7;
+  return sturm_real_roots(T);
+}
 HD int sturm_roots_above(const Sturm& S, double x) {  // distinct roots in (x, +inf)
   return sturm_changes(S, x, 0) - sturm_changes(S, 0.0, 1);
 }
@@ -169,6 +180,145 @@ HD int laguerre_extreme(const double c[7], double& x, int kmax, bool simple) {
   return -1;
 }
 
+// The nearly double eigenvalue of a plane Jacobian whose Sturm chain is resolved up to p5.
+// A characteristic polynomial cannot separate two roots closer than about sqrt(eps) (its
+// coefficients carry rounding of order eps), but the matrix can: inverse subspace
+// iteration with J - zI, z the root of p5, converges to the invariant subspace of the pair
+// (the other four eigenvalues are resolved by the chain and lie far from z), and the 2x2
+// block B = Q^T J Q on that subspace has the pair as its eigenvalues, to the rounding of
+// J's entries.  Returns +1: a real pair; 0: a complex pair with |imag| (raw scale, sig
+// times the standardized value) clearly above the reference's 1000 eps test
+// (eigenvalues6_hyperbolic_3D.jl:35); -1: not decided here (within the rounding bound of
+// either case, or a singular shift); 2: a complex pair whose |imag| is clearly below the
+// 1000 eps test (treated as real by the reference's decision).
+HD int pair_block_decision(const double* J, double z, double sig) {
+  double A[36];
+#pragma unroll
+  for (int t = 0; t < 36; ++t) A[t] = J[t];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) A[i * 7] -= z;
+  int piv[6];
+  // LU with partial pivoting, in place
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    int pk = k;
+    double best = fabs(A[k * 6 + k]);
+#pragma unroll
+    for (int i = k + 1; i < 6; ++i)
+      if (fabs(A[i * 6 + k]) > best) {
+        best = fabs(A[i * 6 + k]);
+        pk = i;
+      }
+    piv[k] = pk;
+    if (!(best > 0.0)) return -1;
+    if (pk != k) {
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        const double t = A[k * 6 + j];
+        A[k * 6 + j] = A[pk * 6 + j];
+        A[pk * 6 + j] = t;
+      }
+    }
+    const double inv = 1.0 / A[k * 6 + k];
+#pragma unroll
+    for (int i = k + 1; i < 6; ++i) {
+      const double l = A[i * 6 + k] * inv;
+      A[i * 6 + k] = l;
+#pragma unroll
+      for (int j = k + 1; j < 6; ++j) A[i * 6 + j] -= l * A[k * 6 + j];
+    }
+  }
+  // two generic start vectors, three inverse iterations, Gram-Schmidt after each
+  double X[2][6] = {{1.0, 0.8, 0.6, 0.4, 0.2, 0.1}, {0.1, -0.3, 0.5, -0.7, 0.9, -1.1}};
+  for (int it = 0; it < 3; ++it) {
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      double* x = X[v];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {  // apply the row exchanges, then L, then U
+        const double t = x[k];
+        double tk = x[k];
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+          if (q == piv[k]) tk = x[q];
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+          if (q == piv[k]) x[q] = t;
+        x[k] = tk;
+      }
+#pragma unroll
+      for (int i = 1; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < i; ++j) x[i] -= A[i * 6 + j] * x[j];
+#pragma unroll
+      for (int i = 5; i >= 0; --i) {
+#pragma unroll
+        for (int j = i + 1; j < 6; ++j) x[i] -= A[i * 6 + j] * x[j];
+        x[i] /= A[i * 6 + i];
+      }
+    }
+    double n0 = 0.0;
+#pragma unroll
+    for (int t = 0; t < 6; ++t) n0 += X[0][t] * X[0][t];
+    n0 = 1.0 / sqrt(n0);
+    double d = 0.0;
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+      X[0][t] *= n0;
+      d += X[0][t] * X[1][t];
+    }
+    double n1 = 0.0;
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+      X[1][t] -= d * X[0][t];
+      n1 += X[1][t] * X[1][t];
+    }
+    if (!(n1 > 0.0) || !(n1 < 1e300)) return -1;
+    n1 = 1.0 / sqrt(n1);
+#pragma unroll
+    for (int t = 0; t < 6; ++t) X[1][t] *= n1;
+  }
+  // B = Q^T J Q and its discriminant
+  double JQ[2][6];
+  double jn = 0.0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      a0 += J[i * 6 + j] * X[0][j];
+      a1 += J[i * 6 + j] * X[1][j];
+      jn += J[i * 6 + j] * J[i * 6 + j];
+    }
+    JQ[0][i] = a0;
+    JQ[1][i] = a1;
+  }
+  double b[2][2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      double acc = 0.0;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) acc += X[r][i] * JQ[q][i];
+      b[r][q] = acc;
+    }
+  const double dd = b[0][0] - b[1][1];
+  const double disc = dd * dd + 4.0 * b[0][1] * b[1][0];
+  // rounding bound: entries of B within e of the exact block (Frobenius norm of J times a
+  // generous multiple of eps, covering the subspace's residual after three iterations)
+  const double e = 256.0 * EPS * sqrt(jn);
+  const double err = 4.0 * fabs(dd) * e + 8.0 * (fabs(b[0][1]) + fabs(b[1][0])) * e + 16.0 * e * e;
+  if (!(disc == disc)) return -1;
+  if (disc > err) return 1;
+  if (disc < -err) {
+    const double im = sig * 0.5 * sqrt(-disc);
+    if (im > 4.0 * 1000 * EPS) return 0;
+    if (im < 0.25 * 1000 * EPS) return 2;
+  }
+  return -1;
+}
+
 // A warm start w is moved outward by this margin (relative, plus absolute) and used only if
 // the Sturm count proves the moved point lies outside all roots.
 constexpr double kWarmMargin = 0.005;
@@ -176,13 +326,19 @@ constexpr double kWarmMargin = 0.005;
 struct FastEig {
   double lmin, lmax;  // standardized extreme eigenvalues
   int iters;          // root-polishing iterations (both ends)
-  bool real;          // proven: all six eigenvalues real
+  bool real;          // proven: no eigenvalue with |imag| above the reference's test
+  bool cplx;          // proven: a complex pair above the reference's test
 };
 // Standardized fast solve.  wmin/wmax: warm starts (NaN = none).  need_vals=false only
-// proves hyperbolicity (pass 2).
-HD FastEig fast_eigs_std(const double* J, double wmin, double wmax, bool need_vals) {
+// proves hyperbolicity (pass 2).  sig: the plane's standard deviation (the raw scale of the
+// imaginary parts).  PAIR: decide a nearly double pair with the 2x2 block test; without it
+// such a plane is left undecided (kernels that must stay small hand it on).
+template <bool PAIR = true>
+HD FastEig fast_eigs_std(const double* J, double wmin, double wmax, bool need_vals,
+                         double sig = 1.0) {
   FastEig r;
   r.real = false;
+  r.cplx = false;
   r.iters = 0;
   r.lmin = r.lmax = NAN;
   double c[7];
@@ -191,10 +347,31 @@ HD FastEig fast_eigs_std(const double* J, double wmin, double wmax, bool need_va
     if (!(fabs(c[t]) < 1e300)) return r;
   Sturm S;
   sturm_build(c, S);
-  if (!S.ok) return r;
-  // all six roots real <=> distinct real roots = 6 - deg gcd(p, p')
-  const int nreal = sturm_real_roots(S);
-  if (nreal != 6 - sturm_gcd_degree(S)) return r;
+  int nreal;
+  bool pair = false;
+  if (S.ok) {
+    // all six roots real <=> distinct real roots = 6 - deg gcd(p, p')
+    nreal = sturm_real_roots(S);
+    if (nreal != 6 - sturm_gcd_degree(S)) {
+      r.cplx = true;  // a resolved complex pair: the eigenvalue correction's path decides
+      return r;
+    }
+  } else {
+    if (!PAIR || !S.pair) return r;
+    // p0..p5 resolved: four simple real roots and a nearly double pair near the root of p5,
+    // real when the unresolved p6 takes one sign and complex with the other
+    const int np = sturm_real_roots_p6(S, 1.0), nm = sturm_real_roots_p6(S, -1.0);
+    if (!((np == 6 && nm == 4) || (np == 4 && nm == 6))) return r;
+    if (!(fabs(S.p5[1]) > 0.0)) return r;
+    const int d = pair_block_decision(J, -S.p5[0] / S.p5[1], sig);
+    if (d < 0) return r;
+    if (d == 0) {
+      r.cplx = true;
+      return r;
+    }
+    pair = true;
+    nreal = 6;
+  }
   r.real = true;
   if (!need_vals) return r;
   // Samuelson bounds: all roots in mu +- sqrt(5) s
@@ -210,7 +387,7 @@ HD FastEig fast_eigs_std(const double* J, double wmin, double wmax, bool need_va
     const double g = wmax + kWarmMargin * (1.0 + fabs(wmax));
     if (g < x1 && sturm_roots_above(S, g) == 0) x1 = g;
   }
-  const bool simple = sturm_gcd_degree(S) == 0;  // no multiple root anywhere
+  const bool simple = S.ok && sturm_gcd_degree(S) == 0;  // no multiple root anywhere
   const int k0 = laguerre_extreme(c, x0, 40, simple), k1 = laguerre_extreme(c, x1, 40, simple);
   r.iters = (k0 < 0 ? 40 : k0) + (k1 < 0 ? 40 : k1);
   if (k0 < 0 || k1 < 0) {
@@ -219,6 +396,11 @@ HD FastEig fast_eigs_std(const double* J, double wmin, double wmax, bool need_va
   }
   r.lmin = x0;
   r.lmax = x1;
+  if (pair) {
+    // the pair must be interior: an extreme root nearly double is left to the full solve
+    const double z = -S.p5[0] / S.p5[1];
+    if (!(z > x0 + 1e-3 * (1.0 + fabs(x0)) && z < x1 - 1e-3 * (1.0 + fabs(x1)))) r.real = false;
+  }
   return r;
 }
 

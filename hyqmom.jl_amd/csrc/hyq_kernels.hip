@@ -65,7 +65,10 @@ struct DevScalars {  // device-resident step scalars
   unsigned long long vmax_bits;
   long long nn, nnmax;
   unsigned long long err;  // bit 0: a non-finite z Jacobian (where the reference throws)
-  unsigned int nhard1, nhard2;  // cells deferred to the full pass 1 / pass 2 this step
+  // fast modes, this step: cells on the full pass 1 / pass 2 (nhard), cells with pending
+  // planes (nslot), pending planes (nitem), and cells a pending plane sent to the full path
+  // (ncplx: a complex pair, the eigenvalue correction)
+  unsigned int nhard1, nhard2, nslot1, nslot2, nitem1, nitem2, ncplx1, ncplx2;
 };
 
 // ---- pass 1 -------------------------------------------------------------------------
@@ -119,6 +122,7 @@ __global__ void k_step_begin(DevScalars* sc) {
   sc->vmax_bits = 0ull;
   sc->nhard1 = 0u;
   sc->nhard2 = 0u;
+  sc->nslot1 = sc->nslot2 = sc->nitem1 = sc->nitem2 = sc->ncplx1 = sc->ncplx2 = 0u;
 }
 __global__ void k_dt(DevScalars* sc, double CFL, double hmin, double dtmax, double Kn) {
   if (sc->active == 0.0) return;
@@ -406,16 +410,47 @@ struct ListSpread {
 // without spilling, but on the 120x120x60 workload that was 15-20% slower than 2 (their
 // time is one wave's latency through the full path, and two waves per SIMD share the cells).
 // Grid: that many waves per SIMD on every CU, 128-thread blocks.
-constexpr int kFullWaves = 2;
+constexpr int kFullWaves = 2, kSolveWaves = 2;
 constexpr unsigned kListBlocks = 256 * 4 * kFullWaves / 2;
 
-// ---- the fast step with deferred cells (mixed_newton / mixed_learned modes) ----------------
-// Pass 1 of the cells that take no rare branch, from the projected state that k_pass1a left
-// in AMr (and, in the learned mode, the network's seeds in P); the others keep that state and
-// go to the list for k_pass1_full.
+// ---- the fast step (mixed_newton / mixed_learned modes) ------------------------------------
+// A cell whose planes the fast solve proves hyperbolic finishes in the fast kernel.  A plane
+// the small fast kernel cannot decide (a nearly double eigenvalue, whose pair the 2x2 block
+// test decides, or an inconclusive Sturm chain, which the QR decides) leaves its cell
+// pending: the fast kernel still computes the cell's closure, fluxes and 1D speeds, and
+// stores the projected state and the proven planes' extremes in a slot.  A slot kernel then
+// decides the pending planes, and a fix-up kernel combines the cell's speeds, unless a
+// plane has a complex pair: the eigenvalue correction changes the state, and the cell takes
+// the full path, as do cells whose z path clamps the density or re-projects and cells with
+// a non-finite plane.  Every cell gives the values the full path would: the same functions
+// on the same inputs.  The heavy deciding code stays out of the kernels that every cell
+// runs, which keeps their registers (and spills) down.
+struct FastLists {
+  int* cells;          // [N] cells for the full path
+  int* slot_cell;      // [N] cell of each slot
+  unsigned* slot_mask; // [N] pending planes (bits 0..4)
+  unsigned* slot_bad;  // [N] nonzero: a pending plane has a complex pair or is not finite
+  uint32_t* slot_fl;   // [N] the cell's flags
+  double* slot_v;      // [16][N] pass 1: plane extremes (10), 1D speeds (6)
+  double* slot_A;      // [35][N] the projected state the planes come from
+};
+__device__ __forceinline__ unsigned new_slot(const FastLists& L, unsigned* count, long long c,
+                                             unsigned pend, const double* A, long long N) {
+  const unsigned slot = atomicAdd(count, 1u);
+  L.slot_cell[slot] = (int)c;
+  L.slot_mask[slot] = pend;
+  L.slot_bad[slot] = 0u;
+#pragma unroll
+  for (int t = 0; t < 35; ++t) L.slot_A[(size_t)t * N + slot] = A[t];
+  return slot;
+}
+
+// Pass 1 from the projected state that k_pass1a left in AMr (and, in the learned mode, the
+// network's seeds in P).
+template <typename FT>
 __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_fast(
-    double* __restrict__ AMr, float* __restrict__ F, float* __restrict__ vp,
-    uint32_t* __restrict__ flags, const float* __restrict__ P, int* __restrict__ list, DevScalars* sc,
+    double* __restrict__ AMr, FT* __restrict__ F, FT* __restrict__ vp,
+    uint32_t* __restrict__ flags, const float* __restrict__ P, FastLists L, DevScalars* sc,
     long long N, int flag2D, double Ma, int positivity) {
   if (sc->active == 0.0) return;
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -433,35 +468,125 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_fast(
         w.w[2 * q + 1] = P[((size_t)q * N + c) * 2 + 1];
       }
     }
-    double v6[6];
-    if (fast_cell_speeds(a, positivity != 0, true, w, v6, fl)) {
-      double mr[35], v[6];
-      pass1_finish<float>(a, flag2D, Ma, mr, F + c, F + (size_t)35 * N + c, F + (size_t)70 * N + c,
-                          (size_t)N, v6, v, fl);
+    double v[10];
+    unsigned pend;
+    const int st = fast_cell_planes<false, true>(a, positivity != 0, w, v, pend, fl);
+    if (st == kPlaneFull) {
+      L.cells[atomicAdd(&sc->nhard1, 1u)] = (int)c;
+    } else {
+      const unsigned slot = st == kPlaneSolve ? new_slot(L, &sc->nslot1, c, pend, a, N) : 0u;
+      double mr[35], s1[6];
+      flux_closure35<true, FT>(a, flag2D, Ma, mr, F + c, F + (size_t)35 * N + c,
+                               F + (size_t)70 * N + c, (size_t)N, fl);
+      closure1d_all(mr, s1);
+      if (st == kPlaneProven) {
+        double v6[6], vd[6];
+        combine_planes(v, v6);
+        combine_vp(s1, v6, vd);
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+          vp[(size_t)d * N + c] = (FT)vd[d];
+          const unsigned long long b = __double_as_longlong(fabs(vd[d]));
+          vb = b > vb ? b : vb;
+        }
+        flags[c] = fl;
+      } else {
+        L.slot_fl[slot] = fl;
+#pragma unroll
+        for (int t = 0; t < 10; ++t) L.slot_v[(size_t)t * N + slot] = v[t];
+#pragma unroll
+        for (int t = 0; t < 6; ++t) L.slot_v[(size_t)(10 + t) * N + slot] = s1[t];
+      }
 #pragma unroll
       for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = mr[t];
-#pragma unroll
-      for (int d = 0; d < 6; ++d) {
-        vp[(size_t)d * N + c] = (float)v[d];
-        const unsigned long long b = __double_as_longlong(fabs(v[d]));
-        vb = b > vb ? b : vb;
-      }
-      flags[c] = fl;
-    } else {
-      flags[c] = fl & 0xFFFFu;  // the full path counts its own iterations
-      list[atomicAdd(&sc->nhard1, 1u)] = (int)c;
     }
   }
   vb = wave_max_u64(vb);
   if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
 }
 
-// The full pass 1 (QR fallbacks, corrections, z clamp / re-projection) from the projected
-// state in Mr, for the listed cells.  P: the network's seeds (learned mode) or null.
+// The pending planes of every slot: the fast solve with the 2x2 block test of a nearly double
+// pair, the QR where that does not decide.  VALS: pass 1 (extremes into the slot; P: the
+// network's seeds or null); pass 2 only needs the complex-pair decision.
+template <bool VALS>
+__global__ __launch_bounds__(kBlock, kSolveWaves) void k_slot_solve(FastLists L, const unsigned* count,
+                                                                    unsigned* nplanes,
+                                                                    const float* __restrict__ P,
+                                                                    DevScalars* sc, long long N) {
+  if (sc->active == 0.0) return;
+  const unsigned n = *count;
+  for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    atomicAdd(nplanes, (unsigned)__popc(L.slot_mask[s]));
+    double A[35], v[10];
+#pragma unroll
+    for (int t = 0; t < 35; ++t) A[t] = L.slot_A[(size_t)t * N + s];
+    Warm w = no_warm();
+    if (VALS) {
+#pragma unroll
+      for (int t = 0; t < 10; ++t) v[t] = L.slot_v[(size_t)t * N + s];
+      if (P != nullptr) {
+        const long long c = L.slot_cell[s];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          w.w[2 * q] = P[((size_t)q * N + c) * 2];
+          w.w[2 * q + 1] = P[((size_t)q * N + c) * 2 + 1];
+        }
+      }
+    }
+    uint32_t fl = VALS ? L.slot_fl[s] : 0u;
+    if (!solve_pending_planes(A, L.slot_mask[s], VALS, w, v, fl)) {
+      L.slot_bad[s] = 1u;
+      continue;
+    }
+    if (VALS) {
+#pragma unroll
+      for (int t = 0; t < 10; ++t) L.slot_v[(size_t)t * N + s] = v[t];
+      L.slot_fl[s] = fl;
+    }
+  }
+}
+
+// Pass 1 of the cells with plane solves: their speeds, or the full path after a complex pair.
+template <typename FT>
+__global__ __launch_bounds__(kBlock) void k_pass1_fixup(FastLists L, FT* __restrict__ vp,
+                                                        uint32_t* __restrict__ flags, DevScalars* sc,
+                                                        long long N) {
+  if (sc->active == 0.0) return;
+  const unsigned n = sc->nslot1;
+  unsigned long long vb = 0;
+  for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    const long long c = L.slot_cell[s];
+    if (L.slot_bad[s]) {
+      atomicAdd(&sc->ncplx1, 1u);
+      L.cells[atomicAdd(&sc->nhard1, 1u)] = (int)c;
+      continue;
+    }
+    double v[10], s1[6], v6[6], vd[6];
+#pragma unroll
+    for (int t = 0; t < 10; ++t) v[t] = L.slot_v[(size_t)t * N + s];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) s1[t] = L.slot_v[(size_t)(10 + t) * N + s];
+    combine_planes(v, v6);
+    combine_vp(s1, v6, vd);
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+      vp[(size_t)d * N + c] = (FT)vd[d];
+      const unsigned long long b = __double_as_longlong(fabs(vd[d]));
+      vb = b > vb ? b : vb;
+    }
+    flags[c] = L.slot_fl[s];
+  }
+  vb = wave_max_u64(vb);
+  if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
+}
+
+// The full pass 1 (projection, QR fallbacks, corrections, z clamp / re-projection) from the
+// state M, for the listed cells.  P: the network's seeds (learned mode) or null.
+template <typename FT>
 __global__ __launch_bounds__(kBlock, kFullWaves) void k_pass1_full(
-    double* __restrict__ AMr, float* __restrict__ F, float* __restrict__ vp, uint32_t* __restrict__ flags,
-    const float* __restrict__ P, const int* __restrict__ list, DevScalars* sc, long long N, int flag2D,
-    double Ma, int positivity) {
+    const double* __restrict__ M, double* __restrict__ AMr, FT* __restrict__ F,
+    FT* __restrict__ vp, uint32_t* __restrict__ flags, const float* __restrict__ P,
+    const int* __restrict__ list, DevScalars* sc, long long N, int flag2D, double Ma, int positivity) {
   if (sc->active == 0.0) return;
   const ListSpread ls(sc->nhard1);
   if (ls.idle()) return;  // wave-uniform
@@ -469,9 +594,9 @@ __global__ __launch_bounds__(kBlock, kFullWaves) void k_pass1_full(
   for (unsigned i = ls.first(); i < ls.n; i += ls.stride) {
     if (!ls.mine(i)) continue;
     const long long c = list[i];
-    double a[35], mr[35], v[6];
+    double m[35], mr[35], v[6];
 #pragma unroll
-    for (int t = 0; t < 35; ++t) a[t] = AMr[(size_t)t * N + c];
+    for (int t = 0; t < 35; ++t) m[t] = M[(size_t)t * N + c];
     Warm w = no_warm();
     if (P != nullptr) {
 #pragma unroll
@@ -480,14 +605,14 @@ __global__ __launch_bounds__(kBlock, kFullWaves) void k_pass1_full(
         w.w[2 * q + 1] = P[((size_t)q * N + c) * 2 + 1];
       }
     }
-    uint32_t fl = flags[c];
-    pass1b_cell<float, 1>(a, flag2D, Ma, positivity != 0, mr, F + c, F + (size_t)35 * N + c,
-                          F + (size_t)70 * N + c, (size_t)N, v, fl, w);
+    uint32_t fl = 0;
+    pass1_cell<FT, 1>(m, flag2D, Ma, positivity != 0, mr, F + c, F + (size_t)35 * N + c,
+                         F + (size_t)70 * N + c, (size_t)N, v, fl, w);
 #pragma unroll
     for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = mr[t];
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
-      vp[(size_t)d * N + c] = (float)v[d];
+      vp[(size_t)d * N + c] = (FT)v[d];
       const unsigned long long b = __double_as_longlong(fabs(v[d]));
       vb = b > vb ? b : vb;
     }
@@ -498,14 +623,14 @@ __global__ __launch_bounds__(kBlock, kFullWaves) void k_pass1_full(
   if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
 }
 
-// The fast modes' pass 2 in two kernels: projection and the x/y hyperbolicity checks
-// (projected state to A; A[0] = NaN marks a deferred cell, which goes to the list), then
-// projection and BGK of the other cells.
+// The fast modes' pass 2: projection and the x/y hyperbolicity checks (projected state to
+// A; A[0] = NaN marks a cell that k_pass2_relax leaves alone), BGK of the proven cells, the
+// plane solves, the fix-up of the cells with plane solves, and the full pass 2 of the rest.
 __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_check(const double* __restrict__ M,
                                                                    double* __restrict__ A,
-                                                                   int* __restrict__ list,
-                                                                   DevScalars* sc, long long N,
-                                                                   int flag2D, double Ma) {
+                                                                   FastLists L, DevScalars* sc,
+                                                                   long long N, int flag2D,
+                                                                   double Ma) {
   if (sc->active == 0.0) return;
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
@@ -513,14 +638,18 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_check(const double
 #pragma unroll
   for (int n = 0; n < 35; ++n) m[n] = M[(size_t)n * N + c];
   uint32_t fl = 0;
-  const bool ok = pass2_check(m, flag2D, Ma, a, fl);
-  if (ok) {
+  unsigned pend;
+  const int st = pass2_check_planes<false>(m, flag2D, Ma, a, pend, fl);
+  if (st == kPlaneProven) {
 #pragma unroll
     for (int n = 0; n < 35; ++n) A[(size_t)n * N + c] = a[n];
-  } else {
-    A[c] = NAN;
-    list[atomicAdd(&sc->nhard2, 1u)] = (int)c;
+    return;
   }
+  A[c] = NAN;
+  if (st == kPlaneFull)
+    L.cells[atomicAdd(&sc->nhard2, 1u)] = (int)c;
+  else
+    new_slot(L, &sc->nslot2, c, pend, a, N);
 }
 
 __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_relax(double* __restrict__ M,
@@ -541,6 +670,31 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_relax(double* __re
   pass2_relax<true>(a, flag2D, Ma, positivity != 0, sc->dt, Kn, o, fl);
 #pragma unroll
   for (int n = 0; n < 35; ++n) M[(size_t)n * N + c] = o[n];
+}
+
+// Pass 2 of the cells with plane solves: BGK, or the full path after a complex pair.
+__global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_fixup(double* __restrict__ M,
+                                                                   FastLists L, DevScalars* sc,
+                                                                   long long N, int flag2D,
+                                                                   double Ma, int positivity,
+                                                                   double Kn) {
+  if (sc->active == 0.0) return;
+  const unsigned n = sc->nslot2;
+  for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    const long long c = L.slot_cell[s];
+    if (L.slot_bad[s]) {
+      atomicAdd(&sc->ncplx2, 1u);
+      L.cells[atomicAdd(&sc->nhard2, 1u)] = (int)c;
+      continue;
+    }
+    double a[35], o[35];
+#pragma unroll
+    for (int t = 0; t < 35; ++t) a[t] = L.slot_A[(size_t)t * N + s];
+    uint32_t fl = 0;
+    pass2_relax<true>(a, flag2D, Ma, positivity != 0, sc->dt, Kn, o, fl);
+#pragma unroll
+    for (int t = 0; t < 35; ++t) M[(size_t)t * N + c] = o[t];
+  }
 }
 
 // The full pass 2 for the listed cells.
@@ -741,7 +895,7 @@ struct hyq_ctx {
   hyq_mlp* mlp;  // learned mode: warm-start network (not owned)
   float* X;      // [5N][16] plane features
   float* P;      // [5N][2] predictions
-  int *list1, *list2;  // cells deferred to the full pass 1 / pass 2 (fast modes)
+  FastLists L;             // fast modes: full-path list, slots and plane items
   double* diag;            // [8] results of the diagnostics
 };
 
@@ -784,9 +938,16 @@ int hyq_create(const hyq_params* params, hyq_ctx** out) {
   if (ok && p.precision == HYQ_PREC_MIXED_LEARNED)
     ok = hipMalloc(&c->X, sizeof(float) * 16 * 5 * c->N) == hipSuccess &&
          hipMalloc(&c->P, sizeof(float) * 2 * 5 * c->N) == hipSuccess;
-  if (ok && (params->precision == HYQ_PREC_MIXED_NEWTON || params->precision == HYQ_PREC_MIXED_LEARNED))
-    ok = hipMalloc(&c->list1, sizeof(int) * c->N) == hipSuccess &&
-         hipMalloc(&c->list2, sizeof(int) * c->N) == hipSuccess;
+  if (ok && (params->precision == HYQ_PREC_MIXED_NEWTON || params->precision == HYQ_PREC_MIXED_LEARNED)) {
+    FastLists& L = c->L;
+    ok = hipMalloc(&L.cells, sizeof(int) * c->N) == hipSuccess &&
+         hipMalloc(&L.slot_cell, sizeof(int) * c->N) == hipSuccess &&
+         hipMalloc(&L.slot_mask, sizeof(unsigned) * c->N) == hipSuccess &&
+         hipMalloc(&L.slot_bad, sizeof(unsigned) * c->N) == hipSuccess &&
+         hipMalloc(&L.slot_fl, sizeof(uint32_t) * c->N) == hipSuccess &&
+         hipMalloc(&L.slot_v, sizeof(double) * 16 * c->N) == hipSuccess &&
+         hipMalloc(&L.slot_A, sizeof(double) * 35 * c->N) == hipSuccess;
+  }
   if (!ok) {
     hyq_destroy(c);
     return fail(HYQ_ENOMEM, "device allocation failed");
@@ -813,8 +974,13 @@ int hyq_destroy(hyq_ctx* c) {
   hipFree(c->diag);
   hipFree(c->X);
   hipFree(c->P);
-  hipFree(c->list1);
-  hipFree(c->list2);
+  hipFree(c->L.cells);
+  hipFree(c->L.slot_cell);
+  hipFree(c->L.slot_bad);
+  hipFree(c->L.slot_fl);
+  hipFree(c->L.slot_v);
+  hipFree(c->L.slot_mask);
+  hipFree(c->L.slot_A);
   for (int i = 0; i < kNev * kChunk; ++i)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -869,13 +1035,16 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
       hipLaunchKernelGGL(k_pass1a<false>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
                          (short*)nullptr, c->flags, c->sc, c->N, p.flag2D, p.Ma, (const float*)nullptr);
     }
-    hipLaunchKernelGGL(k_pass1_fast, dim3(g), dim3(kBlock), 0, c->stream, c->Mr, (float*)c->F,
-                       (float*)c->vp, c->flags, learned ? (const float*)c->P : (const float*)nullptr,
-                       c->list1, c->sc, c->N, p.flag2D, p.Ma, p.positivity);
-    hipLaunchKernelGGL(k_pass1_full, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->Mr,
-                       (float*)c->F, (float*)c->vp, c->flags,
-                       learned ? (const float*)c->P : (const float*)nullptr, (const int*)c->list1,
-                       c->sc, c->N, p.flag2D, p.Ma, p.positivity);
+    const float* P = learned ? (const float*)c->P : (const float*)nullptr;
+    hipLaunchKernelGGL(k_pass1_fast<float>, dim3(g), dim3(kBlock), 0, c->stream, c->Mr, (float*)c->F,
+                       (float*)c->vp, c->flags, P, c->L, c->sc, c->N, p.flag2D, p.Ma, p.positivity);
+    hipLaunchKernelGGL(k_slot_solve<true>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
+                       (const unsigned*)&c->sc->nslot1, &c->sc->nitem1, P, c->sc, c->N);
+    hipLaunchKernelGGL(k_pass1_fixup<float>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
+                       (float*)c->vp, c->flags, c->sc, c->N);
+    hipLaunchKernelGGL(k_pass1_full<float>, dim3(kListBlocks), dim3(kBlock), 0, c->stream,
+                       (const double*)c->M, c->Mr, (float*)c->F, (float*)c->vp, c->flags, P,
+                       (const int*)c->L.cells, c->sc, c->N, p.flag2D, p.Ma, p.positivity);
   }
   if (ev) hipEventRecord(ev[1], c->stream);
   hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, c->stream, c->sc, p.CFL, hmin, p.dtmax, p.Kn);
@@ -898,11 +1067,16 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
                        c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
   } else {
     hipLaunchKernelGGL(k_pass2_check, dim3(g), dim3(kBlock), 0, c->stream, (const double*)c->M,
-                       c->Mr, c->list2, c->sc, c->N, p.flag2D, p.Ma);
+                       c->Mr, c->L, c->sc, c->N, p.flag2D, p.Ma);
     hipLaunchKernelGGL(k_pass2_relax, dim3(g), dim3(kBlock), 0, c->stream, c->M,
                        (const double*)c->Mr, c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
+    hipLaunchKernelGGL(k_slot_solve<false>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
+                       (const unsigned*)&c->sc->nslot2, &c->sc->nitem2, (const float*)nullptr, c->sc,
+                       c->N);
+    hipLaunchKernelGGL(k_pass2_fixup, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->M, c->L,
+                       c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
     hipLaunchKernelGGL(k_pass2_full, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->M,
-                       (const int*)c->list2, c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
+                       (const int*)c->L.cells, c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
   }
   if (ev) hipEventRecord(ev[4], c->stream);
   const hipError_t e = hipGetLastError();
@@ -1102,8 +1276,19 @@ int hyq_deferred_cells(hyq_ctx* c, int64_t* pass1, int64_t* pass2) {
   DevScalars s;
   HIPCHK(hipMemcpyAsync(&s, c->sc, sizeof(s), hipMemcpyDeviceToHost, c->stream), "scalars back");
   HIPCHK(hipStreamSynchronize(c->stream), "sync");
-  if (pass1) *pass1 = (int64_t)s.nhard1;
-  if (pass2) *pass2 = (int64_t)s.nhard2;
+  if (pass1) *pass1 = (int64_t)s.nhard1 + (int64_t)s.nslot1 - (int64_t)s.ncplx1;
+  if (pass2) *pass2 = (int64_t)s.nhard2 + (int64_t)s.nslot2 - (int64_t)s.ncplx2;
+  return HYQ_OK;
+}
+
+int hyq_fast_path_counts(hyq_ctx* c, int64_t* counts) {
+  if (!c || !counts) return fail(HYQ_EINVAL, "null argument");
+  DevScalars s;
+  HIPCHK(hipMemcpyAsync(&s, c->sc, sizeof(s), hipMemcpyDeviceToHost, c->stream), "scalars back");
+  HIPCHK(hipStreamSynchronize(c->stream), "sync");
+  const unsigned v[8] = {s.nslot1, s.nitem1, s.ncplx1, s.nhard1 - s.ncplx1,
+                         s.nslot2, s.nitem2, s.ncplx2, s.nhard2 - s.ncplx2};
+  for (int k = 0; k < 8; ++k) counts[k] = (int64_t)v[k];
   return HYQ_OK;
 }
 
@@ -1125,20 +1310,92 @@ int hyq_closure_batch(const double* M, int64_t n, double Ma, int32_t flag2D, dou
   return HYQ_OK;
 }
 
+static int batch_args(const double* M, int64_t n, int32_t precision, const double* Mr,
+                      const double* Fx, const double* Fy, const double* Fz, const double* vp,
+                      const uint32_t* flags) {
+  if (!M || !Mr || !Fx || !Fy || !Fz || !vp || n < 0) return fail(HYQ_EINVAL, "bad argument");
+  if (precision != HYQ_PREC_FP64 && precision != HYQ_PREC_MIXED_NEWTON)
+    return fail(HYQ_EINVAL, "batch wave speeds: precision must be FP64 or MIXED_NEWTON");
+  if (precision == HYQ_PREC_MIXED_NEWTON && !flags)
+    return fail(HYQ_EINVAL, "the fast batch needs the flags output");
+  if (n > 0x7FFFFFFFLL / 5) return fail(HYQ_EINVAL, "batch too large");
+  // the three flux outputs must be one [105][n] block for the pass-1 kernels
+  if (n > 0 && (Fy != Fx + (size_t)35 * n || Fz != Fx + (size_t)70 * n))
+    return fail(HYQ_EINVAL, "Fx, Fy, Fz must be consecutive [35][n] blocks");
+  return HYQ_OK;
+}
+
+// The fast batch's scratch for n cells, in one block: the step scalars, then the full-path
+// list, the slots and the plane items of FastLists.
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+static size_t fast_batch_bytes(long long n) {
+  const size_t un = (size_t)n;
+  return align256(sizeof(DevScalars)) + 5 * align256(4 * un) + align256(16 * 8 * un) +
+         align256(35 * 8 * un);
+}
+static void fast_batch_carve(char* base, long long n, DevScalars** sc, FastLists* L) {
+  const size_t un = (size_t)n;
+  char* p = base;
+  *sc = reinterpret_cast<DevScalars*>(p); p += align256(sizeof(DevScalars));
+  L->cells = reinterpret_cast<int*>(p); p += align256(4 * un);
+  L->slot_cell = reinterpret_cast<int*>(p); p += align256(4 * un);
+  L->slot_mask = reinterpret_cast<unsigned*>(p); p += align256(4 * un);
+  L->slot_bad = reinterpret_cast<unsigned*>(p); p += align256(4 * un);
+  L->slot_fl = reinterpret_cast<uint32_t*>(p); p += align256(4 * un);
+  L->slot_v = reinterpret_cast<double*>(p); p += align256(16 * 8 * un);
+  L->slot_A = reinterpret_cast<double*>(p);
+}
+// Scalars of a batch: active, everything else zero.  A kernel rather than a memset, so that
+// the launch sequence can be captured in a graph.
+__global__ void k_batch_begin(DevScalars* sc) {
+  sc->t = sc->dt = sc->tmax = 0.0;
+  sc->active = 1.0;
+  sc->vmax_bits = 0ull;
+  sc->nn = sc->nnmax = 0;
+  sc->err = 0ull;
+  sc->nhard1 = sc->nhard2 = sc->nslot1 = sc->nslot2 = 0u;
+  sc->nitem1 = sc->nitem2 = sc->ncplx1 = sc->ncplx2 = 0u;
+}
+// Pass 1 of the fast step on a batch (FP64 outputs) with its scratch at `scr`.
+static void enqueue_fast_batch(const double* M, long long n, double Ma, int flag2D, int positivity,
+                               double* Mr, double* F, double* vp, uint32_t* flags, char* scr) {
+  DevScalars* sc;
+  FastLists L;
+  fast_batch_carve(scr, n, &sc, &L);
+  const unsigned g = grid_for(n);
+  hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(1), 0, 0, sc);
+  hipLaunchKernelGGL(k_pass1a<false>, dim3(g), dim3(kBlock), 0, 0, M, Mr, (short*)nullptr, flags,
+                     sc, n, flag2D, Ma, (const float*)nullptr);
+  hipLaunchKernelGGL(k_pass1_fast<double>, dim3(g), dim3(kBlock), 0, 0, Mr, F, vp, flags,
+                     (const float*)nullptr, L, sc, n, flag2D, Ma, positivity);
+  hipLaunchKernelGGL(k_slot_solve<true>, dim3(kListBlocks), dim3(kBlock), 0, 0, L,
+                     (const unsigned*)&sc->nslot1, &sc->nitem1, (const float*)nullptr, sc, n);
+  hipLaunchKernelGGL(k_pass1_fixup<double>, dim3(kListBlocks), dim3(kBlock), 0, 0, L, vp, flags, sc, n);
+  hipLaunchKernelGGL(k_pass1_full<double>, dim3(kListBlocks), dim3(kBlock), 0, 0, M, Mr, F, vp, flags,
+                     (const float*)nullptr, (const int*)L.cells, sc, n, flag2D, Ma, positivity);
+}
+
 int hyq_wavespeed_batch(const double* M, int64_t n, double Ma, int32_t flag2D,
                         int32_t positivity, int32_t precision, double* Mr, double* Fx,
                         double* Fy, double* Fz, double* vp, uint32_t* flags) {
-  if (!M || !Mr || !Fx || !Fy || !Fz || !vp || n < 0) return fail(HYQ_EINVAL, "bad argument");
-  if (precision != HYQ_PREC_FP64) return fail(HYQ_EINVAL, "batch wave speeds are FP64");
+  const int rc = batch_args(M, n, precision, Mr, Fx, Fy, Fz, vp, flags);
+  if (rc) return rc;
   if (n == 0) return HYQ_OK;
-  // the three flux outputs must be one [105][n] block for the pass-1 kernel
-  if (Fy != Fx + (size_t)35 * n || Fz != Fx + (size_t)70 * n)
-    return fail(HYQ_EINVAL, "Fx, Fy, Fz must be consecutive [35][n] blocks");
-  hipLaunchKernelGGL((k_pass1<double, 0>), dim3(grid_for(n)), dim3(kBlock), 0, 0, M, Mr, Fx, vp,
-                     flags,
-                     (DevScalars*)nullptr, (long long)n, flag2D, Ma, positivity);
-  HIPCHK(hipGetLastError(), "pass1 launch");
-  HIPCHK(hipDeviceSynchronize(), "pass1 sync");
+  if (precision == HYQ_PREC_FP64) {
+    hipLaunchKernelGGL((k_pass1<double, 0>), dim3(grid_for(n)), dim3(kBlock), 0, 0, M, Mr, Fx, vp,
+                       flags, (DevScalars*)nullptr, (long long)n, flag2D, Ma, positivity);
+    HIPCHK(hipGetLastError(), "pass1 launch");
+    HIPCHK(hipDeviceSynchronize(), "pass1 sync");
+    return HYQ_OK;
+  }
+  void* scr = nullptr;
+  HIPCHK(hipMalloc(&scr, fast_batch_bytes(n)), "fast batch scratch");
+  enqueue_fast_batch(M, n, Ma, flag2D, positivity, Mr, Fx, vp, flags, static_cast<char*>(scr));
+  hipError_t e = hipGetLastError();
+  const hipError_t e2 = hipDeviceSynchronize();
+  if (e == hipSuccess) e = e2;
+  hipFree(scr);
+  if (e != hipSuccess) return fail(HYQ_EHIP, "fast batch", e);
   return HYQ_OK;
 }
 

@@ -52,11 +52,16 @@ HD bool plane_near_gaussian(const Plane& p) { return p.ok && plane_gauss_dev(p) 
 // warm starts (wmin/wmax in standardized units, NaN = none), falling back to the full solve
 // when hyperbolicity is not proven or the iteration does not converge.
 // One plane by the fast path: Gaussian seed (when no warm start is given and the plane is
-// near-Gaussian), Sturm test, Laguerre extremes.  Returns false when the plane is not
-// proven hyperbolic (or not finite): the caller then needs the full solve.
-HD bool plane_fast(const Plane& p, float wmin, float wmax, bool need_vals, uint32_t& fl,
-                   double& vmin, double& vmax) {
-  if (!p.ok) return false;
+// near-Gaussian), Sturm test (with the 2x2 block test of a nearly double pair), Laguerre
+// extremes.  Returns kPlaneProven with vmin/vmax set, kPlaneSolve when the plane is not
+// proven hyperbolic (the full solve of this plane decides), or kPlaneFull when the cell
+// must take the full path: the plane or its Jacobian is not finite (the full path carries
+// the reference's NaN semantics), or a complex pair is proven (the eigenvalue correction).
+constexpr int kPlaneProven = 0, kPlaneSolve = 1, kPlaneFull = 2;
+template <bool PAIR = true>
+HD int plane_fast_status(const Plane& p, float wmin, float wmax, bool need_vals, uint32_t& fl,
+                         double& vmin, double& vmax) {
+  if (!p.ok) return kPlaneFull;
   if (wmin != wmin && plane_near_gaussian(p)) {
     // a Gaussian plane's standardized extreme eigenvalues are -sqrt(3), +sqrt(3)
     wmin = -1.7320508075688772f;
@@ -67,15 +72,20 @@ HD bool plane_fast(const Plane& p, float wmin, float wmax, bool need_vals, uint3
   bool fin = true;
 #pragma unroll
   for (int t = 0; t < 36; ++t) fin = fin && (fabs(J[t]) < 1e300);
-  if (!fin) return false;
-  const FastEig fe = fast_eigs_std(J, (double)wmin, (double)wmax, need_vals);
+  if (!fin) return kPlaneFull;
+  const FastEig fe = fast_eigs_std<PAIR>(J, (double)wmin, (double)wmax, need_vals, p.sig);
   // root-polishing iterations, summed over the cell's plane solves, in flag bits 16..27
   const uint32_t it = (fl >> 16) + (uint32_t)fe.iters;
   fl = (fl & 0xFFFFu) | ((it < 0xFFFu ? it : 0xFFFu) << 16);
-  if (!fe.real) return false;
+  if (fe.cplx) return kPlaneFull;
+  if (!fe.real) return kPlaneSolve;
   vmin = need_vals ? p.a + p.sig * fe.lmin : 0.0;
   vmax = need_vals ? p.a + p.sig * fe.lmax : 0.0;
-  return true;
+  return kPlaneProven;
+}
+HD bool plane_fast(const Plane& p, float wmin, float wmax, bool need_vals, uint32_t& fl,
+                   double& vmin, double& vmax) {
+  return plane_fast_status(p, wmin, wmax, need_vals, fl, vmin, vmax) == kPlaneProven;
 }
 
 template <int EM>
@@ -310,40 +320,91 @@ HD void pass2_cell(const double* M, int flag2D, double Ma, bool positivity, doub
 // on a compacted list, so the rare, register-hungry branches neither widen the common
 // kernel nor hold up its waves.
 
-// x/y/z (min, max) of the 6x6 Jacobians of projected state A when the cell takes none of
-// the rare branches (the planes UV, UW, VU, VW, WU; the z direction's second plane is the
-// y direction's VW, as eigenvalues6z_hyperbolic_3D.jl:117-120 maps it).  Returns false
-// when the cell needs the full path.
-HD bool fast_cell_speeds(const double* A, bool positivity, bool need_vals, const Warm& w,
-                         double v6[6], uint32_t& fl) {
-  // z path preconditions (eigenvalues6z_hyperbolic_3D.jl:27-104): no clamp, no re-projection
-  if (positivity && !(A[0] > 0.0)) return false;
-  if (positivity && A[0] > 1e-15) {
-    const double m0 = A[0];
-    const double vx = A[2] / m0 - (A[1] / m0) * (A[1] / m0);
-    const double vy = A[9] / m0 - (A[5] / m0) * (A[5] / m0);
-    const double vz = A[19] / m0 - (A[15] / m0) * (A[15] / m0);
-    if (vx < 0.0 || vy < 0.0 || vz < 0.0) return false;
+// The five distinct plane Jacobians of a cell, in this order (the z direction's second plane
+// is the y direction's VW, as eigenvalues6z_hyperbolic_3D.jl:117-120 maps it).
+constexpr int kUV = 0, kUW = 1, kVU = 2, kVW = 3, kWU = 4;
+HD Plane cell_plane(const double* c, double um, double vm, double wm, int q) {
+  switch (q) {
+    case kUV: return plane<0, 1>(c, um, vm, wm);
+    case kUW: return plane<0, 2>(c, um, vm, wm);
+    case kVU: return plane<1, 0>(c, um, vm, wm);
+    case kVW: return plane<1, 2>(c, um, vm, wm);
+    default: return plane<2, 0>(c, um, vm, wm);
+  }
+}
+
+// The fast step's view of a cell's planes in pass 1 (need_vals, five planes) or pass 2 (the
+// x/y checks, four planes).  Returns kPlaneFull when the cell must take the full path
+// (z density clamp or re-projection, eigenvalues6z_hyperbolic_3D.jl:27-104, or a non-finite
+// plane); otherwise the planes the fast solve proved have their (vmin, vmax) in v[2q],
+// v[2q + 1], and bit q of `pend` marks each plane that needs its own full solve.
+template <bool PAIR = true, bool VALS = true>
+HD int fast_cell_planes(const double* A, bool positivity, const Warm& w, double v[10],
+                        unsigned& pend, uint32_t& fl) {
+  pend = 0;
+  if (VALS) {
+    if (positivity && !(A[0] > 0.0)) return kPlaneFull;
+    if (positivity && A[0] > 1e-15) {
+      const double m0 = A[0];
+      const double vx = A[2] / m0 - (A[1] / m0) * (A[1] / m0);
+      const double vy = A[9] / m0 - (A[5] / m0) * (A[5] / m0);
+      const double vz = A[19] / m0 - (A[15] / m0) * (A[15] / m0);
+      if (vx < 0.0 || vy < 0.0 || vz < 0.0) return kPlaneFull;
+    }
   }
   double c[35], M000, um, vm, wm;
   raw_to_central4(A, c, M000, um, vm, wm);
-  double uv0, uv1, uw0, uw1, vu0, vu1, vw0, vw1, wu0, wu1;
-  if (!plane_fast(plane<0, 1>(c, um, vm, wm), w.w[0], w.w[1], need_vals, fl, uv0, uv1)) return false;
-  if (!plane_fast(plane<0, 2>(c, um, vm, wm), w.w[2], w.w[3], need_vals, fl, uw0, uw1)) return false;
-  if (!plane_fast(plane<1, 0>(c, um, vm, wm), w.w[4], w.w[5], need_vals, fl, vu0, vu1)) return false;
-  if (!plane_fast(plane<1, 2>(c, um, vm, wm), w.w[6], w.w[7], need_vals, fl, vw0, vw1)) return false;
-  if (need_vals &&
-      !plane_fast(plane<2, 0>(c, um, vm, wm), w.w[8], w.w[9], need_vals, fl, wu0, wu1))
-    return false;
-  if (need_vals) {
-    v6[0] = jmin(uv0, uw0);
-    v6[1] = jmax(uv1, uw1);
-    v6[2] = jmin(vu0, vw0);
-    v6[3] = jmax(vu1, vw1);
-    v6[4] = jmin(wu0, vw0);
-    v6[5] = jmax(wu1, vw1);
+  const float nan = NAN;
+  bool full = false;
+#pragma unroll
+  for (int q = 0; q < (VALS ? 5 : 4); ++q) {
+    v[2 * q] = v[2 * q + 1] = 0.0;
+    if (full) continue;
+    const int st = plane_fast_status<PAIR>(cell_plane(c, um, vm, wm, q), VALS ? w.w[2 * q] : nan,
+                                           VALS ? w.w[2 * q + 1] : nan, VALS, fl, v[2 * q],
+                                           v[2 * q + 1]);
+    if (st == kPlaneFull) full = true;
+    if (st == kPlaneSolve) pend |= 1u << q;
   }
+  if (full) return kPlaneFull;
+  return pend ? kPlaneSolve : kPlaneProven;
+}
+
+// x/y/z (min, max) from the five planes' extremes, combined as pass 1 of the full path does.
+HD void combine_planes(const double v[10], double v6[6]) {
+  v6[0] = jmin(v[2 * kUV], v[2 * kUW]);
+  v6[1] = jmax(v[2 * kUV + 1], v[2 * kUW + 1]);
+  v6[2] = jmin(v[2 * kVU], v[2 * kVW]);
+  v6[3] = jmax(v[2 * kVU + 1], v[2 * kVW + 1]);
+  v6[4] = jmin(v[2 * kWU], v[2 * kVW]);
+  v6[5] = jmax(v[2 * kWU + 1], v[2 * kVW + 1]);
+}
+
+// x/y/z (min, max) of the 6x6 Jacobians of projected state A when the cell takes none of
+// the rare branches.  Returns false when the cell needs the full path or a plane solve.
+HD bool fast_cell_speeds(const double* A, bool positivity, const Warm& w, double v6[6],
+                         uint32_t& fl) {
+  double v[10];
+  unsigned pend;
+  if (fast_cell_planes<true, true>(A, positivity, w, v, pend, fl) != kPlaneProven) return false;
+  combine_planes(v, v6);
   return true;
+}
+
+// The 1D closure speeds of the x, y and z slices of Mr (simulation_runner.jl:370-372):
+// (min, max) per direction.
+HD void closure1d_all(const double* Mr, double s[6]) {
+  closure1d_speeds(Mr[0], Mr[1], Mr[2], Mr[3], Mr[4], s[0], s[1]);
+  closure1d_speeds(Mr[0], Mr[5], Mr[9], Mr[12], Mr[14], s[2], s[3]);
+  closure1d_speeds(Mr[0], Mr[15], Mr[19], Mr[22], Mr[24], s[4], s[5]);
+}
+// vp from the 1D speeds s and the 6x6 speeds v6 (simulation_runner.jl:374-379).
+HD void combine_vp(const double s[6], const double v6[6], double vp[6]) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    vp[2 * d] = jmin(s[2 * d], v6[2 * d]);
+    vp[2 * d + 1] = jmax(s[2 * d + 1], v6[2 * d + 1]);
+  }
 }
 
 // The rest of pass 1 for a cell that took none of the rare branches: closure and fluxes of
@@ -352,31 +413,54 @@ template <typename FT>
 HD void pass1_finish(const double* A, int flag2D, double Ma, double* Mr, FT* Fx, FT* Fy, FT* Fz,
                      size_t st, const double v6[6], double vp[6], uint32_t& fl) {
   flux_closure35<true, FT>(A, flag2D, Ma, Mr, Fx, Fy, Fz, st, fl);
-  double a, b;
-  closure1d_speeds(Mr[0], Mr[1], Mr[2], Mr[3], Mr[4], a, b);
-  vp[0] = jmin(a, v6[0]);
-  vp[1] = jmax(b, v6[1]);
-  closure1d_speeds(Mr[0], Mr[5], Mr[9], Mr[12], Mr[14], a, b);
-  vp[2] = jmin(a, v6[2]);
-  vp[3] = jmax(b, v6[3]);
-  closure1d_speeds(Mr[0], Mr[15], Mr[19], Mr[22], Mr[24], a, b);
-  vp[4] = jmin(a, v6[4]);
-  vp[5] = jmax(b, v6[5]);
+  double s[6];
+  closure1d_all(Mr, s);
+  combine_vp(s, v6, vp);
 }
 
 // Pass 2 of a cell when its x/y planes need no correction: closure, hyperbolicity check of
 // UV, UW, VU, VW, closure, BGK.  Returns false (Mo untouched) when the cell needs the full
 // pass 2.
 // First half: projection of M to A and the hyperbolicity checks of UV, UW, VU, VW.
-HD bool pass2_check(const double* M, int flag2D, double Ma, double* A, uint32_t& fl) {
+// Returns kPlaneProven, kPlaneSolve (bit q of pend: plane q needs its own full solve) or
+// kPlaneFull (the full pass 2).
+template <bool PAIR = true>
+HD int pass2_check_planes(const double* M, int flag2D, double Ma, double* A, unsigned& pend,
+                          uint32_t& fl) {
   flux_closure35<false, double>(M, flag2D, Ma, A, nullptr, nullptr, nullptr, 0, fl);
-  double c[35], M000, um, vm, wm, d0, d1;
+  double v[10];
+  return fast_cell_planes<PAIR, false>(A, false, no_warm(), v, pend, fl);
+}
+// The full decision of the planes in pend (fast solve with the pair test, else QR) of a
+// cell whose other planes the small fast kernel already proved: v[2q], v[2q+1] set for
+// each such plane; returns false when one of them has a complex pair (or is not finite),
+// i.e. the cell needs the eigenvalue correction's full path.
+HD bool solve_pending_planes(const double* A, unsigned pend, bool need_vals, const Warm& w,
+                             double v[10], uint32_t& fl) {
+  double c[35], M000, um, vm, wm;
   raw_to_central4(A, c, M000, um, vm, wm);
   const float nan = NAN;
-  if (!plane_fast(plane<0, 1>(c, um, vm, wm), nan, nan, false, fl, d0, d1)) return false;
-  if (!plane_fast(plane<0, 2>(c, um, vm, wm), nan, nan, false, fl, d0, d1)) return false;
-  if (!plane_fast(plane<1, 0>(c, um, vm, wm), nan, nan, false, fl, d0, d1)) return false;
-  return plane_fast(plane<1, 2>(c, um, vm, wm), nan, nan, false, fl, d0, d1);
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    if (!(pend & (1u << q))) continue;
+    const Plane p = cell_plane(c, um, vm, wm, q);
+    const int st = plane_fast_status<true>(p, need_vals ? w.w[2 * q] : nan,
+                                           need_vals ? w.w[2 * q + 1] : nan, need_vals, fl,
+                                           v[2 * q], v[2 * q + 1]);
+    if (st == kPlaneFull) return false;
+    if (st == kPlaneSolve) {
+      fl |= F_FALLBACK;
+      const EigRes r = plane_eigs(p, fl);
+      if (r.cplx || !r.ok) return false;
+      v[2 * q] = r.vmin;
+      v[2 * q + 1] = r.vmax;
+    }
+  }
+  return true;
+}
+HD bool pass2_check(const double* M, int flag2D, double Ma, double* A, uint32_t& fl) {
+  unsigned pend;
+  return pass2_check_planes<true>(M, flag2D, Ma, A, pend, fl) == kPlaneProven;
 }
 // Second half: projection of A and BGK.
 template <bool MIXED>

@@ -77,14 +77,13 @@ HD UpdDir<T> upd_dir(const T* vl, const T* vh, int s, int n, long long st) {
   return d;
 }
 
-// Updated value of one moment along one direction (FP64 fluxes and arithmetic).
-// m, f: the moment's state and flux planes at this cell's position.
+// Updated value of one moment along one direction (FP64 fluxes and arithmetic), general
+// mode: m0, m1, m2 and f0, f1, f2 are the moment's state and flux at cells j-1, j, j+1
+// (f0 is not read when d.first).
 template <typename FT>
-HD double upd_value(const UpdDir<double>& d, const double* m, const FT* f, double dtdx) {
-  if (d.mode == 0) return m[0];
-  if (d.mode == 1) return m[d.o1];
-  const double m0 = m[d.o0], m1 = m[d.o1], m2 = m[d.o2];
-  const double f1 = (double)f[d.o1], f2 = (double)f[d.o2];
+HD double upd_core(const UpdDir<double>& d, double m0, double m1, double m2, FT f0_, FT f1_, FT f2_,
+                   double dtdx) {
+  const double f1 = (double)f1_, f2 = (double)f2_;
   const double wsB = d.B.sep ? (d.B.ll * m1 - d.B.lr * m2) * d.B.inv - (f1 - f2) * d.B.inv : 0.0;
   const double fB = 0.5 * (f1 + (d.last ? f1 : f2)) - 0.5 * (d.B.all * (wsB - m1) - d.B.alr * (wsB - m2));
   double fA;
@@ -92,7 +91,7 @@ HD double upd_value(const UpdDir<double>& d, const double* m, const FT* f, doubl
     // face 0: zero wave speeds, W* of face 1, F[1] := F[2]
     fA = 0.5 * (f1 + f1) - 0.5 * (0.0 * (wsB - m0) - 0.0 * (wsB - m1));
   } else {
-    const double f0 = (double)f[d.o0];
+    const double f0 = (double)f0_;
     const double wsA = d.A.sep ? (d.A.ll * m0 - d.A.lr * m1) * d.A.inv - (f0 - f1) * d.A.inv : 0.0;
     fA = 0.5 * (f0 + f1) - 0.5 * (d.A.all * (wsA - m0) - d.A.alr * (wsA - m1));
   }
@@ -102,11 +101,8 @@ HD double upd_value(const UpdDir<double>& d, const double* m, const FT* f, doubl
 // Mixed precision: FP32 fluxes, wave speeds and HLL arithmetic in difference form (W* - M_q
 // and W* - M_{q+1} from M_q - M_{q+1}), so FP32 rounding applies to differences and to the
 // increment, never to the FP64 state.
-HD double upd_value(const UpdDir<float>& d, const double* m, const float* f, double dtdx) {
-  if (d.mode == 0) return m[0];
-  if (d.mode == 1) return m[d.o1];
-  const double m0 = m[d.o0], m1 = m[d.o1], m2 = m[d.o2];
-  const float f1 = f[d.o1], f2 = f[d.o2];
+HD double upd_core(const UpdDir<float>& d, double m0, double m1, double m2, float f0, float f1,
+                   float f2, double dtdx) {
   const float dmB = (float)(m1 - m2);
   const float w0B = d.B.sep ? (d.B.lr * dmB - (f1 - f2)) * d.B.inv : -(float)m1;
   const float w1B = w0B + dmB;
@@ -115,7 +111,6 @@ HD double upd_value(const UpdDir<float>& d, const double* m, const float* f, dou
   if (d.first) {
     fA = f1;  // zero wave speeds on face 0: the flux of cell 1
   } else {
-    const float f0 = f[d.o0];
     const float dmA = (float)(m0 - m1);
     const float w0A = d.A.sep ? (d.A.lr * dmA - (f0 - f1)) * d.A.inv : -(float)m0;
     const float w1A = w0A + dmA;
@@ -123,6 +118,15 @@ HD double upd_value(const UpdDir<float>& d, const double* m, const float* f, dou
   }
   const float inc = -(float)dtdx * (fB - fA);
   return m1 + (double)inc;
+}
+
+// Updated value of one moment along one direction.  m, f: the moment's state and flux
+// planes at this cell's position.
+template <typename T, typename FT>
+HD double upd_value(const UpdDir<T>& d, const double* m, const FT* f, double dtdx) {
+  if (d.mode == 0) return m[0];
+  if (d.mode == 1) return m[d.o1];
+  return upd_core(d, m[d.o0], m[d.o1], m[d.o2], d.first ? f[d.o1] : f[d.o0], f[d.o1], f[d.o2], dtdx);
 }
 
 // Combined x/y/z update of cell c (i, j, k) for all 35 moments: Mx + My + Mz - 2M

@@ -57,6 +57,7 @@ SIGNATURES = {
     "hyq_kernel_times": ([VP, D, D, D, ctypes.POINTER(I64)], ctypes.c_int),
     "hyq_reset_kernel_times": ([VP], ctypes.c_int),
     "hyq_deferred_cells": ([VP, ctypes.POINTER(I64), ctypes.POINTER(I64)], ctypes.c_int),
+    "hyq_fast_path_counts": ([VP, ctypes.POINTER(I64)], ctypes.c_int),
     "hyq_snapshot_fields": ([VP, D, D], ctypes.c_int),
     "hyq_symmetry_maxdiff": ([VP, D, D], ctypes.c_int),
     "hyq_vmax_breakdown": ([VP, D, ctypes.POINTER(I64)], ctypes.c_int),

@@ -186,6 +186,16 @@ class Solver:
         _lib.check(_lib.lib().hyq_deferred_cells(self.h, ctypes.byref(a), ctypes.byref(b)), "deferred cells")
         return a.value, b.value
 
+    def fast_path_counts(self):
+        """Why cells of the last step left the fast kernels (fast modes): a dict with, per
+        pass, the cells with plane solves, the plane solves, the cells a plane solve found
+        complex (eigenvalue correction, full path) and the cells on the full path from the
+        start (z density clamp or re-projection, non-finite plane)."""
+        a = (_lib.I64 * 8)()
+        _lib.check(_lib.lib().hyq_fast_path_counts(self.h, a), "fast path counts")
+        keys = ("plane_solve_cells", "plane_solves", "complex_cells", "full_path_cells")
+        return {"pass1": dict(zip(keys, a[:4])), "pass2": dict(zip(keys, a[4:]))}
+
     def snapshot_fields(self):
         """(C, S): the central and standardized moment fields of the current state
         (compute_central_field / compute_standardized_field), computed on the device, in the

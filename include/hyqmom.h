@@ -119,10 +119,18 @@ int hyq_kernel_times(hyq_ctx* ctx, double* ms_pass1, double* ms_update, double* 
                      int64_t* nsteps);
 int hyq_reset_kernel_times(hyq_ctx* ctx);
 
-/* Cells of the last step that the fast modes (mixed_newton, mixed_learned) handed to the
- * full pass 1 and the full pass 2 (QR fallback, eigenvalue corrections, z-path clamps); 0 in
- * the other modes.  No reference counterpart (the reference has a single path). */
+/* Cells of the last step that the fast modes (mixed_newton, mixed_learned) did not finish in
+ * their fast kernel, in pass 1 and pass 2: cells with a full plane solve plus cells on the
+ * full path (eigenvalue corrections, z-path clamps, non-finite planes); 0 in the other modes.
+ * No reference counterpart (the reference has a single path). */
 int hyq_deferred_cells(hyq_ctx* ctx, int64_t* pass1, int64_t* pass2);
+
+/* Why cells of the last step left the fast kernels (fast modes; zeros otherwise), counts[8]:
+ * pass 1 [0] cells with planes the fast solve could not decide, [1] such plane solves (QR),
+ * [2] cells a plane solve found complex (eigenvalue correction: full path), [3] cells on the
+ * full path from the start (z density clamp or re-projection, non-finite plane); [4..7] the
+ * same for pass 2.  No reference counterpart. */
+int hyq_fast_path_counts(hyq_ctx* ctx, int64_t* counts);
 
 /* The snapshot fields of the current state on the device: the central moments C
  * (compute_central_field) and the standardized moments S (compute_standardized_field,
@@ -155,7 +163,12 @@ int hyq_closure_batch(const double* M, int64_t n, double Ma, int32_t flag2D, dou
 
 /* The pass-1 cell pipeline of simulation_runner.jl:352-380 for n independent cells:
  * closure -> x/y/z wave speeds (eigenvalues6_hyperbolic_3D.jl, eigenvalues6z_hyperbolic_3D.jl)
- * -> closure -> 1D closures (closure_and_eigenvalues.jl).  vp is [6][n]. */
+ * -> closure -> 1D closures (closure_and_eigenvalues.jl).  vp is [6][n]; Fx, Fy, Fz must be
+ * consecutive [35][n] blocks.  precision HYQ_PREC_FP64: QR eigen solves;
+ * HYQ_PREC_MIXED_NEWTON: the fast step's pass-1 kernels (Sturm test, 2x2 block test of a
+ * nearly double pair, Laguerre extremes, QR where those do not decide, full path for the
+ * eigenvalue corrections), with scratch the call allocates; needs flags.  Both give FP64
+ * outputs. */
 int hyq_wavespeed_batch(const double* M, int64_t n, double Ma, int32_t flag2D,
                         int32_t positivity, int32_t precision, double* Mr, double* Fx,
                         double* Fy, double* Fz, double* vp, uint32_t* flags);
