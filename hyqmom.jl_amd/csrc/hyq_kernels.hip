@@ -15,6 +15,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1396,6 +1397,52 @@ int hyq_wavespeed_batch(const double* M, int64_t n, double Ma, int32_t flag2D,
   if (e == hipSuccess) e = e2;
   hipFree(scr);
   if (e != hipSuccess) return fail(HYQ_EHIP, "fast batch", e);
+  return HYQ_OK;
+}
+
+// ---- the same without allocation ---------------------------------------------------------
+// A fixed device array holds the fast batch's scratch, for callers that cannot allocate
+// (graph capture, serving loops).  Each device holds its own copy until the process ends;
+// every call on a device shares it, so calls that overlap on one device race.
+constexpr long long kNoAllocMaxCells = 1LL << 16;
+static __device__ __align__(256) char scratch[(5 * 4 + 16 * 8 + 35 * 8) * kNoAllocMaxCells + 8 * 256];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[256];  // per device: the array's device address, once looked up
+
+static int scratch_address(char** out) {
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= 256) return fail(HYQ_ENODEV, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = static_cast<char*>(g_scratch_ptr[dev]);
+  return HYQ_OK;
+}
+
+int hyq_wavespeed_batch_noalloc(const double* M, int64_t n, double Ma, int32_t flag2D,
+                                int32_t positivity, int32_t precision, double* Mr, double* Fx,
+                                double* Fy, double* Fz, double* vp, uint32_t* flags) {
+  int rc = batch_args(M, n, precision, Mr, Fx, Fy, Fz, vp, flags);
+  if (rc) return rc;
+  if (n > kNoAllocMaxCells)
+    return fail(HYQ_EINVAL, "hyq_wavespeed_batch_noalloc: at most 65536 cells per call");
+  if (n == 0) return HYQ_OK;
+  if (precision == HYQ_PREC_FP64) {  // no scratch on this path
+    hipLaunchKernelGGL((k_pass1<double, 0>), dim3(grid_for(n)), dim3(kBlock), 0, 0, M, Mr, Fx, vp,
+                       flags, (DevScalars*)nullptr, (long long)n, flag2D, Ma, positivity);
+  } else {
+    if (fast_batch_bytes(n) > sizeof(scratch)) return fail(HYQ_EINVAL, "scratch too small");
+    char* base = nullptr;
+    rc = scratch_address(&base);
+    if (rc) return rc;
+    enqueue_fast_batch(M, n, Ma, flag2D, positivity, Mr, Fx, vp, flags, base);
+  }
+  HIPCHK(hipGetLastError(), "batch launch");
+  HIPCHK(hipDeviceSynchronize(), "batch sync");
   return HYQ_OK;
 }
 

@@ -64,6 +64,8 @@ SIGNATURES = {
     "hyq_closure_batch": ([VP, I64, ctypes.c_double, ctypes.c_int32, VP, VP, VP, VP, VP], ctypes.c_int),
     "hyq_wavespeed_batch": ([VP, I64, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                              VP, VP, VP, VP, VP, VP], ctypes.c_int),
+    "hyq_wavespeed_batch_noalloc": ([VP, I64, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                     VP, VP, VP, VP, VP, VP], ctypes.c_int),
     "hyq_eig6_batch": ([VP, I64, VP, VP], ctypes.c_int),
     "hyq_collision_batch": ([VP, I64, ctypes.c_double, ctypes.c_double, ctypes.c_int32, VP], ctypes.c_int),
     "hyq_closure1d_batch": ([VP, I64, VP, VP], ctypes.c_int),

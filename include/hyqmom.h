@@ -173,6 +173,16 @@ int hyq_wavespeed_batch(const double* M, int64_t n, double Ma, int32_t flag2D,
                         int32_t positivity, int32_t precision, double* Mr, double* Fx,
                         double* Fy, double* Fz, double* vp, uint32_t* flags);
 
+/* hyq_wavespeed_batch without any allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  The fast mode's scratch lives in a fixed device array
+ * that every GPU holds until the process ends; at most 65536 cells per call (HYQ_EINVAL
+ * beyond), and calls that overlap on one GPU race.  Same arguments and results as
+ * hyq_wavespeed_batch. */
+int hyq_wavespeed_batch_noalloc(const double* M, int64_t n, double Ma, int32_t flag2D,
+                                int32_t positivity, int32_t precision, double* Mr, double* Fx,
+                                double* Fy, double* Fz, double* vp, uint32_t* flags);
+
 /* eigvals of jacobian6 (src/autogen/jacobian6.jl, compute_jacobian_eigenvalues.jl:18-50)
  * for n 15-moment slices ([15][n]); outputs the 6 eigenvalues sorted by real part
  * ([6][n] real, [6][n] imaginary). */
