@@ -407,11 +407,12 @@ struct ListSpread {
   __device__ unsigned first() const { return wave * per + lane; }
   __device__ bool mine(unsigned i) const { return lane < per && i < n; }
 };
-// Waves per SIMD of the list kernels.  With 1 they keep their state in the 512 registers
-// without spilling, but on the 120x120x60 workload that was 15-20% slower than 2 (their
-// time is one wave's latency through the full path, and two waves per SIMD share the cells).
-// Grid: that many waves per SIMD on every CU, 128-thread blocks.
-constexpr int kFullWaves = 2, kSolveWaves = 2;
+// Waves per SIMD of the list kernels (the full path's stages, the slot solver): their lists
+// are short, so each kernel's time is one wave's latency through its stage, which is
+// shortest with the whole register file (1 wave per SIMD: 256 VGPRs and 256 AGPRs, almost
+// no scratch), not with more waves.  Grid: one wave per SIMD (256 CUs x 4 SIMDs), so that
+// every wave with work runs at once, in 128-thread blocks.
+constexpr int kFullWaves = 1, kSolveWaves = 1;
 constexpr unsigned kListBlocks = 256 * 4 * kFullWaves / 2;
 
 // ---- the fast step (mixed_newton / mixed_learned modes) ------------------------------------
@@ -434,6 +435,9 @@ struct FastLists {
   uint32_t* slot_fl;   // [N] the cell's flags
   double* slot_v;      // [16][N] pass 1: plane extremes (10), 1D speeds (6)
   double* slot_A;      // [35][N] the projected state the planes come from
+  double* full_S;      // [35][N] the state of each full-path cell between its stages
+  double* full_v;      // [6][N] its x/y/z speeds
+  uint32_t* full_fl;   // [N] its flags
 };
 __device__ __forceinline__ unsigned new_slot(const FastLists& L, unsigned* count, long long c,
                                              unsigned pend, const double* A, long long N) {
@@ -581,47 +585,110 @@ __global__ __launch_bounds__(kBlock) void k_pass1_fixup(FastLists L, FT* __restr
   if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
 }
 
-// The full pass 1 (projection, QR fallbacks, corrections, z clamp / re-projection) from the
-// state M, for the listed cells.  P: the network's seeds (learned mode) or null.
-template <typename FT>
-__global__ __launch_bounds__(kBlock, kFullWaves) void k_pass1_full(
-    const double* __restrict__ M, double* __restrict__ AMr, FT* __restrict__ F,
-    FT* __restrict__ vp, uint32_t* __restrict__ flags, const float* __restrict__ P,
-    const int* __restrict__ list, DevScalars* sc, long long N, int flag2D, double Ma, int positivity) {
+// The full pass 1 of the listed cells, in stages: projection, x, y and z wave speeds (with
+// their eigenvalue corrections, the z density clamp and re-projection), and the closure with
+// the fluxes.  Each stage is one kernel over the list, the cell's state kept in full_S
+// between stages: the same functions, on the same values, as pass1_cell, so the results are
+// the same bits, but each kernel holds one stage's registers instead of the whole path's,
+// which keeps its latency (the kernel's time on a short list) down.  P: the network's seeds
+// (learned mode) or null.
+__device__ __forceinline__ Warm list_warm(const float* P, long long c, long long N) {
+  Warm w = no_warm();
+  if (P != nullptr) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      w.w[2 * q] = P[((size_t)q * N + c) * 2];
+      w.w[2 * q + 1] = P[((size_t)q * N + c) * 2 + 1];
+    }
+  }
+  return w;
+}
+// STAGE 0: projection of M; 1: x; 2: y; 3: z.
+template <int STAGE>
+__global__ __launch_bounds__(kBlock, kFullWaves) void k_full1_stage(
+    const double* __restrict__ M, const float* __restrict__ P, FastLists L, DevScalars* sc,
+    long long N, int flag2D, double Ma, int positivity) {
   if (sc->active == 0.0) return;
   const ListSpread ls(sc->nhard1);
   if (ls.idle()) return;  // wave-uniform
+  for (unsigned i = ls.first(); i < ls.n; i += ls.stride) {
+    if (!ls.mine(i)) continue;
+    const long long c = L.cells[i];
+    double a[35], b[35];
+    uint32_t fl = 0;
+    if (STAGE == 0) {
+#pragma unroll
+      for (int t = 0; t < 35; ++t) a[t] = M[(size_t)t * N + c];
+      pass1a_cell(a, flag2D, Ma, b, fl);
+    } else {
+#pragma unroll
+      for (int t = 0; t < 35; ++t) a[t] = L.full_S[(size_t)t * N + i];
+      fl = L.full_fl[i];
+      const Warm w = list_warm(P, c, N);
+      double mn, mx;
+      if (STAGE == 1) eig6_axis<0, 1>(a, flag2D, Ma, mn, mx, b, fl, w);
+      if (STAGE == 2) eig6_axis<1, 1>(a, flag2D, Ma, mn, mx, b, fl, w);
+      if (STAGE == 3) eig6_z<1>(a, flag2D, Ma, positivity != 0, mn, mx, b, fl, w);
+      L.full_v[(size_t)(2 * (STAGE - 1)) * N + i] = mn;
+      L.full_v[(size_t)(2 * (STAGE - 1) + 1) * N + i] = mx;
+    }
+#pragma unroll
+    for (int t = 0; t < 35; ++t) L.full_S[(size_t)t * N + i] = b[t];
+    L.full_fl[i] = fl;
+  }
+}
+// Last stage: closure and fluxes, 1D speeds, outputs.
+template <typename FT>
+__global__ __launch_bounds__(kBlock, kFullWaves) void k_full1_finish(
+    double* __restrict__ AMr, FT* __restrict__ F, FT* __restrict__ vp, uint32_t* __restrict__ flags,
+    FastLists L, DevScalars* sc, long long N, int flag2D, double Ma) {
+  if (sc->active == 0.0) return;
+  const ListSpread ls(sc->nhard1);
+  if (ls.idle()) return;
   unsigned long long vb = 0;
   for (unsigned i = ls.first(); i < ls.n; i += ls.stride) {
     if (!ls.mine(i)) continue;
-    const long long c = list[i];
-    double m[35], mr[35], v[6];
+    const long long c = L.cells[i];
+    double a[35], mr[35], s1[6], v6[6], v[6];
 #pragma unroll
-    for (int t = 0; t < 35; ++t) m[t] = M[(size_t)t * N + c];
-    Warm w = no_warm();
-    if (P != nullptr) {
+    for (int t = 0; t < 35; ++t) a[t] = L.full_S[(size_t)t * N + i];
 #pragma unroll
-      for (int q = 0; q < 5; ++q) {
-        w.w[2 * q] = P[((size_t)q * N + c) * 2];
-        w.w[2 * q + 1] = P[((size_t)q * N + c) * 2 + 1];
-      }
-    }
-    uint32_t fl = 0;
-    pass1_cell<FT, 1>(m, flag2D, Ma, positivity != 0, mr, F + c, F + (size_t)35 * N + c,
-                         F + (size_t)70 * N + c, (size_t)N, v, fl, w);
+    for (int t = 0; t < 6; ++t) v6[t] = L.full_v[(size_t)t * N + i];
+    uint32_t fl = L.full_fl[i];
+    flux_closure35<true, FT>(a, flag2D, Ma, mr, F + c, F + (size_t)35 * N + c,
+                             F + (size_t)70 * N + c, (size_t)N, fl);
+    closure1d_all(mr, s1);
+    combine_vp(s1, v6, v);
 #pragma unroll
     for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = mr[t];
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
       vp[(size_t)d * N + c] = (FT)v[d];
-      const unsigned long long b = __double_as_longlong(fabs(v[d]));
-      vb = b > vb ? b : vb;
+      const unsigned long long bb = __double_as_longlong(fabs(v[d]));
+      vb = bb > vb ? bb : vb;
     }
     flags[c] = fl;
     if (fl & F_NONFIN) atomicOr(&sc->err, 1ull);
   }
   vb = wave_max_u64(vb);
   if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
+}
+
+// The full pass 1's stages on `stream`.
+template <typename FT>
+static void enqueue_full1(hipStream_t stream, const double* M, double* AMr, FT* F, FT* vp,
+                          uint32_t* flags, const float* P, const FastLists& L, DevScalars* sc,
+                          long long N, int flag2D, double Ma, int positivity) {
+  hipLaunchKernelGGL(k_full1_stage<0>, dim3(kListBlocks), dim3(kBlock), 0, stream, M, P, L, sc, N,
+                     flag2D, Ma, positivity);
+  hipLaunchKernelGGL(k_full1_stage<1>, dim3(kListBlocks), dim3(kBlock), 0, stream, M, P, L, sc, N,
+                     flag2D, Ma, positivity);
+  hipLaunchKernelGGL(k_full1_stage<2>, dim3(kListBlocks), dim3(kBlock), 0, stream, M, P, L, sc, N,
+                     flag2D, Ma, positivity);
+  hipLaunchKernelGGL(k_full1_stage<3>, dim3(kListBlocks), dim3(kBlock), 0, stream, M, P, L, sc, N,
+                     flag2D, Ma, positivity);
+  hipLaunchKernelGGL(k_full1_finish<FT>, dim3(kListBlocks), dim3(kBlock), 0, stream, AMr, F, vp,
+                     flags, L, sc, N, flag2D, Ma);
 }
 
 // The fast modes' pass 2: projection and the x/y hyperbolicity checks (projected state to
@@ -698,23 +765,53 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_fixup(double* __re
   }
 }
 
-// The full pass 2 for the listed cells.
-__global__ __launch_bounds__(kBlock, kFullWaves) void k_pass2_full(double* __restrict__ M,
-                                                                  const int* __restrict__ list,
-                                                                  DevScalars* sc, long long N,
-                                                                  int flag2D, double Ma,
-                                                                  int positivity, double Kn) {
+// The full pass 2 of the listed cells, in stages like the full pass 1: projection, the x
+// and y corrections, then the closure and BGK.  STAGE 0: projection of M; 1: x; 2: y.
+template <int STAGE>
+__global__ __launch_bounds__(kBlock, kFullWaves) void k_full2_stage(const double* __restrict__ M,
+                                                                   FastLists L, DevScalars* sc,
+                                                                   long long N, int flag2D,
+                                                                   double Ma) {
   if (sc->active == 0.0) return;
   const ListSpread ls(sc->nhard2);
   if (ls.idle()) return;
   for (unsigned i = ls.first(); i < ls.n; i += ls.stride) {
     if (!ls.mine(i)) continue;
-    const long long c = list[i];
-    double m[35], o[35];
-#pragma unroll
-    for (int t = 0; t < 35; ++t) m[t] = M[(size_t)t * N + c];
+    const long long c = L.cells[i];
+    double a[35], b[35];
     uint32_t fl = 0;
-    pass2_cell<true, 1>(m, flag2D, Ma, positivity != 0, sc->dt, Kn, o, fl);
+    if (STAGE == 0) {
+#pragma unroll
+      for (int t = 0; t < 35; ++t) a[t] = M[(size_t)t * N + c];
+      flux_closure35<false, double>(a, flag2D, Ma, b, nullptr, nullptr, nullptr, 0, fl);
+    } else {
+#pragma unroll
+      for (int t = 0; t < 35; ++t) a[t] = L.full_S[(size_t)t * N + i];
+      double d0, d1;
+      // the wave speeds of pass 2 are not used, only its corrections
+      if (STAGE == 1) eig6_axis<0, 1>(a, flag2D, Ma, d0, d1, b, fl, no_warm(), false);
+      if (STAGE == 2) eig6_axis<1, 1>(a, flag2D, Ma, d0, d1, b, fl, no_warm(), false);
+    }
+#pragma unroll
+    for (int t = 0; t < 35; ++t) L.full_S[(size_t)t * N + i] = b[t];
+  }
+}
+__global__ __launch_bounds__(kBlock, kFullWaves) void k_full2_finish(double* __restrict__ M,
+                                                                    FastLists L, DevScalars* sc,
+                                                                    long long N, int flag2D,
+                                                                    double Ma, int positivity,
+                                                                    double Kn) {
+  if (sc->active == 0.0) return;
+  const ListSpread ls(sc->nhard2);
+  if (ls.idle()) return;
+  for (unsigned i = ls.first(); i < ls.n; i += ls.stride) {
+    if (!ls.mine(i)) continue;
+    const long long c = L.cells[i];
+    double a[35], o[35];
+#pragma unroll
+    for (int t = 0; t < 35; ++t) a[t] = L.full_S[(size_t)t * N + i];
+    uint32_t fl = 0;
+    pass2_relax<true>(a, flag2D, Ma, positivity != 0, sc->dt, Kn, o, fl);
 #pragma unroll
     for (int t = 0; t < 35; ++t) M[(size_t)t * N + c] = o[t];
   }
@@ -947,7 +1044,10 @@ int hyq_create(const hyq_params* params, hyq_ctx** out) {
          hipMalloc(&L.slot_bad, sizeof(unsigned) * c->N) == hipSuccess &&
          hipMalloc(&L.slot_fl, sizeof(uint32_t) * c->N) == hipSuccess &&
          hipMalloc(&L.slot_v, sizeof(double) * 16 * c->N) == hipSuccess &&
-         hipMalloc(&L.slot_A, sizeof(double) * 35 * c->N) == hipSuccess;
+         hipMalloc(&L.slot_A, sizeof(double) * 35 * c->N) == hipSuccess &&
+         hipMalloc(&L.full_S, sizeof(double) * 35 * c->N) == hipSuccess &&
+         hipMalloc(&L.full_v, sizeof(double) * 6 * c->N) == hipSuccess &&
+         hipMalloc(&L.full_fl, sizeof(uint32_t) * c->N) == hipSuccess;
   }
   if (!ok) {
     hyq_destroy(c);
@@ -982,6 +1082,9 @@ int hyq_destroy(hyq_ctx* c) {
   hipFree(c->L.slot_v);
   hipFree(c->L.slot_mask);
   hipFree(c->L.slot_A);
+  hipFree(c->L.full_S);
+  hipFree(c->L.full_v);
+  hipFree(c->L.full_fl);
   for (int i = 0; i < kNev * kChunk; ++i)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -1043,9 +1146,8 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
                        (const unsigned*)&c->sc->nslot1, &c->sc->nitem1, P, c->sc, c->N);
     hipLaunchKernelGGL(k_pass1_fixup<float>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
                        (float*)c->vp, c->flags, c->sc, c->N);
-    hipLaunchKernelGGL(k_pass1_full<float>, dim3(kListBlocks), dim3(kBlock), 0, c->stream,
-                       (const double*)c->M, c->Mr, (float*)c->F, (float*)c->vp, c->flags, P,
-                       (const int*)c->L.cells, c->sc, c->N, p.flag2D, p.Ma, p.positivity);
+    enqueue_full1<float>(c->stream, c->M, c->Mr, (float*)c->F, (float*)c->vp, c->flags, P, c->L,
+                         c->sc, c->N, p.flag2D, p.Ma, p.positivity);
   }
   if (ev) hipEventRecord(ev[1], c->stream);
   hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, c->stream, c->sc, p.CFL, hmin, p.dtmax, p.Kn);
@@ -1076,8 +1178,14 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
                        c->N);
     hipLaunchKernelGGL(k_pass2_fixup, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->M, c->L,
                        c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
-    hipLaunchKernelGGL(k_pass2_full, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->M,
-                       (const int*)c->L.cells, c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
+    hipLaunchKernelGGL(k_full2_stage<0>, dim3(kListBlocks), dim3(kBlock), 0, c->stream,
+                       (const double*)c->M, c->L, c->sc, c->N, p.flag2D, p.Ma);
+    hipLaunchKernelGGL(k_full2_stage<1>, dim3(kListBlocks), dim3(kBlock), 0, c->stream,
+                       (const double*)c->M, c->L, c->sc, c->N, p.flag2D, p.Ma);
+    hipLaunchKernelGGL(k_full2_stage<2>, dim3(kListBlocks), dim3(kBlock), 0, c->stream,
+                       (const double*)c->M, c->L, c->sc, c->N, p.flag2D, p.Ma);
+    hipLaunchKernelGGL(k_full2_finish, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->M, c->L,
+                       c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
   }
   if (ev) hipEventRecord(ev[4], c->stream);
   const hipError_t e = hipGetLastError();
@@ -1331,8 +1439,8 @@ static int batch_args(const double* M, int64_t n, int32_t precision, const doubl
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 static size_t fast_batch_bytes(long long n) {
   const size_t un = (size_t)n;
-  return align256(sizeof(DevScalars)) + 5 * align256(4 * un) + align256(16 * 8 * un) +
-         align256(35 * 8 * un);
+  return align256(sizeof(DevScalars)) + 6 * align256(4 * un) + align256(16 * 8 * un) +
+         2 * align256(35 * 8 * un) + align256(6 * 8 * un);
 }
 static void fast_batch_carve(char* base, long long n, DevScalars** sc, FastLists* L) {
   const size_t un = (size_t)n;
@@ -1344,7 +1452,10 @@ static void fast_batch_carve(char* base, long long n, DevScalars** sc, FastLists
   L->slot_bad = reinterpret_cast<unsigned*>(p); p += align256(4 * un);
   L->slot_fl = reinterpret_cast<uint32_t*>(p); p += align256(4 * un);
   L->slot_v = reinterpret_cast<double*>(p); p += align256(16 * 8 * un);
-  L->slot_A = reinterpret_cast<double*>(p);
+  L->slot_A = reinterpret_cast<double*>(p); p += align256(35 * 8 * un);
+  L->full_S = reinterpret_cast<double*>(p); p += align256(35 * 8 * un);
+  L->full_v = reinterpret_cast<double*>(p); p += align256(6 * 8 * un);
+  L->full_fl = reinterpret_cast<uint32_t*>(p);
 }
 // Scalars of a batch: active, everything else zero.  A kernel rather than a memset, so that
 // the launch sequence can be captured in a graph.
@@ -1372,8 +1483,8 @@ static void enqueue_fast_batch(const double* M, long long n, double Ma, int flag
   hipLaunchKernelGGL(k_slot_solve<true>, dim3(kListBlocks), dim3(kBlock), 0, 0, L,
                      (const unsigned*)&sc->nslot1, &sc->nitem1, (const float*)nullptr, sc, n);
   hipLaunchKernelGGL(k_pass1_fixup<double>, dim3(kListBlocks), dim3(kBlock), 0, 0, L, vp, flags, sc, n);
-  hipLaunchKernelGGL(k_pass1_full<double>, dim3(kListBlocks), dim3(kBlock), 0, 0, M, Mr, F, vp, flags,
-                     (const float*)nullptr, (const int*)L.cells, sc, n, flag2D, Ma, positivity);
+  enqueue_full1<double>(0, M, Mr, F, vp, flags, (const float*)nullptr, L, sc, n, flag2D, Ma,
+                        positivity);
 }
 
 int hyq_wavespeed_batch(const double* M, int64_t n, double Ma, int32_t flag2D,
@@ -1405,7 +1516,7 @@ int hyq_wavespeed_batch(const double* M, int64_t n, double Ma, int32_t flag2D,
 // (graph capture, serving loops).  Each device holds its own copy until the process ends;
 // every call on a device shares it, so calls that overlap on one device race.
 constexpr long long kNoAllocMaxCells = 1LL << 16;
-static __device__ __align__(256) char scratch[(5 * 4 + 16 * 8 + 35 * 8) * kNoAllocMaxCells + 8 * 256];
+static __device__ __align__(256) char scratch[(6 * 4 + 16 * 8 + 2 * 35 * 8 + 6 * 8) * kNoAllocMaxCells + 12 * 256];
 static std::mutex g_scratch_mu;
 static void* g_scratch_ptr[256];  // per device: the array's device address, once looked up
 

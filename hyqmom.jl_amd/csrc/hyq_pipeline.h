@@ -54,10 +54,11 @@ HD bool plane_near_gaussian(const Plane& p) { return p.ok && plane_gauss_dev(p) 
 // One plane by the fast path: Gaussian seed (when no warm start is given and the plane is
 // near-Gaussian), Sturm test (with the 2x2 block test of a nearly double pair), Laguerre
 // extremes.  Returns kPlaneProven with vmin/vmax set, kPlaneSolve when the plane is not
-// proven hyperbolic (the full solve of this plane decides), or kPlaneFull when the cell
-// must take the full path: the plane or its Jacobian is not finite (the full path carries
-// the reference's NaN semantics), or a complex pair is proven (the eigenvalue correction).
-constexpr int kPlaneProven = 0, kPlaneSolve = 1, kPlaneFull = 2;
+// proven hyperbolic (the full solve of this plane decides), kPlaneComplex when a complex
+// pair above the reference's test is proven (the eigenvalue correction), or kPlaneFull
+// when the plane or its Jacobian is not finite (the full path carries the reference's NaN
+// semantics).
+constexpr int kPlaneProven = 0, kPlaneSolve = 1, kPlaneFull = 2, kPlaneComplex = 3;
 template <bool PAIR = true>
 HD int plane_fast_status(const Plane& p, float wmin, float wmax, bool need_vals, uint32_t& fl,
                          double& vmin, double& vmax) {
@@ -77,7 +78,7 @@ HD int plane_fast_status(const Plane& p, float wmin, float wmax, bool need_vals,
   // root-polishing iterations, summed over the cell's plane solves, in flag bits 16..27
   const uint32_t it = (fl >> 16) + (uint32_t)fe.iters;
   fl = (fl & 0xFFFFu) | ((it < 0xFFFu ? it : 0xFFFu) << 16);
-  if (fe.cplx) return kPlaneFull;
+  if (fe.cplx) return kPlaneComplex;
   if (!fe.real) return kPlaneSolve;
   vmin = need_vals ? p.a + p.sig * fe.lmin : 0.0;
   vmax = need_vals ? p.a + p.sig * fe.lmax : 0.0;
@@ -88,12 +89,22 @@ HD bool plane_fast(const Plane& p, float wmin, float wmax, bool need_vals, uint3
   return plane_fast_status(p, wmin, wmax, need_vals, fl, vmin, vmax) == kPlaneProven;
 }
 
-template <int EM>
+// CORRECTS: a complex pair leads to the eigenvalue correction, which replaces the speeds, so
+// a proven complex pair needs no values; after the correction (CORRECTS = false) the
+// reference takes the real parts of whatever the eigenvalues are, so the QR gives them.
+template <int EM, bool CORRECTS = true>
 HD EigRes plane_solve(const Plane& p, float wmin, float wmax, bool need_vals, uint32_t& fl) {
   if (EM == 0) return plane_eigs(p, fl);
   EigRes r;
-  if (plane_fast(p, wmin, wmax, need_vals, fl, r.vmin, r.vmax)) {
+  const int st = plane_fast_status<true>(p, wmin, wmax, need_vals, fl, r.vmin, r.vmax);
+  if (st == kPlaneProven) {
     r.cplx = false;
+    r.ok = true;
+    return r;
+  }
+  if (CORRECTS && st == kPlaneComplex) {
+    r.vmin = r.vmax = NAN;
+    r.cplx = true;
     r.ok = true;
     return r;
   }
@@ -160,11 +171,11 @@ HD void eig6_axis(const double* M, int flag2D, double Ma, double& vmin, double& 
     flux_closure35<false, double>(Mh, flag2D, Ma, Mr, nullptr, nullptr, nullptr, 0, fl);
     raw_to_central4(Mr, c, M000, um, vm, wm);
     if (AXIS == 0) {
-      A = plane_solve<EM>(plane<0, 1>(c, um, vm, wm), wm_.w[2 * ia], wm_.w[2 * ia + 1], need_vals, fl);
-      B = plane_solve<EM>(plane<0, 2>(c, um, vm, wm), wm_.w[2 * ib], wm_.w[2 * ib + 1], need_vals, fl);
+      A = plane_solve<EM, false>(plane<0, 1>(c, um, vm, wm), wm_.w[2 * ia], wm_.w[2 * ia + 1], need_vals, fl);
+      B = plane_solve<EM, false>(plane<0, 2>(c, um, vm, wm), wm_.w[2 * ib], wm_.w[2 * ib + 1], need_vals, fl);
     } else {
-      A = plane_solve<EM>(plane<1, 0>(c, um, vm, wm), wm_.w[2 * ia], wm_.w[2 * ia + 1], need_vals, fl);
-      B = plane_solve<EM>(plane<1, 2>(c, um, vm, wm), wm_.w[2 * ib], wm_.w[2 * ib + 1], need_vals, fl);
+      A = plane_solve<EM, false>(plane<1, 0>(c, um, vm, wm), wm_.w[2 * ia], wm_.w[2 * ia + 1], need_vals, fl);
+      B = plane_solve<EM, false>(plane<1, 2>(c, um, vm, wm), wm_.w[2 * ib], wm_.w[2 * ib + 1], need_vals, fl);
     }
     combine_guarded(A, B, vmin, vmax);
   } else {
@@ -221,8 +232,8 @@ HD void eig6_z(const double* M, int flag2D, double Ma, bool positivity, double& 
     pack35(cn, Mh);
     flux_closure35<false, double>(Mh, flag2D, Ma, Mr, nullptr, nullptr, nullptr, 0, fl);
     raw_to_central4(Mr, c, M000, um, vm, wm);
-    A = plane_solve<EM>(plane<2, 0>(c, um, vm, wm), wm_.w[8], wm_.w[9], true, fl);
-    B = plane_solve<EM>(plane<1, 2>(c, um, vm, wm), wm_.w[6], wm_.w[7], true, fl);
+    A = plane_solve<EM, false>(plane<2, 0>(c, um, vm, wm), wm_.w[8], wm_.w[9], true, fl);
+    B = plane_solve<EM, false>(plane<1, 2>(c, um, vm, wm), wm_.w[6], wm_.w[7], true, fl);
     vmin = jmin(A.vmin, B.vmin);
     vmax = jmax(A.vmax, B.vmax);
   }
@@ -363,7 +374,7 @@ HD int fast_cell_planes(const double* A, bool positivity, const Warm& w, double 
     const int st = plane_fast_status<PAIR>(cell_plane(c, um, vm, wm, q), VALS ? w.w[2 * q] : nan,
                                            VALS ? w.w[2 * q + 1] : nan, VALS, fl, v[2 * q],
                                            v[2 * q + 1]);
-    if (st == kPlaneFull) full = true;
+    if (st == kPlaneFull || st == kPlaneComplex) full = true;
     if (st == kPlaneSolve) pend |= 1u << q;
   }
   if (full) return kPlaneFull;
@@ -447,7 +458,7 @@ HD bool solve_pending_planes(const double* A, unsigned pend, bool need_vals, con
     const int st = plane_fast_status<true>(p, need_vals ? w.w[2 * q] : nan,
                                            need_vals ? w.w[2 * q + 1] : nan, need_vals, fl,
                                            v[2 * q], v[2 * q + 1]);
-    if (st == kPlaneFull) return false;
+    if (st == kPlaneFull || st == kPlaneComplex) return false;
     if (st == kPlaneSolve) {
       fl |= F_FALLBACK;
       const EigRes r = plane_eigs(p, fl);
