@@ -109,12 +109,24 @@ HD void hessenberg_n(double* A) {
   for (int k = 0; k < N - 2; ++k) {
     double alpha = 0;
     for (int i = k + 1; i < N; ++i) alpha += A[i * N + k] * A[i * N + k];
-    alpha = sqrt(alpha);
-    if (alpha == 0.0) continue;
-    if (A[(k + 1) * N + k] > 0) alpha = -alpha;
     double v[N];
     for (int i = 0; i < N; ++i) v[i] = 0.0;
     for (int i = k + 1; i < N; ++i) v[i] = A[i * N + k];
+    if (alpha < 1e-280) {
+      // squares at the underflow threshold (subnormal entries): the reflector is that of
+      // the column scaled to unit max-abs entry, as LAPACK's dlarfg rescales tiny columns
+      double amax = 0.0;
+      for (int i = k + 1; i < N; ++i) amax = fmax(amax, fabs(v[i]));
+      if (amax == 0.0) continue;
+      alpha = 0.0;
+      for (int i = k + 1; i < N; ++i) {
+        v[i] /= amax;
+        alpha += v[i] * v[i];
+      }
+    }
+    alpha = sqrt(alpha);
+    if (alpha == 0.0) continue;
+    if (A[(k + 1) * N + k] > 0) alpha = -alpha;
     v[k + 1] -= alpha;
     double vn = 0;
     for (int i = k + 1; i < N; ++i) vn += v[i] * v[i];

@@ -69,10 +69,11 @@ def closure1d(m5):
 
 def plane_compare(M, f, g, wmin=float("nan"), wmax=float("nan")):
     """QR and fast (Sturm + Laguerre) solves of plane (f, g): dict of both results."""
-    M = _f(M); out = np.zeros(9)
+    M = _f(M); out = np.zeros(10)
     lib().cell_plane_compare(_p(M), f, g, ctypes.c_double(wmin), ctypes.c_double(wmax), _p(out))
     return {"qr": (out[0], out[1]), "qr_cplx": bool(out[2]), "fast": (out[3], out[4]),
-            "fast_real": bool(out[5]), "iters": int(out[6]), "shift": out[7], "scale": out[8]}
+            "fast_real": bool(out[5]), "iters": int(out[6]), "shift": out[7], "scale": out[8],
+            "fast_cplx": bool(out[9])}
 
 
 def jacobian_std(M, f, g):
@@ -80,6 +81,14 @@ def jacobian_std(M, f, g):
     M = _f(M); J = np.zeros(36)
     lib().cell_jacobian_std(_p(M), f, g, _p(J))
     return J.reshape(6, 6)
+
+
+def fast_eigs_J(J, sig=1.0, need_vals=True):
+    """The fast standardized solve on a 6x6 matrix: dict (real, cplx, lmin, lmax, iters)."""
+    J = _f(J).ravel(); out = np.zeros(5)
+    lib().cell_fast_eigs_J(_p(J), ctypes.c_double(sig), int(need_vals), _p(out))
+    return {"real": bool(out[0]), "cplx": bool(out[1]), "lmin": out[2], "lmax": out[3],
+            "iters": int(out[4])}
 
 
 def plane_std_extremes(M, f, g):

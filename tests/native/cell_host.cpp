@@ -105,7 +105,8 @@ extern "C" void cell_pass2_flags(const double* M, int flag2D, double Ma, double 
   *fl_out = fl;
 }
 // Plane solve both ways on the standardized Jacobian of plane (f,g) of a 35-moment state.
-// out: [qr_min, qr_max, qr_cplx, fast_min, fast_max, fast_real, fast_iters, shift, scale]
+// out: [qr_min, qr_max, qr_cplx, fast_min, fast_max, fast_real, fast_iters, shift, scale,
+// fast_cplx]
 // (raw scale; standardized = (raw - shift) / scale)
 extern "C" void cell_plane_compare(const double* M, int f, int g, double wmin, double wmax,
                                    double* out) {
@@ -121,11 +122,11 @@ extern "C" void cell_plane_compare(const double* M, int f, int g, double wmin, d
   const EigRes q = plane_eigs(p, fl);
   double J[36];
   jacobian_std(p, J);
-  const FastEig fe = fast_eigs_std(J, wmin, wmax, true);
+  const FastEig fe = fast_eigs_std(J, wmin, wmax, true, p.sig);
   out[0] = q.vmin; out[1] = q.vmax; out[2] = q.cplx ? 1 : 0;
   out[3] = p.a + p.sig * fe.lmin; out[4] = p.a + p.sig * fe.lmax; out[5] = fe.real ? 1 : 0;
   out[6] = fe.iters;
-  out[7] = p.a; out[8] = p.sig;
+  out[7] = p.a; out[8] = p.sig; out[9] = fe.cplx ? 1 : 0;
 }
 
 // The QR iteration both ways on a 6x6 matrix (balanced and reduced to Hessenberg form first):
@@ -159,3 +160,11 @@ extern "C" void cell_jacobian_std(const double* M, int f, int g, double* J) {
   jacobian_std(p, J);
 }
 
+
+// The fast standardized solve on an arbitrary 6x6 matrix (row-major).  out: [real, cplx,
+// lmin, lmax, iters].
+extern "C" void cell_fast_eigs_J(const double* J36, double sig, int need_vals, double* out) {
+  const FastEig fe = fast_eigs_std(J36, NAN, NAN, need_vals != 0, sig);
+  out[0] = fe.real ? 1 : 0; out[1] = fe.cplx ? 1 : 0; out[2] = fe.lmin; out[3] = fe.lmax;
+  out[4] = fe.iters;
+}
