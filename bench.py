@@ -17,12 +17,15 @@ run therefore starts from the exact initial state and, if that trajectory breaks
 from the seeded one-ulp perturbations 1, 2, ... until one completes the steps; the failed
 attempts are reported ("initial_state") and not timed.
 
-Roofline: per kernel (pass 1, update, pass 2) the average duration from HIP events on the
-context stream over the timed steps; pass 1 and pass 2 are priced as FP64 work
-(algorithmic FLOPs counted on the states of the run by the op-counting host build of the
-same cell code) against the 78.6 TFLOP/s FP64 peak, the update as HBM traffic
-(algorithmic bytes per cell) against 8 TB/s.  The "roofline" object is the kernel with the
-largest share of the step.
+Roofline: per kernel group (pass 1, update, pass 2: the launches of each phase) the average
+duration from HIP events on the context stream over the timed steps; pass 1 and pass 2 are
+priced as FP64 vector (VALU) work (algorithmic FLOPs counted on the states of the run by the
+op-counting host build of the same cell code) against MI355X's 78.6 TFLOP/s FP64 vector
+peak, the update as HBM traffic (algorithmic bytes per cell) against 8 TB/s.  The
+"roofline" object is the group with the largest share of the step; its "traffic" is the
+HBM bytes per step that rocprofv3's FETCH_SIZE and WRITE_SIZE counters measured for the
+same group on the same workload and step count (profiles/hbm_traffic.json, made by the
+commands recorded there), or null where no such measurement exists.
 """
 import argparse
 import json
@@ -33,11 +36,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "hyqmom.jl_amd"))
 
-# HBM bytes per step of pass 1 in the mixed+learned mode (k_pass1a + k_mlp_fused +
-# k_pass1_fast + k_pass1_full), measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE in
-# separate runs, each scaled by what the counter reads for a 1 GiB device copy (bench.py
-# --copy-calibration 1): 0.907 + 0.173 + 1.720 + 0.460 GB, averaged over the first 40 steps
-MEASURED_PASS1_TRAFFIC = {"mixed_learned": 3.26e9}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 GRID = (120, 120, 60)
@@ -254,32 +253,69 @@ def accuracy(N, precision, M, model=None):
     return out
 
 
-def cpu_baseline(M, step, seconds=10.0):
-    """The oracle (FP64 CPU restatement of the reference step) on this host's cores, on a
-    bounded sample of the same workload: steps of the 16x16x16 block around a jet's centre
-    (cells 22..37 in x, y and z), cut from the FP64 state at the given step."""
+def oracle_rate(M, lo, n, nthreads, seconds, max_steps):
+    """cell-steps/s of the oracle on the n^3 block of cells lo..lo+n-1 in x, y and z."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    nth = os.cpu_count() or 1
-    sub = np.asfortranarray(M[22:38, 22:38, 22:38, :])
+    sub = np.asfortranarray(M[lo:lo + n, lo:lo + n, lo:lo + n, :])
     t0 = time.time()
     steps = 0
-    while time.time() - t0 < seconds and steps < 50:
-        sub, _, _, _, _ = O.run(sub, 16, 16, 16, 0.01, 1.5, 0, 0.7, 1e-2, 1 / 120, 1 / 120, 1 / 60, True,
-                                1e9, 1, nthreads=nth)
+    while time.time() - t0 < seconds and steps < max_steps:
+        sub, _, _, _, _ = O.run(sub, n, n, n, 0.01, 1.5, 0, 0.7, 1e-2, 1 / 120, 1 / 120, 1 / 60, True,
+                                1e9, 1, nthreads=nthreads)
         steps += 1
-    el = time.time() - t0
-    return {"value": 16 ** 3 * steps / el, "unit": "cell-steps/s", "cores": nth, "kind": "port",
+    return n ** 3 * steps / (time.time() - t0), steps
+
+
+def cpu_baseline(M, step, seconds=10.0):
+    """The oracle (FP64 CPU restatement of the reference step, no MPI: one rank) on this
+    host's cores, on a bounded sample of the same workload: steps of the 16x16x16 block
+    around a jet's centre (cells 22..37 in x, y and z), cut from the FP64 state at the given
+    step, with one OpenMP thread per core; and steps of the 8x8x8 block at its centre (cells
+    26..33) on one thread."""
+    nth = os.cpu_count() or 1
+    v, steps = oracle_rate(M, 22, 16, nth, seconds, 50)
+    v1, steps1 = oracle_rate(M, 26, 8, 1, seconds / 2, 50)
+    return {"value": v, "unit": "cell-steps/s", "cores": nth, "kind": "port", "mpi_ranks": 1,
+            "cpu_model": host_cpu_model(),
             "sample": "%d consecutive steps of the 16^3 block around a jet centre of the 120x120x60 "
-                      "crossing-jets FP64 state at step %d, %d OpenMP threads" % (steps, step, nth)}
+                      "crossing-jets FP64 state at step %d, %d OpenMP threads" % (steps, step, nth),
+            "one_thread": {"value": v1, "unit": "cell-steps/s",
+                           "sample": "%d consecutive steps of the 8^3 block at the same jet centre, "
+                                     "1 thread" % steps1}}
+
+
+def traffic_key(precision, N3, steps):
+    return "%s %dx%dx%d %d steps" % ((precision,) + tuple(N3) + (steps,))
+
+
+def measured_traffic(precision, N3, steps, group):
+    """HBM bytes per step of a kernel group measured by rocprofv3 on this workload, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)["runs"].get(traffic_key(precision, N3, steps))
+    except (OSError, ValueError, KeyError):
+        return None
+    return None if rec is None else rec.get(group)
+
+
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def kernel_report(run, flops, precision, N):
     p1f, p2f = flops if flops else (None, None)
     ks = {}
     for name, ms, fl in (("pass1", run["pass1_ms"], p1f), ("pass2", run["pass2_ms"], p2f)):
-        d = {"ms": ms, "bound": "mfma", "unit": "TFLOP/s", "peak": PEAK_FP64_TFLOPS}
+        d = {"ms": ms, "bound": "valu", "unit": "TFLOP/s", "peak": PEAK_FP64_TFLOPS}
         if fl is not None:
             d["flops_per_cell"] = fl
             d["achieved"] = fl * N / (ms * 1e-3) / 1e12
@@ -322,7 +358,7 @@ def main():
     ks = kernel_report(run, flops, a.precision, N)
     dom = max(ks, key=lambda k: ks[k]["ms"])
     d = ks[dom]
-    traffic = MEASURED_PASS1_TRAFFIC.get(a.precision) if dom == "pass1" and N3 == GRID else None
+    traffic = measured_traffic(a.precision, N3, a.steps, dom)
     roof = {"kernel": dom, "bound": d["bound"], "achieved": d.get("achieved"), "peak": d["peak"],
             "unit": d["unit"], "frac": d.get("frac"), "traffic": traffic}
     out = {

@@ -363,9 +363,14 @@ HD FastEig fast_eigs_std(const double* J, double wmin, double wmax, bool need_va
   if (S.ok) {
     // all six roots real <=> distinct real roots = 6 - deg gcd(p, p')
     nreal = sturm_real_roots(S);
-    // a resolved complex pair: the count does not bound its imaginary part, which may lie
-    // below the reference's 1000 eps test (no correction there), so the full solve decides
-    if (nreal != 6 - sturm_gcd_degree(S)) return r;
+    if (nreal != 6 - sturm_gcd_degree(S)) {
+      // a complex pair in a resolved chain: every remainder is above 1e-9 of its dividend,
+      // so no two roots are nearly double and the pair's imaginary part is of the order of
+      // sqrt(1e-9) of the roots' spread, orders of magnitude above the reference's
+      // 1000 eps test (eigenvalues6_hyperbolic_3D.jl:35): the eigenvalue correction
+      r.cplx = true;
+      return r;
+    }
   } else {
     if (!PAIR || !S.pair) return r;
     // p0..p5 resolved: four simple real roots and a nearly double pair near the root of p5,

@@ -22,3 +22,22 @@ def test_update_bytes():
     # read Mr (35 FP64), three flux blocks (3 x 35) and 6 speeds, write the state (35 FP64)
     assert bench.update_bytes("fp64") == 280 + 840 + 48 + 280
     assert bench.update_bytes("mixed_learned") == 280 + 420 + 24 + 280
+
+
+def test_measured_traffic_lookup(tmp_path, monkeypatch):
+    f = tmp_path / "t.json"
+    f.write_text('{"runs": {"mixed_learned 120x120x60 20 steps": {"pass1": 2.5e9}}}')
+    monkeypatch.setattr(bench, "TRAFFIC_FILE", str(f))
+    assert bench.measured_traffic("mixed_learned", (120, 120, 60), 20, "pass1") == 2.5e9
+    assert bench.measured_traffic("mixed_learned", (120, 120, 60), 200, "pass1") is None
+    assert bench.measured_traffic("fp64", (120, 120, 60), 20, "pass1") is None
+    assert bench.measured_traffic("mixed_learned", (120, 120, 60), 20, "update") is None
+
+
+def test_cpu_baseline_record():
+    import hyqmom_amd as H
+    M = np.asfortranarray(H.initial_state(bench.jets_params((40, 40, 40), "fp64")))
+    r = bench.cpu_baseline(M, 0, seconds=0.2)
+    assert r["value"] > 0 and r["one_thread"]["value"] > 0
+    assert r["mpi_ranks"] == 1 and r["kind"] == "port" and r["cores"] >= 1
+    assert "16^3" in r["sample"] and "8^3" in r["one_thread"]["sample"]
