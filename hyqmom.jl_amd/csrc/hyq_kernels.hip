@@ -438,7 +438,31 @@ struct FastLists {
   double* full_S;      // [35][N] the state of each full-path cell between its stages
   double* full_v;      // [6][N] its x/y/z speeds
   uint32_t* full_fl;   // [N] its flags
+  unsigned* full_known;  // [N] pass 2: the decisions of its x/y planes on full_S (known_of)
 };
+// A slot whose pending planes include a complex pair or a non-finite plane: slot_bad holds
+// this bit and the decisions the plane solves reached (known_of).
+constexpr unsigned kSlotBad = 0x80000000u;
+// Appends cell c with its projected state A to the full path's list; returns its index.
+__device__ __forceinline__ unsigned new_hard(const FastLists& L, unsigned* count, long long c,
+                                             const double* A, uint32_t fl, long long N) {
+  const unsigned i = atomicAdd(count, 1u);
+  L.cells[i] = (int)c;
+#pragma unroll
+  for (int t = 0; t < 35; ++t) L.full_S[(size_t)t * N + i] = A[t];
+  L.full_fl[i] = fl;
+  return i;
+}
+// Moves slot s (cell c) to the full path's list with the projected state it holds.
+__device__ __forceinline__ unsigned slot_to_hard(const FastLists& L, unsigned* count, unsigned s,
+                                                 long long c, uint32_t fl, long long N) {
+  const unsigned i = atomicAdd(count, 1u);
+  L.cells[i] = (int)c;
+#pragma unroll
+  for (int t = 0; t < 35; ++t) L.full_S[(size_t)t * N + i] = L.slot_A[(size_t)t * N + s];
+  L.full_fl[i] = fl;
+  return i;
+}
 __device__ __forceinline__ unsigned new_slot(const FastLists& L, unsigned* count, long long c,
                                              unsigned pend, const double* A, long long N) {
   const unsigned slot = atomicAdd(count, 1u);
@@ -464,7 +488,8 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_fast(
     double a[35];
 #pragma unroll
     for (int t = 0; t < 35; ++t) a[t] = AMr[(size_t)t * N + c];
-    uint32_t fl = flags[c];
+    const uint32_t fl0 = flags[c];
+    uint32_t fl = fl0;
     Warm w = no_warm();
     if (P != nullptr) {
 #pragma unroll
@@ -477,7 +502,7 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_fast(
     unsigned pend;
     const int st = fast_cell_planes<false, true>(a, positivity != 0, w, v, pend, fl);
     if (st == kPlaneFull) {
-      L.cells[atomicAdd(&sc->nhard1, 1u)] = (int)c;
+      new_hard(L, &sc->nhard1, c, a, fl0, N);  // the full path starts from the projection
     } else {
       const unsigned slot = st == kPlaneSolve ? new_slot(L, &sc->nslot1, c, pend, a, N) : 0u;
       double mr[35], s1[6];
@@ -539,8 +564,9 @@ __global__ __launch_bounds__(kBlock, kSolveWaves) void k_slot_solve(FastLists L,
       }
     }
     uint32_t fl = VALS ? L.slot_fl[s] : 0u;
-    if (!solve_pending_planes(A, L.slot_mask[s], VALS, w, v, fl)) {
-      L.slot_bad[s] = 1u;
+    unsigned known = 0u;
+    if (!solve_pending_planes(A, L.slot_mask[s], VALS, w, v, fl, &known)) {
+      L.slot_bad[s] = kSlotBad | known;
       continue;
     }
     if (VALS) {
@@ -563,7 +589,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1_fixup(FastLists L, FT* __restr
     const long long c = L.slot_cell[s];
     if (L.slot_bad[s]) {
       atomicAdd(&sc->ncplx1, 1u);
-      L.cells[atomicAdd(&sc->nhard1, 1u)] = (int)c;
+      slot_to_hard(L, &sc->nhard1, s, c, flags[c], N);  // flags[c]: those of the projection
       continue;
     }
     double v[10], s1[6], v6[6], vd[6];
@@ -603,11 +629,12 @@ __device__ __forceinline__ Warm list_warm(const float* P, long long c, long long
   }
   return w;
 }
-// STAGE 0: projection of M; 1: x; 2: y; 3: z.
+// The list holds each cell's projected state (pass1a_cell, computed once by k_pass1a) and
+// the flags of that projection.  STAGE 1: x; 2: y; 3: z.
 template <int STAGE>
 __global__ __launch_bounds__(kBlock, kFullWaves) void k_full1_stage(
-    const double* __restrict__ M, const float* __restrict__ P, FastLists L, DevScalars* sc,
-    long long N, int flag2D, double Ma, int positivity) {
+    const float* __restrict__ P, FastLists L, DevScalars* sc, long long N, int flag2D, double Ma,
+    int positivity) {
   if (sc->active == 0.0) return;
   const ListSpread ls(sc->nhard1);
   if (ls.idle()) return;  // wave-uniform
@@ -615,15 +642,10 @@ __global__ __launch_bounds__(kBlock, kFullWaves) void k_full1_stage(
     if (!ls.mine(i)) continue;
     const long long c = L.cells[i];
     double a[35], b[35];
-    uint32_t fl = 0;
-    if (STAGE == 0) {
 #pragma unroll
-      for (int t = 0; t < 35; ++t) a[t] = M[(size_t)t * N + c];
-      pass1a_cell(a, flag2D, Ma, b, fl);
-    } else {
-#pragma unroll
-      for (int t = 0; t < 35; ++t) a[t] = L.full_S[(size_t)t * N + i];
-      fl = L.full_fl[i];
+    for (int t = 0; t < 35; ++t) a[t] = L.full_S[(size_t)t * N + i];
+    uint32_t fl = L.full_fl[i];
+    {
       const Warm w = list_warm(P, c, N);
       double mn, mx;
       if (STAGE == 1) eig6_axis<0, 1>(a, flag2D, Ma, mn, mx, b, fl, w);
@@ -676,16 +698,14 @@ __global__ __launch_bounds__(kBlock, kFullWaves) void k_full1_finish(
 
 // The full pass 1's stages on `stream`.
 template <typename FT>
-static void enqueue_full1(hipStream_t stream, const double* M, double* AMr, FT* F, FT* vp,
-                          uint32_t* flags, const float* P, const FastLists& L, DevScalars* sc,
-                          long long N, int flag2D, double Ma, int positivity) {
-  hipLaunchKernelGGL(k_full1_stage<0>, dim3(kListBlocks), dim3(kBlock), 0, stream, M, P, L, sc, N,
+static void enqueue_full1(hipStream_t stream, double* AMr, FT* F, FT* vp, uint32_t* flags,
+                          const float* P, const FastLists& L, DevScalars* sc, long long N,
+                          int flag2D, double Ma, int positivity) {
+  hipLaunchKernelGGL(k_full1_stage<1>, dim3(kListBlocks), dim3(kBlock), 0, stream, P, L, sc, N,
                      flag2D, Ma, positivity);
-  hipLaunchKernelGGL(k_full1_stage<1>, dim3(kListBlocks), dim3(kBlock), 0, stream, M, P, L, sc, N,
+  hipLaunchKernelGGL(k_full1_stage<2>, dim3(kListBlocks), dim3(kBlock), 0, stream, P, L, sc, N,
                      flag2D, Ma, positivity);
-  hipLaunchKernelGGL(k_full1_stage<2>, dim3(kListBlocks), dim3(kBlock), 0, stream, M, P, L, sc, N,
-                     flag2D, Ma, positivity);
-  hipLaunchKernelGGL(k_full1_stage<3>, dim3(kListBlocks), dim3(kBlock), 0, stream, M, P, L, sc, N,
+  hipLaunchKernelGGL(k_full1_stage<3>, dim3(kListBlocks), dim3(kBlock), 0, stream, P, L, sc, N,
                      flag2D, Ma, positivity);
   hipLaunchKernelGGL(k_full1_finish<FT>, dim3(kListBlocks), dim3(kBlock), 0, stream, AMr, F, vp,
                      flags, L, sc, N, flag2D, Ma);
@@ -706,8 +726,8 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_check(const double
 #pragma unroll
   for (int n = 0; n < 35; ++n) m[n] = M[(size_t)n * N + c];
   uint32_t fl = 0;
-  unsigned pend;
-  const int st = pass2_check_planes<false>(m, flag2D, Ma, a, pend, fl);
+  unsigned pend, known;
+  const int st = pass2_check_planes<false>(m, flag2D, Ma, a, pend, fl, &known);
   if (st == kPlaneProven) {
 #pragma unroll
     for (int n = 0; n < 35; ++n) A[(size_t)n * N + c] = a[n];
@@ -715,7 +735,7 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_check(const double
   }
   A[c] = NAN;
   if (st == kPlaneFull)
-    L.cells[atomicAdd(&sc->nhard2, 1u)] = (int)c;
+    L.full_known[new_hard(L, &sc->nhard2, c, a, 0u, N)] = known;
   else
     new_slot(L, &sc->nslot2, c, pend, a, N);
 }
@@ -750,9 +770,17 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_fixup(double* __re
   const unsigned n = sc->nslot2;
   for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
     const long long c = L.slot_cell[s];
-    if (L.slot_bad[s]) {
+    const unsigned bad = L.slot_bad[s];
+    if (bad) {
       atomicAdd(&sc->ncplx2, 1u);
-      L.cells[atomicAdd(&sc->nhard2, 1u)] = (int)c;
+      const unsigned i = slot_to_hard(L, &sc->nhard2, s, c, 0u, N);
+      // the planes that were not pending were proven hyperbolic by the check
+      unsigned known = bad & 0xFFu;
+      const unsigned pend = L.slot_mask[s];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (!(pend & (1u << q))) known |= kKnownReal << (2 * q);
+      L.full_known[i] = known;
       continue;
     }
     double a[35], o[35];
@@ -765,11 +793,11 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_fixup(double* __re
   }
 }
 
-// The full pass 2 of the listed cells, in stages like the full pass 1: projection, the x
-// and y corrections, then the closure and BGK.  STAGE 0: projection of M; 1: x; 2: y.
+// The full pass 2 of the listed cells, in stages like the full pass 1: the x and y
+// corrections of the projected state the check left in the list (with the plane decisions the
+// check and the plane solves reached on it), then the closure and BGK.  STAGE 1: x; 2: y.
 template <int STAGE>
-__global__ __launch_bounds__(kBlock, kFullWaves) void k_full2_stage(const double* __restrict__ M,
-                                                                   FastLists L, DevScalars* sc,
+__global__ __launch_bounds__(kBlock, kFullWaves) void k_full2_stage(FastLists L, DevScalars* sc,
                                                                    long long N, int flag2D,
                                                                    double Ma) {
   if (sc->active == 0.0) return;
@@ -777,20 +805,17 @@ __global__ __launch_bounds__(kBlock, kFullWaves) void k_full2_stage(const double
   if (ls.idle()) return;
   for (unsigned i = ls.first(); i < ls.n; i += ls.stride) {
     if (!ls.mine(i)) continue;
-    const long long c = L.cells[i];
     double a[35], b[35];
     uint32_t fl = 0;
-    if (STAGE == 0) {
 #pragma unroll
-      for (int t = 0; t < 35; ++t) a[t] = M[(size_t)t * N + c];
-      flux_closure35<false, double>(a, flag2D, Ma, b, nullptr, nullptr, nullptr, 0, fl);
+    for (int t = 0; t < 35; ++t) a[t] = L.full_S[(size_t)t * N + i];
+    const unsigned known = L.full_known[i];
+    // the wave speeds of pass 2 are not used, only its corrections
+    if (STAGE == 1) {
+      // after an x correction the y decisions no longer hold
+      if (eig6_axis_decided<0>(a, flag2D, Ma, b, fl, known)) L.full_known[i] = known & 0x0Fu;
     } else {
-#pragma unroll
-      for (int t = 0; t < 35; ++t) a[t] = L.full_S[(size_t)t * N + i];
-      double d0, d1;
-      // the wave speeds of pass 2 are not used, only its corrections
-      if (STAGE == 1) eig6_axis<0, 1>(a, flag2D, Ma, d0, d1, b, fl, no_warm(), false);
-      if (STAGE == 2) eig6_axis<1, 1>(a, flag2D, Ma, d0, d1, b, fl, no_warm(), false);
+      eig6_axis_decided<1>(a, flag2D, Ma, b, fl, known);
     }
 #pragma unroll
     for (int t = 0; t < 35; ++t) L.full_S[(size_t)t * N + i] = b[t];
@@ -1047,7 +1072,8 @@ int hyq_create(const hyq_params* params, hyq_ctx** out) {
          hipMalloc(&L.slot_A, sizeof(double) * 35 * c->N) == hipSuccess &&
          hipMalloc(&L.full_S, sizeof(double) * 35 * c->N) == hipSuccess &&
          hipMalloc(&L.full_v, sizeof(double) * 6 * c->N) == hipSuccess &&
-         hipMalloc(&L.full_fl, sizeof(uint32_t) * c->N) == hipSuccess;
+         hipMalloc(&L.full_fl, sizeof(uint32_t) * c->N) == hipSuccess &&
+         hipMalloc(&L.full_known, sizeof(unsigned) * c->N) == hipSuccess;
   }
   if (!ok) {
     hyq_destroy(c);
@@ -1085,6 +1111,7 @@ int hyq_destroy(hyq_ctx* c) {
   hipFree(c->L.full_S);
   hipFree(c->L.full_v);
   hipFree(c->L.full_fl);
+  hipFree(c->L.full_known);
   for (int i = 0; i < kNev * kChunk; ++i)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -1146,7 +1173,7 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
                        (const unsigned*)&c->sc->nslot1, &c->sc->nitem1, P, c->sc, c->N);
     hipLaunchKernelGGL(k_pass1_fixup<float>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
                        (float*)c->vp, c->flags, c->sc, c->N);
-    enqueue_full1<float>(c->stream, c->M, c->Mr, (float*)c->F, (float*)c->vp, c->flags, P, c->L,
+    enqueue_full1<float>(c->stream, c->Mr, (float*)c->F, (float*)c->vp, c->flags, P, c->L,
                          c->sc, c->N, p.flag2D, p.Ma, p.positivity);
   }
   if (ev) hipEventRecord(ev[1], c->stream);
@@ -1178,12 +1205,10 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
                        c->N);
     hipLaunchKernelGGL(k_pass2_fixup, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->M, c->L,
                        c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
-    hipLaunchKernelGGL(k_full2_stage<0>, dim3(kListBlocks), dim3(kBlock), 0, c->stream,
-                       (const double*)c->M, c->L, c->sc, c->N, p.flag2D, p.Ma);
-    hipLaunchKernelGGL(k_full2_stage<1>, dim3(kListBlocks), dim3(kBlock), 0, c->stream,
-                       (const double*)c->M, c->L, c->sc, c->N, p.flag2D, p.Ma);
-    hipLaunchKernelGGL(k_full2_stage<2>, dim3(kListBlocks), dim3(kBlock), 0, c->stream,
-                       (const double*)c->M, c->L, c->sc, c->N, p.flag2D, p.Ma);
+    hipLaunchKernelGGL(k_full2_stage<1>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
+                       c->sc, c->N, p.flag2D, p.Ma);
+    hipLaunchKernelGGL(k_full2_stage<2>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
+                       c->sc, c->N, p.flag2D, p.Ma);
     hipLaunchKernelGGL(k_full2_finish, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->M, c->L,
                        c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
   }
@@ -1439,7 +1464,7 @@ static int batch_args(const double* M, int64_t n, int32_t precision, const doubl
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 static size_t fast_batch_bytes(long long n) {
   const size_t un = (size_t)n;
-  return align256(sizeof(DevScalars)) + 6 * align256(4 * un) + align256(16 * 8 * un) +
+  return align256(sizeof(DevScalars)) + 7 * align256(4 * un) + align256(16 * 8 * un) +
          2 * align256(35 * 8 * un) + align256(6 * 8 * un);
 }
 static void fast_batch_carve(char* base, long long n, DevScalars** sc, FastLists* L) {
@@ -1455,7 +1480,8 @@ static void fast_batch_carve(char* base, long long n, DevScalars** sc, FastLists
   L->slot_A = reinterpret_cast<double*>(p); p += align256(35 * 8 * un);
   L->full_S = reinterpret_cast<double*>(p); p += align256(35 * 8 * un);
   L->full_v = reinterpret_cast<double*>(p); p += align256(6 * 8 * un);
-  L->full_fl = reinterpret_cast<uint32_t*>(p);
+  L->full_fl = reinterpret_cast<uint32_t*>(p); p += align256(4 * un);
+  L->full_known = reinterpret_cast<unsigned*>(p);
 }
 // Scalars of a batch: active, everything else zero.  A kernel rather than a memset, so that
 // the launch sequence can be captured in a graph.
@@ -1483,7 +1509,7 @@ static void enqueue_fast_batch(const double* M, long long n, double Ma, int flag
   hipLaunchKernelGGL(k_slot_solve<true>, dim3(kListBlocks), dim3(kBlock), 0, 0, L,
                      (const unsigned*)&sc->nslot1, &sc->nitem1, (const float*)nullptr, sc, n);
   hipLaunchKernelGGL(k_pass1_fixup<double>, dim3(kListBlocks), dim3(kBlock), 0, 0, L, vp, flags, sc, n);
-  enqueue_full1<double>(0, M, Mr, F, vp, flags, (const float*)nullptr, L, sc, n, flag2D, Ma,
+  enqueue_full1<double>(0, Mr, F, vp, flags, (const float*)nullptr, L, sc, n, flag2D, Ma,
                         positivity);
 }
 
@@ -1516,7 +1542,7 @@ int hyq_wavespeed_batch(const double* M, int64_t n, double Ma, int32_t flag2D,
 // (graph capture, serving loops).  Each device holds its own copy until the process ends;
 // every call on a device shares it, so calls that overlap on one device race.
 constexpr long long kNoAllocMaxCells = 1LL << 16;
-static __device__ __align__(256) char scratch[(6 * 4 + 16 * 8 + 2 * 35 * 8 + 6 * 8) * kNoAllocMaxCells + 12 * 256];
+static __device__ __align__(256) char scratch[(7 * 4 + 16 * 8 + 2 * 35 * 8 + 6 * 8) * kNoAllocMaxCells + 13 * 256];
 static std::mutex g_scratch_mu;
 static void* g_scratch_ptr[256];  // per device: the array's device address, once looked up
 

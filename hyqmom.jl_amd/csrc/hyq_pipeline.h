@@ -134,10 +134,33 @@ HD void combine_guarded(const EigRes& A, const EigRes& B, double& vmin, double& 
   }
 }
 
+// The eigenvalue correction of one axis (eigenvalues6_hyperbolic_3D.jl:35-60): the planes
+// with a complex pair (ca: UV or VU, cb: UW or VW) corrected in the central moments c of the
+// state, which returns to raw moments and is projected again into Mr.
+template <int AXIS>
+HD void axis_correct(const double* c, double M000, double um, double vm, double wm, bool ca,
+                     bool cb, int flag2D, double Ma, double* Mr, uint32_t& fl) {
+  fl |= AXIS == 0 ? F_ECX : F_ECY;
+  double cn[35];
+#pragma unroll
+  for (int t = 0; t < 35; ++t) cn[t] = c[t];
+  if (ca) correct_plane<0, 1>(c, cn);
+  if (cb) {
+    if (AXIS == 0) correct_plane<0, 2>(c, cn);
+    else correct_plane<1, 2>(c, cn);
+  }
+  central_to_raw<4>(cn, M000, um, vm, wm);
+  double Mh[35];
+  pack35(cn, Mh);
+  flux_closure35<false, double>(Mh, flag2D, Ma, Mr, nullptr, nullptr, nullptr, 0, fl);
+}
+
 // eigenvalues6_hyperbolic_3D (src/numerics/eigenvalues6_hyperbolic_3D.jl:22-155).
 // AXIS 0: UV and UW Jacobians; AXIS 1: VU and VW.  In: M.  Out: Mr (may alias nothing).
+// need_vals = false (pass 2, whose speeds are not used): after a correction the speeds of
+// the corrected state are not computed.  Returns true when the state was corrected.
 template <int AXIS, int EM = 0>
-HD void eig6_axis(const double* M, int flag2D, double Ma, double& vmin, double& vmax,
+HD bool eig6_axis(const double* M, int flag2D, double Ma, double& vmin, double& vmax,
                   double* Mr, uint32_t& fl, const Warm& wm_ = no_warm(), bool need_vals = true) {
   double c[35];
   double M000, um, vm, wm;
@@ -153,22 +176,8 @@ HD void eig6_axis(const double* M, int flag2D, double Ma, double& vmin, double& 
   }
   combine_guarded(A, B, vmin, vmax);
   if (A.cplx || B.cplx) {
-    fl |= AXIS == 0 ? F_ECX : F_ECY;
-    double cn[35];
-#pragma unroll
-    for (int t = 0; t < 35; ++t) cn[t] = c[t];
-    if (A.cplx) correct_plane<0, 1>(c, cn);
-    if (B.cplx) {
-      if (AXIS == 0) correct_plane<0, 2>(c, cn);
-      else correct_plane<1, 2>(c, cn);
-    }
-    double h[35];
-#pragma unroll
-    for (int t = 0; t < 35; ++t) h[t] = cn[t];
-    central_to_raw<4>(h, M000, um, vm, wm);
-    double Mh[35];
-    pack35(h, Mh);
-    flux_closure35<false, double>(Mh, flag2D, Ma, Mr, nullptr, nullptr, nullptr, 0, fl);
+    axis_correct<AXIS>(c, M000, um, vm, wm, A.cplx, B.cplx, flag2D, Ma, Mr, fl);
+    if (!need_vals) return true;
     raw_to_central4(Mr, c, M000, um, vm, wm);
     if (AXIS == 0) {
       A = plane_solve<EM, false>(plane<0, 1>(c, um, vm, wm), wm_.w[2 * ia], wm_.w[2 * ia + 1], need_vals, fl);
@@ -178,10 +187,11 @@ HD void eig6_axis(const double* M, int flag2D, double Ma, double& vmin, double& 
       B = plane_solve<EM, false>(plane<1, 2>(c, um, vm, wm), wm_.w[2 * ib], wm_.w[2 * ib + 1], need_vals, fl);
     }
     combine_guarded(A, B, vmin, vmax);
-  } else {
-#pragma unroll
-    for (int n = 0; n < 35; ++n) Mr[n] = M[n];
+    return true;
   }
+#pragma unroll
+  for (int n = 0; n < 35; ++n) Mr[n] = M[n];
+  return false;
 }
 
 // eigenvalues6z_hyperbolic_3D (src/numerics/eigenvalues6z_hyperbolic_3D.jl:23-365): the WU
@@ -317,8 +327,8 @@ HD void pass2_cell(const double* M, int flag2D, double Ma, bool positivity, doub
   double d0, d1;
   flux_closure35<false, double>(M, flag2D, Ma, A, nullptr, nullptr, nullptr, 0, fl);
   // the wave speeds of pass 2 are not used (only the complex-eigenvalue corrections are)
-  eig6_axis<0, EM>(A, flag2D, Ma, d0, d1, B, fl, no_warm(), EM == 0);
-  eig6_axis<1, EM>(B, flag2D, Ma, d0, d1, A, fl, no_warm(), EM == 0);
+  eig6_axis<0, EM>(A, flag2D, Ma, d0, d1, B, fl, no_warm(), false);
+  eig6_axis<1, EM>(B, flag2D, Ma, d0, d1, A, fl, no_warm(), false);
   flux_closure35<false, double>(A, flag2D, Ma, B, nullptr, nullptr, nullptr, 0, fl);
   collision_cell<MIXED>(B, dt, Kn, positivity, Mo);
 }
@@ -344,15 +354,23 @@ HD Plane cell_plane(const double* c, double um, double vm, double wm, int q) {
   }
 }
 
+// Decisions of a cell's x/y planes kUV, kUW, kVU, kVW, 2 bits per plane q at bit 2q: the
+// plane is hyperbolic, has a complex pair (the eigenvalue correction), or is not decided.
+constexpr unsigned kKnownNone = 0u, kKnownReal = 1u, kKnownComplex = 2u;
+HD unsigned known_of(unsigned known, int q) { return (known >> (2 * q)) & 3u; }
+
 // The fast step's view of a cell's planes in pass 1 (need_vals, five planes) or pass 2 (the
 // x/y checks, four planes).  Returns kPlaneFull when the cell must take the full path
-// (z density clamp or re-projection, eigenvalues6z_hyperbolic_3D.jl:27-104, or a non-finite
-// plane); otherwise the planes the fast solve proved have their (vmin, vmax) in v[2q],
-// v[2q + 1], and bit q of `pend` marks each plane that needs its own full solve.
+// (z density clamp or re-projection, eigenvalues6z_hyperbolic_3D.jl:27-104, a complex pair
+// or a non-finite plane); otherwise the planes the fast solve proved have their (vmin, vmax)
+// in v[2q], v[2q + 1], and bit q of `pend` marks each plane that needs its own full solve.
+// Pass 2 (VALS = false) decides all four planes and returns the decisions in `known`; pass 1
+// stops at the first plane that sends the cell to the full path.
 template <bool PAIR = true, bool VALS = true>
 HD int fast_cell_planes(const double* A, bool positivity, const Warm& w, double v[10],
-                        unsigned& pend, uint32_t& fl) {
+                        unsigned& pend, uint32_t& fl, unsigned* known = nullptr) {
   pend = 0;
+  unsigned kn = 0u;
   if (VALS) {
     if (positivity && !(A[0] > 0.0)) return kPlaneFull;
     if (positivity && A[0] > 1e-15) {
@@ -370,13 +388,15 @@ HD int fast_cell_planes(const double* A, bool positivity, const Warm& w, double 
 #pragma unroll
   for (int q = 0; q < (VALS ? 5 : 4); ++q) {
     v[2 * q] = v[2 * q + 1] = 0.0;
-    if (full) continue;
+    if (VALS && full) continue;
     const int st = plane_fast_status<PAIR>(cell_plane(c, um, vm, wm, q), VALS ? w.w[2 * q] : nan,
                                            VALS ? w.w[2 * q + 1] : nan, VALS, fl, v[2 * q],
                                            v[2 * q + 1]);
     if (st == kPlaneFull || st == kPlaneComplex) full = true;
     if (st == kPlaneSolve) pend |= 1u << q;
+    kn |= (st == kPlaneProven ? kKnownReal : (st == kPlaneComplex ? kKnownComplex : kKnownNone)) << (2 * q);
   }
+  if (known != nullptr) *known = kn;
   if (full) return kPlaneFull;
   return pend ? kPlaneSolve : kPlaneProven;
 }
@@ -437,17 +457,17 @@ HD void pass1_finish(const double* A, int flag2D, double Ma, double* Mr, FT* Fx,
 // kPlaneFull (the full pass 2).
 template <bool PAIR = true>
 HD int pass2_check_planes(const double* M, int flag2D, double Ma, double* A, unsigned& pend,
-                          uint32_t& fl) {
+                          uint32_t& fl, unsigned* known = nullptr) {
   flux_closure35<false, double>(M, flag2D, Ma, A, nullptr, nullptr, nullptr, 0, fl);
   double v[10];
-  return fast_cell_planes<PAIR, false>(A, false, no_warm(), v, pend, fl);
+  return fast_cell_planes<PAIR, false>(A, false, no_warm(), v, pend, fl, known);
 }
 // The full decision of the planes in pend (fast solve with the pair test, else QR) of a
 // cell whose other planes the small fast kernel already proved: v[2q], v[2q+1] set for
 // each such plane; returns false when one of them has a complex pair (or is not finite),
 // i.e. the cell needs the eigenvalue correction's full path.
 HD bool solve_pending_planes(const double* A, unsigned pend, bool need_vals, const Warm& w,
-                             double v[10], uint32_t& fl) {
+                             double v[10], uint32_t& fl, unsigned* known = nullptr) {
   double c[35], M000, um, vm, wm;
   raw_to_central4(A, c, M000, um, vm, wm);
   const float nan = NAN;
@@ -458,15 +478,45 @@ HD bool solve_pending_planes(const double* A, unsigned pend, bool need_vals, con
     const int st = plane_fast_status<true>(p, need_vals ? w.w[2 * q] : nan,
                                            need_vals ? w.w[2 * q + 1] : nan, need_vals, fl,
                                            v[2 * q], v[2 * q + 1]);
-    if (st == kPlaneFull || st == kPlaneComplex) return false;
+    bool cplx = st == kPlaneComplex;
+    if (st == kPlaneFull) return false;
     if (st == kPlaneSolve) {
       fl |= F_FALLBACK;
       const EigRes r = plane_eigs(p, fl);
-      if (r.cplx || !r.ok) return false;
+      if (!r.ok) return false;
+      cplx = r.cplx;
       v[2 * q] = r.vmin;
       v[2 * q + 1] = r.vmax;
     }
+    if (known != nullptr) *known |= (cplx ? kKnownComplex : kKnownReal) << (2 * q);
+    if (cplx) return false;
   }
+  return true;
+}
+
+// Pass 2's x or y check of a full-path cell whose two planes (UV, UW or VU, VW) were already
+// decided on this state (known, as fast_cell_planes returns it): the correction of the planes
+// with a complex pair, without solving the planes again, or a copy when there is none; when
+// a decision is missing, eig6_axis.  The same result as eig6_axis with need_vals = false:
+// the decisions came from the same solves on the same state.  Returns true when the state
+// was corrected.
+template <int AXIS>
+HD bool eig6_axis_decided(const double* M, int flag2D, double Ma, double* Mr, uint32_t& fl,
+                          unsigned known) {
+  const unsigned sa = known_of(known, AXIS == 0 ? kUV : kVU);
+  const unsigned sb = known_of(known, AXIS == 0 ? kUW : kVW);
+  if (sa == kKnownNone || sb == kKnownNone) {
+    double d0, d1;
+    return eig6_axis<AXIS, 1>(M, flag2D, Ma, d0, d1, Mr, fl, no_warm(), false);
+  }
+  if (sa == kKnownReal && sb == kKnownReal) {
+#pragma unroll
+    for (int n = 0; n < 35; ++n) Mr[n] = M[n];
+    return false;
+  }
+  double c[35], M000, um, vm, wm;
+  raw_to_central4(M, c, M000, um, vm, wm);
+  axis_correct<AXIS>(c, M000, um, vm, wm, sa == kKnownComplex, sb == kKnownComplex, flag2D, Ma, Mr, fl);
   return true;
 }
 HD bool pass2_check(const double* M, int flag2D, double Ma, double* A, uint32_t& fl) {
