@@ -49,6 +49,23 @@ struct ClosureState {
   double C200, C020, C002, M000, um, vm, wm;
 };
 
+// M2CS4_35 (src/moments/M2CS4_35.jl:47-49) standardization of the central moments c
+// (packed, order <= 4) with sx, sy, sz = sqrt(max(C, eps)) of the three variances: the 28
+// standardized moments the closure uses.
+HD S28 standardize28(const double* c, double sx, double sy, double sz) {
+  S28 s;
+#define SD(i, j, k) (c[tpos(i, j, k)] / (pw<i>(sx) * pw<j>(sy) * pw<k>(sz)))
+  s.S300 = SD(3, 0, 0); s.S400 = SD(4, 0, 0); s.S110 = SD(1, 1, 0); s.S210 = SD(2, 1, 0);
+  s.S310 = SD(3, 1, 0); s.S120 = SD(1, 2, 0); s.S220 = SD(2, 2, 0); s.S030 = SD(0, 3, 0);
+  s.S130 = SD(1, 3, 0); s.S040 = SD(0, 4, 0); s.S101 = SD(1, 0, 1); s.S201 = SD(2, 0, 1);
+  s.S301 = SD(3, 0, 1); s.S102 = SD(1, 0, 2); s.S202 = SD(2, 0, 2); s.S003 = SD(0, 0, 3);
+  s.S103 = SD(1, 0, 3); s.S004 = SD(0, 0, 4); s.S011 = SD(0, 1, 1); s.S111 = SD(1, 1, 1);
+  s.S211 = SD(2, 1, 1); s.S021 = SD(0, 2, 1); s.S121 = SD(1, 2, 1); s.S031 = SD(0, 3, 1);
+  s.S012 = SD(0, 1, 2); s.S112 = SD(1, 1, 2); s.S013 = SD(0, 1, 3); s.S022 = SD(0, 2, 2);
+#undef SD
+  return s;
+}
+
 // Flux_closure35_and_realizable_3D (src/numerics/Flux_closure35_and_realizable_3D.jl:33-234),
 // first half: standardization and the realizability projection (:33-176).
 HD void closure_project(const double* M4, int flag2D, double Ma, ClosureState& q, uint32_t& fl) {
@@ -58,23 +75,9 @@ HD void closure_project(const double* M4, int flag2D, double Ma, ClosureState& q
   raw_to_central4(M4, c, M000, um, vm, wm);
   const double C200 = jmax(EPS, c[tpos(2, 0, 0)]), C020 = jmax(EPS, c[tpos(0, 2, 0)]),
                C002 = jmax(EPS, c[tpos(0, 0, 2)]);
-  // M2CS4_35.jl:47-49 standardization (max(C, eps) under the square root)
   const double sx = sqrt(jmax(c[tpos(2, 0, 0)], EPS)), sy = sqrt(jmax(c[tpos(0, 2, 0)], EPS)),
                sz = sqrt(jmax(c[tpos(0, 0, 2)], EPS));
-  S28 s;
-  {
-    const double ix = 1.0 / sx, iy = 1.0 / sy, iz = 1.0 / sz;
-    (void)ix; (void)iy; (void)iz;
-#define SD(i, j, k) (c[tpos(i, j, k)] / (pw<i>(sx) * pw<j>(sy) * pw<k>(sz)))
-    s.S300 = SD(3, 0, 0); s.S400 = SD(4, 0, 0); s.S110 = SD(1, 1, 0); s.S210 = SD(2, 1, 0);
-    s.S310 = SD(3, 1, 0); s.S120 = SD(1, 2, 0); s.S220 = SD(2, 2, 0); s.S030 = SD(0, 3, 0);
-    s.S130 = SD(1, 3, 0); s.S040 = SD(0, 4, 0); s.S101 = SD(1, 0, 1); s.S201 = SD(2, 0, 1);
-    s.S301 = SD(3, 0, 1); s.S102 = SD(1, 0, 2); s.S202 = SD(2, 0, 2); s.S003 = SD(0, 0, 3);
-    s.S103 = SD(1, 0, 3); s.S004 = SD(0, 0, 4); s.S011 = SD(0, 1, 1); s.S111 = SD(1, 1, 1);
-    s.S211 = SD(2, 1, 1); s.S021 = SD(0, 2, 1); s.S121 = SD(1, 2, 1); s.S031 = SD(0, 3, 1);
-    s.S012 = SD(0, 1, 2); s.S112 = SD(1, 1, 2); s.S013 = SD(0, 1, 3); s.S022 = SD(0, 2, 2);
-#undef SD
-  }
+  S28 s = standardize28(c, sx, sy, sz);
   double H200, H020, H002;
   {
     const double o[6] = {s.S300, s.S400, s.S030, s.S040, s.S003, s.S004};
@@ -124,13 +127,9 @@ HD void closure_project(const double* M4, int flag2D, double Ma, ClosureState& q
 
 // Second half: the 5th-order closure, S -> C (S_to_C_batch, moment_array_utils.jl:137-204),
 // C -> M.  With FLUX, Fx/Fy/Fz (35 each) are written through the pointers with stride `st`.
-template <bool FLUX, typename FT = double>
-HD void closure_emit(const ClosureState& q, double* Mr, FT* Fx, FT* Fy, FT* Fz, size_t st) {
-  const S28& s = q.s;
-  const double C200 = q.C200, C020 = q.C020, C002 = q.C002;
-  double M000 = q.M000, um = q.um, vm = q.vm, wm = q.wm;
-  double c[56];
-  s28_to_packed(s, c);
+// hyqmom_3D (src/autogen/hyqmom_3D.jl): the 21 standardized 5th-order moments of the
+// closure from the 28 standardized moments s, into the packed array c.
+HD void hyqmom5_std(const S28& s, double* c) {
   c[tpos(5, 0, 0)] = f500(s.S300, s.S400);
   c[tpos(0, 5, 0)] = f500(s.S030, s.S040);
   c[tpos(0, 0, 5)] = f500(s.S003, s.S004);
@@ -152,6 +151,16 @@ HD void closure_emit(const ClosureState& q, double* Mr, FT* Fx, FT* Fy, FT* Fz, 
   c[tpos(2, 2, 1)] = f221(s.S300, s.S030, s.S121, s.S211, s.S101, s.S011, s.S201, s.S021, s.S111);
   c[tpos(2, 1, 2)] = f221(s.S300, s.S003, s.S112, s.S211, s.S110, s.S011, s.S210, s.S012, s.S111);
   c[tpos(1, 2, 2)] = f221(s.S003, s.S030, s.S121, s.S112, s.S101, s.S110, s.S102, s.S120, s.S111);
+}
+
+template <bool FLUX, typename FT = double>
+HD void closure_emit(const ClosureState& q, double* Mr, FT* Fx, FT* Fy, FT* Fz, size_t st) {
+  const S28& s = q.s;
+  const double C200 = q.C200, C020 = q.C020, C002 = q.C002;
+  double M000 = q.M000, um = q.um, vm = q.vm, wm = q.wm;
+  double c[56];
+  s28_to_packed(s, c);
+  hyqmom5_std(s, c);
   const double qx = sqrt(jmax(EPS, C200)), qy = sqrt(jmax(EPS, C020)), qz = sqrt(jmax(EPS, C002));
   double px[6], py[6], pz[6];
   px[0] = py[0] = pz[0] = 1.0;
@@ -190,6 +199,43 @@ HD void flux_closure35(const double* M4, int flag2D, double Ma, double* Mr, FT* 
   ClosureState q;
   closure_project(M4, flag2D, Ma, q, fl);
   closure_emit<FLUX, FT>(q, Mr, Fx, Fy, Fz, st);
+}
+
+// Moments5_3D (src/moments/Moments5_3D.jl:16-138): the HyQMOM closure of the 5th-order
+// moments of M4 without any realizability projection.  S5: hyqmom_3D of M4's standardized
+// moments; C5 = S5 sx^i sy^j sz^k; M5: raw moments of order 5 from C5 and M4's own central
+// moments (C5toM5_3D).  Outputs in the reference's order 500 410 320 230 140 401 302 203 104
+// 311 221 131 212 113 122 050 041 032 023 014 005, each with stride st.
+HD double ipow5(double x, int n) {  // x^n, 0 <= n <= 5
+  double r = 1.0;
+#pragma unroll
+  for (int t = 0; t < 5; ++t)
+    if (t < n) r *= x;
+  return r;
+}
+HD void moments5_cell(const double* M4, double* M5, double* C5, double* S5, size_t st) {
+  constexpr int E21[21][3] = {{5, 0, 0}, {4, 1, 0}, {3, 2, 0}, {2, 3, 0}, {1, 4, 0}, {4, 0, 1},
+                              {3, 0, 2}, {2, 0, 3}, {1, 0, 4}, {3, 1, 1}, {2, 2, 1}, {1, 3, 1},
+                              {2, 1, 2}, {1, 1, 3}, {1, 2, 2}, {0, 5, 0}, {0, 4, 1}, {0, 3, 2},
+                              {0, 2, 3}, {0, 1, 4}, {0, 0, 5}};
+  double c[56];
+  double M000, um, vm, wm;
+  raw_to_central4(M4, c, M000, um, vm, wm);
+  const double sx = sqrt(jmax(c[tpos(2, 0, 0)], EPS)), sy = sqrt(jmax(c[tpos(0, 2, 0)], EPS)),
+               sz = sqrt(jmax(c[tpos(0, 0, 2)], EPS));
+  hyqmom5_std(standardize28(c, sx, sy, sz), c);
+#pragma unroll
+  for (int n = 0; n < 21; ++n) {
+    const int i = E21[n][0], j = E21[n][1], k = E21[n][2];
+    const double sv = c[tpos(i, j, k)];
+    const double cv = sv * (ipow5(sx, i) * ipow5(sy, j) * ipow5(sz, k));
+    S5[n * st] = sv;
+    C5[n * st] = cv;
+    c[tpos(i, j, k)] = cv;
+  }
+  central_to_raw<5>(c, M000, um, vm, wm);
+#pragma unroll
+  for (int n = 0; n < 21; ++n) M5[n * st] = c[tpos(E21[n][0], E21[n][1], E21[n][2])];
 }
 
 }  // namespace hyq

@@ -195,6 +195,19 @@ __global__ __launch_bounds__(kBlock) void k_closure_batch(const double* __restri
   if (flags) flags[c] = fl;
 }
 
+// Moments5_3D of n independent cells: M [35][n] in, M5, C5, S5 [21][n] out.
+__global__ __launch_bounds__(kBlock) void k_moments5_batch(const double* __restrict__ M,
+                                                           long long N, double* __restrict__ M5,
+                                                           double* __restrict__ C5,
+                                                           double* __restrict__ S5) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  double m[35];
+#pragma unroll
+  for (int n = 0; n < 35; ++n) m[n] = M[(size_t)n * N + c];
+  moments5_cell(m, M5 + c, C5 + c, S5 + c, (size_t)N);
+}
+
 __global__ __launch_bounds__(kBlock) void k_collision_batch(const double* __restrict__ M,
                                                             long long N, double dt, double Kn,
                                                             int positivity, double* Mo) {
@@ -1441,6 +1454,16 @@ int hyq_closure_batch(const double* M, int64_t n, double Ma, int32_t flag2D, dou
                      flag2D, Mr, Fx, Fy, Fz, flags);
   HIPCHK(hipGetLastError(), "closure launch");
   HIPCHK(hipDeviceSynchronize(), "closure sync");
+  return HYQ_OK;
+}
+
+int hyq_moments5_batch(const double* M, int64_t n, double* M5, double* C5, double* S5) {
+  if (!M || !M5 || !C5 || !S5 || n < 0) return fail(HYQ_EINVAL, "bad argument");
+  if (n == 0) return HYQ_OK;
+  hipLaunchKernelGGL(k_moments5_batch, dim3(grid_for(n)), dim3(kBlock), 0, 0, M, (long long)n, M5,
+                     C5, S5);
+  HIPCHK(hipGetLastError(), "moments5 launch");
+  HIPCHK(hipDeviceSynchronize(), "moments5 sync");
   return HYQ_OK;
 }
 

@@ -62,6 +62,7 @@ SIGNATURES = {
     "hyq_symmetry_maxdiff": ([VP, D, D], ctypes.c_int),
     "hyq_vmax_breakdown": ([VP, D, ctypes.POINTER(I64)], ctypes.c_int),
     "hyq_closure_batch": ([VP, I64, ctypes.c_double, ctypes.c_int32, VP, VP, VP, VP, VP], ctypes.c_int),
+    "hyq_moments5_batch": ([VP, I64, VP, VP, VP], ctypes.c_int),
     "hyq_wavespeed_batch": ([VP, I64, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                              VP, VP, VP, VP, VP, VP], ctypes.c_int),
     "hyq_wavespeed_batch_noalloc": ([VP, I64, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,

@@ -161,6 +161,12 @@ int hyq_vmax_breakdown(hyq_ctx* ctx, double* vdir, int64_t* worst);
 int hyq_closure_batch(const double* M, int64_t n, double Ma, int32_t flag2D, double* Mr,
                       double* Fx, double* Fy, double* Fz, uint32_t* flags);
 
+/* Moments5_3D (src/moments/Moments5_3D.jl:16-138) for n independent cells: the HyQMOM
+ * closure of the 5th-order moments without realizability projection.  M is [35][n]; M5, C5
+ * and S5 are [21][n], rows in the reference's order 500 410 320 230 140 401 302 203 104 311
+ * 221 131 212 113 122 050 041 032 023 014 005. */
+int hyq_moments5_batch(const double* M, int64_t n, double* M5, double* C5, double* S5);
+
 /* The pass-1 cell pipeline of simulation_runner.jl:352-380 for n independent cells:
  * closure -> x/y/z wave speeds (eigenvalues6_hyperbolic_3D.jl, eigenvalues6z_hyperbolic_3D.jl)
  * -> closure -> 1D closures (closure_and_eigenvalues.jl).  vp is [6][n]; Fx, Fy, Fz must be

@@ -61,9 +61,9 @@ extern "C" void cell_delta2star(const double* in28, double* E21) {
 }
 // Host emulation of one device step (k_pass1, k_dt, k_update, k_pass2) on the SoA state
 // M[35][N]; returns dt.  Same cell functions and line logic as the kernels.
-extern "C" double host_step(double* M, int Nx, int Ny, int Nz, double Kn, double Ma, int flag2D,
-                            double CFL, double dtmax, double dx, double dy, double dz,
-                            double tmax, double t, uint32_t* flags) {
+static double host_step_impl(double* M, int Nx, int Ny, int Nz, double Kn, double Ma, int flag2D,
+                             double CFL, double dtmax, double dx, double dy, double dz,
+                             double tmax, double t, uint32_t* flags, bool skip_pass2) {
   const long long N = (long long)Nx * Ny * Nz;
   std::vector<double> Mr(35 * N), F(105 * N), vp(6 * N);
   double vmax = 0.0;
@@ -89,6 +89,7 @@ extern "C" double host_step(double* M, int Nx, int Ny, int Nz, double Kn, double
     update_cell<double>(Mr.data(), F.data(), vp.data(), M, c, i, j, k, Nx, Ny, Nz, N, dt / dx,
                         dt / dy, dt / dz);
   }
+  if (skip_pass2) return dt;
   for (long long c = 0; c < N; ++c) {
     double m[35], o[35];
     for (int n = 0; n < 35; ++n) m[n] = M[n * N + c];
@@ -97,6 +98,17 @@ extern "C" double host_step(double* M, int Nx, int Ny, int Nz, double Kn, double
     for (int n = 0; n < 35; ++n) M[n * N + c] = o[n];
   }
   return dt;
+}
+extern "C" double host_step(double* M, int Nx, int Ny, int Nz, double Kn, double Ma, int flag2D,
+                            double CFL, double dtmax, double dx, double dy, double dz,
+                            double tmax, double t, uint32_t* flags) {
+  return host_step_impl(M, Nx, Ny, Nz, Kn, Ma, flag2D, CFL, dtmax, dx, dy, dz, tmax, t, flags, false);
+}
+// The same step without pass 2: the state after the update, as pass 2 receives it.
+extern "C" double host_step_to_update(double* M, int Nx, int Ny, int Nz, double Kn, double Ma,
+                                      int flag2D, double CFL, double dtmax, double dx, double dy,
+                                      double dz, double tmax, double t) {
+  return host_step_impl(M, Nx, Ny, Nz, Kn, Ma, flag2D, CFL, dtmax, dx, dy, dz, tmax, t, nullptr, true);
 }
 extern "C" void cell_pass2_flags(const double* M, int flag2D, double Ma, double dt, double Kn,
                                  double* Mo, uint32_t* fl_out) {

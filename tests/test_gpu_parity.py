@@ -1,9 +1,10 @@
 """GPU parity: libhyqmom_hip.so (through the C ABI) against the CPU oracle.
 
-FP64 bar: on cells whose branch flags match, every output agrees with the oracle to
-1e-12 relative to the field's magnitude (the per-moment maximum over the batch), except
-where the reference computation itself is ill-conditioned (flagged flips are counted and
-bounded separately)."""
+FP64 bar (tests/parity_util.py): every output agrees with the oracle to 1e-12 relative to
+the field's magnitude (the per-component maximum over the batch), or to 16 x the oracle's
+own spread under one-ulp input perturbations where that is larger; cells where that spread
+exceeds 1e-8 (a branch of the reference decided at rounding level) are counted and bounded,
+never exempted by the device/oracle difference itself."""
 import ctypes
 import os
 import sys
@@ -12,6 +13,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+import parity_util as PU
 from _common import gaussian_raw, mixture_raw, to35
 
 pytestmark = pytest.mark.gpu
@@ -42,48 +44,55 @@ def scaled_err(a, b):
     return (np.abs(a - b) / scale).max()
 
 
-def test_closure_batch():
+def _closure_dev(M, flags=False):
     from gpu_util import empty_dev, to_host
     L = _lib()
-    M = cells(256, 1)
     n = len(M)
-    Md = soa_dev(M)
     Mr, Fx, Fy, Fz = (empty_dev((35, n)) for _ in range(4))
     fl = empty_dev((n,), "int32")
-    L.check(L.lib().hyq_closure_batch(Md.data_ptr(), n, 1.5, 0, Mr.data_ptr(), Fx.data_ptr(), Fy.data_ptr(),
-                                      Fz.data_ptr(), fl.data_ptr()), "closure")
-    g = [to_host(x).T for x in (Fx, Fy, Fz, Mr)]
-    gf = to_host(fl).view(np.uint32)
-    ref = [O.flux_closure35(M[c], 0, 1.5) for c in range(n)]
-    of = np.array([r[4] for r in ref], dtype=np.uint32)
-    match = gf == of
-    assert match.mean() > 0.98, (match.mean(), gf[~match], of[~match])
-    for k in range(4):
-        o = np.array([r[k] for r in ref])
-        e = scaled_err(g[k][match], o[match])
-        assert e < RTOL * 10, (k, e)
+    L.check(L.lib().hyq_closure_batch(soa_dev(M).data_ptr(), n, 1.5, 0, Mr.data_ptr(), Fx.data_ptr(),
+                                      Fy.data_ptr(), Fz.data_ptr(), fl.data_ptr()), "closure")
+    g = np.hstack([to_host(x).T for x in (Fx, Fy, Fz, Mr)])
+    return (g, to_host(fl).view(np.uint32)) if flags else g
+
+
+def test_closure_batch():
+    M = cells(256, 1)
+    g, gf = _closure_dev(M, flags=True)
+
+    def ora(X):
+        return np.array([np.concatenate(O.flux_closure35(m, 0, 1.5)[:4]) for m in X])
+
+    o = ora(M)
+    of = np.array([O.flux_closure35(m, 0, 1.5)[4] for m in M], dtype=np.uint32)
+    print("flag match %.4f" % (gf == of).mean())
+    PU.check(g, o, PU.sensitivity2(ora, _closure_dev, M, o, g, np.random.default_rng(11)),
+             label="closure batch")
+
+
+def pass1_dev(M, positivity=1, precision=0):
+    """hyq_wavespeed_batch on cells M [n][35]: [n][35 + 105 + 6] (Mr, Fx, Fy, Fz, vp)."""
+    from gpu_util import empty_dev, to_dev, to_host
+    L = _lib()
+    n = len(M)
+    Mr = empty_dev((35, n)); F = empty_dev((105, n)); vp = empty_dev((6, n)); fl = empty_dev((n,), "int32")
+    p = F.data_ptr()
+    L.check(L.lib().hyq_wavespeed_batch(to_dev(np.ascontiguousarray(np.asarray(M).T)).data_ptr(), n, 1.5, 0,
+                                        positivity, precision, Mr.data_ptr(), p, p + 35 * n * 8,
+                                        p + 70 * n * 8, vp.data_ptr(), fl.data_ptr()), "wavespeed")
+    return np.hstack([to_host(Mr).T, to_host(F).T, to_host(vp).T])
+
+
+def pass1_ora(M, positivity=True):
+    Mr_, Fx_, Fy_, Fz_, vp_, _ = O.pass1_batch(M, 0, 1.5, positivity)
+    return np.hstack([Mr_, Fx_, Fy_, Fz_, vp_])
 
 
 def test_pass1_batch():
-    from gpu_util import empty_dev, to_host
-    L = _lib()
     M = cells(200, 2)
-    n = len(M)
-    Md = soa_dev(M)
-    Mr = empty_dev((35, n)); F = empty_dev((105, n)); vp = empty_dev((6, n)); fl = empty_dev((n,), "int32")
-    p = F.data_ptr()
-    L.check(L.lib().hyq_wavespeed_batch(Md.data_ptr(), n, 1.5, 0, 1, 0, Mr.data_ptr(), p, p + 35 * n * 8,
-                                        p + 70 * n * 8, vp.data_ptr(), fl.data_ptr()), "wavespeed")
-    gMr, gF, gvp = to_host(Mr).T, to_host(F), to_host(vp).T
-    gf = to_host(fl).view(np.uint32)
-    oMr, oFx, oFy, oFz, ovp, ofl = O.pass1_batch(M, 0, 1.5, True)
-    match = gf == ofl
-    assert match.mean() > 0.97, match.mean()
-    assert scaled_err(gMr[match], oMr[match]) < RTOL * 10
-    for k, o in enumerate((oFx, oFy, oFz)):
-        assert scaled_err(gF[35 * k:35 * (k + 1)].T[match], o[match]) < RTOL * 10
-    # wave speeds: absolute error relative to the speed scale
-    assert np.abs(gvp[match] - ovp[match]).max() / np.abs(ovp).max() < 1e-10
+    g, o = pass1_dev(M), pass1_ora(M)
+    PU.check(g, o, PU.sensitivity2(pass1_ora, pass1_dev, M, o, g, np.random.default_rng(12)),
+             label="pass 1 batch")
 
 
 def test_eig6_batch_against_lapack():
@@ -92,14 +101,20 @@ def test_eig6_batch_against_lapack():
     M = cells(128, 3)
     sl = [1, 6, 10, 13, 15, 2, 7, 11, 14, 3, 8, 12, 4, 9, 5]
     m15 = np.array([[m[i - 1] for i in sl] for m in M])
-    n = len(m15)
-    d = to_dev(np.ascontiguousarray(m15.T))
-    wr = empty_dev((6, n)); wi = empty_dev((6, n))
-    L.check(L.lib().hyq_eig6_batch(d.data_ptr(), n, wr.data_ptr(), wi.data_ptr()), "eig6")
-    g = to_host(wr).T + 1j * to_host(wi).T
-    for c in range(n):
-        ref = np.linalg.eigvals(O.jacobian6(m15[c]))
-        np.testing.assert_allclose(np.sort(g[c].real), np.sort(ref.real), atol=1e-7 * max(1, np.abs(ref).max()))
+
+    def dev(X):
+        d_ = to_dev(np.ascontiguousarray(X.T))
+        wr_ = empty_dev((6, len(X))); wi_ = empty_dev((6, len(X)))
+        L.check(L.lib().hyq_eig6_batch(d_.data_ptr(), len(X), wr_.data_ptr(), wi_.data_ptr()), "eig6")
+        return np.sort(to_host(wr_).T, axis=1)
+
+    def lapack(X):
+        return np.array([np.sort(np.linalg.eigvals(O.jacobian6(m)).real) for m in X])
+
+    g, o = dev(m15), lapack(m15)
+    # the real parts, sorted; near a double eigenvalue LAPACK's own result moves by
+    # ~sqrt(eps) under one-ulp input changes, and the bar follows that spread
+    PU.check(g, o, PU.sensitivity2(lapack, dev, m15, o, g, np.random.default_rng(13)), label="eig6 batch")
 
 
 def test_collision_batch():
@@ -189,16 +204,36 @@ def _params(N, Ma, Kn, nnmax, tmax, precision="fp64"):
                 use_custom_ic=True, ic_background=bg, ic_jets=jets, precision=precision)
 
 
+def device_steps(p, M, steps, t0=0.0, nn0=0):
+    """The state after `steps` device steps from M (params p), as [cells][35]."""
+    import hyqmom_amd as H
+    s = H.Solver(p)
+    s.set_state(np.asfortranarray(M))
+    s.t, s.nn = t0, nn0
+    for _ in range(steps):
+        s.step()
+    out = s.get_state().reshape(-1, 35, order="F")
+    s.close()
+    return out
+
+
 def test_step_config1():
-    """Config 1 (20x20x10, Ma=1, Kn=1, tmax=0.01): one step, whole state."""
+    """Config 1 (20x20x10, Ma=1, Kn=1, tmax=0.01): the whole run, whole state."""
     import hyqmom_amd as H
     p = _params((20, 20, 10), 1.0, 1.0, 100000, 0.01)
-    M0 = H.initial_state(p)
+    M0 = np.asfortranarray(H.initial_state(p))
     Mg, t, nn, _ = H.simulation_runner(p)
+
+    def ora(X):
+        Mo_, to_, nno_, _, rc_ = O.run(X, 20, 20, 10, 1.0, 1.0, 0, 0.7, 1e-2, 0.05, 0.05, 0.1, True, 0.01, 100000)
+        assert rc_ == 0 and nno_ == nn
+        return Mo_.reshape(-1, 35, order="F")
+
     Mo, to, nno, _, rc = O.run(M0, 20, 20, 10, 1.0, 1.0, 0, 0.7, 1e-2, 0.05, 0.05, 0.1, True, 0.01, 100000)
     assert rc == 0 and nn == nno and t == to
-    scale = np.abs(Mo).max(axis=(0, 1, 2))
-    assert (np.abs(Mg - Mo).max(axis=(0, 1, 2)) / scale).max() < 1e-11
+    o, g = Mo.reshape(-1, 35, order="F"), Mg.reshape(-1, 35, order="F")
+    sens = PU.sensitivity2(ora, lambda X: device_steps(p, X, nn), M0, o, g, np.random.default_rng(14))
+    PU.check(g, o, sens, max_chaotic=0.0, label="config 1, %d steps" % nn)
 
 
 @pytest.mark.parametrize("N,Ma,Kn,steps", [((20, 20, 10), 1.0, 1.0, 8), ((24, 24, 12), 1.5, 0.01, 5)])
@@ -209,9 +244,9 @@ def test_multistep_small_grid(N, Ma, Kn, steps):
     Whole trajectories are not comparable cell by cell: the step is chaotic at
     realizability-boundary cells (a clamped marginal, H = 1e-8), where a 2e-16 relative
     perturbation of the input makes the oracle diverge from itself by O(1e-2) within a few
-    steps.  From identical inputs, cells whose result differs by more than 1e-10 (scaled
-    by the moment's field maximum) are such knife-edge cells; they are counted and bounded
-    (<= 1% of cells), and every other cell meets the FP64 bar."""
+    steps.  From identical inputs, each step meets the FP64 bar of parity_util, with the
+    cells where the oracle's own step moves by more than 1e-8 under one-ulp input changes
+    counted and bounded (<= 1% of cells)."""
     import hyqmom_amd as H
     p = _params(N, Ma, Kn, steps, 10.0)
     A = np.asfortranarray(H.initial_state(p))
@@ -225,15 +260,64 @@ def test_multistep_small_grid(N, Ma, Kn, steps):
         s.t, s.nn = t, it
         s.step()
         Mg = s.get_state()
+        def ora(X, t=t):
+            Y, _, _, _, rc_ = O.run(X, *N, Kn, Ma, 0, 0.7, 1e-2, 1 / N[0], 1 / N[1], 1 / N[2], True, 10.0, 1, t0=t)
+            assert rc_ == 0
+            return Y.reshape(-1, 35, order="F")
+
+        A0 = A
         A, t2, _, dts, rc = O.run(A, *N, Kn, Ma, 0, 0.7, 1e-2, 1 / N[0], 1 / N[1], 1 / N[2], True, 10.0, 1, t0=t)
         assert rc == 0
         assert abs(s.t - t2) <= 1e-13 * t2, (s.t, t2)
+        o, g = A.reshape(-1, 35, order="F"), Mg.reshape(-1, 35, order="F")
+        sens = PU.sensitivity2(ora, lambda X, t=t, it=it: device_steps(p, X, 1, t, it), np.asfortranarray(A0),
+                               o, g, np.random.default_rng(100 + it))
         t = t2
-        scale = np.abs(A).max(axis=(0, 1, 2))
-        e = (np.abs(Mg - A) / scale).max(axis=3)
-        bad = e > 1e-10
-        print("step %d: knife-edge cells %d/%d (max %.2e), others max %.2e, L2 rho %.2e E %.2e"
-              % (it, bad.sum(), ncell, e.max(), e[~bad].max(), l2(Mg[..., 0], A[..., 0]), l2(E(Mg), E(A))))
-        assert bad.sum() <= 0.01 * ncell
-        assert e[~bad].max() < 1e-10
+        PU.check(g, o, sens, label="step %d" % it)
+        print("  L2 rho %.2e E %.2e" % (l2(Mg[..., 0], A[..., 0]), l2(E(Mg), E(A))))
     s.close()
+
+
+E21 = [(5, 0, 0), (4, 1, 0), (3, 2, 0), (2, 3, 0), (1, 4, 0), (4, 0, 1), (3, 0, 2), (2, 0, 3),
+       (1, 0, 4), (3, 1, 1), (2, 2, 1), (1, 3, 1), (2, 1, 2), (1, 1, 3), (1, 2, 2), (0, 5, 0),
+       (0, 4, 1), (0, 3, 2), (0, 2, 3), (0, 1, 4), (0, 0, 5)]
+
+
+def _moments5_dev(M):
+    from gpu_util import empty_dev, to_host
+    L = _lib()
+    n = len(M)
+    M5, C5, S5 = (empty_dev((21, n)) for _ in range(3))
+    L.check(L.lib().hyq_moments5_batch(soa_dev(M).data_ptr(), n, M5.data_ptr(), C5.data_ptr(),
+                                       S5.data_ptr()), "moments5")
+    return [to_host(x).T for x in (M5, C5, S5)]
+
+
+def test_moments5_batch():
+    """Moments5_3D (Moments5_3D.jl:16-138) on the device: mixture cells against the oracle at
+    the FP64 bar; the reference's golden Gaussian case (test_closures_golden); and the closed
+    form of Gaussians, whose standardized and central 5th-order moments vanish and whose raw
+    5th-order moments are the Gaussian's own (Isserlis)."""
+    from _common import load_json
+    M = cells(200, 41)
+
+    def dev(X):
+        return np.hstack(_moments5_dev(X))
+
+    def ora(X):
+        return np.array([np.concatenate(O.moments5_3d(m)) for m in X])
+
+    g, o = dev(M), ora(M)
+    PU.check(g, o, PU.sensitivity2(ora, dev, M, o, g, np.random.default_rng(42)), label="moments5 batch")
+    gold = load_json("mat_goldens.json")["test_closures_golden"]["gaussian"]
+    dM5, dC5, dS5 = (x[0] for x in _moments5_dev(np.array([gold["input"]])))
+    for got, key in ((dM5, "M5"), (dC5, "C5"), (dS5, "S5")):
+        want = np.asarray(gold["output"][key], dtype=float)
+        assert np.abs(got - want).max() <= 1e-12 * max(1.0, np.abs(want).max()), (key, got, want)
+    for rho, u, C in [(0.7, (0.3, -0.2, 0.1), [[1.2, 0.3, 0.2], [0.3, 1.5, 0.4], [0.2, 0.4, 1.8]]),
+                      (1.0, (0.0, 0.0, 0.0), np.eye(3))]:
+        raw = gaussian_raw(rho, u, np.asarray(C, float), 5)
+        dM5, dC5, dS5 = (x[0] for x in _moments5_dev(np.array([to35(raw)])))
+        want = np.array([raw[e] for e in E21])
+        assert np.abs(dS5).max() < 1e-12 and np.abs(dC5).max() < 1e-12
+        assert np.abs(dM5 - want).max() <= 1e-12 * np.abs(want).max(), (dM5, want)
