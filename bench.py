@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--compare-fp64", type=int, default=1)
     ap.add_argument("--compare-newton", type=int, default=1,
                     help="with mixed_learned: also time the same step without the network")
+    ap.add_argument("--compare-h128", type=int, default=1,
+                    help="with mixed_learned: also time the step with the width-128 network")
     ap.add_argument("--count-flops", type=int, default=1)
     ap.add_argument("--accuracy", type=int, default=1)
     ap.add_argument("--cpu-baseline", type=int, default=1)
@@ -404,6 +406,17 @@ def main():
                                "initial_state": wstart}
         if "fp64" in out:
             out["mixed_newton"]["speedup_vs_fp64"] = wv / out["fp64"]["value"]
+    if a.compare_h128 and a.precision == "mixed_learned" and a.model is None:
+        from hyqmom_amd.warmstart import MODEL_H128
+        w, _, wstart = surviving_run(N3, "mixed_learned", M0, a.steps, a.warmup, MODEL_H128)
+        wv = N * a.steps / w["seconds"]
+        out["mixed_learned_h128"] = {"value": wv, "ms_per_step": w["seconds"] * 1e3 / a.steps,
+                                     "network": "15-128-128-128-2",
+                                     "kernels": {k: w[k] for k in ("pass1_ms", "update_ms", "pass2_ms")},
+                                     "fast_wave_speeds_last_step": w["last_step"],
+                                     "initial_state": wstart}
+        if "fp64" in out:
+            out["mixed_learned_h128"]["speedup_vs_fp64"] = wv / out["fp64"]["value"]
     if a.accuracy and a.precision != "fp64":
         out["accuracy"] = dict(accuracy(N3, a.precision, Mmid, a.model), from_step=mid)
     if a.cpu_baseline:

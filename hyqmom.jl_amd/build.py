@@ -20,8 +20,13 @@ FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC",
          # cells, ensemble), which keeps those paths bitwise identical
          "-ffp-contract=on"]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-Wno-unknown-pragmas"]
-SOURCES = ["hyq_kernels.hip", "hyq_ensemble.hip", "hyq_ens_fp64.hip", "hyq_ens_mixed.hip",
-           "hyq_ens_newton.hip", "hyq_mlp.hip"]
+# (source, object, extra flags): the warm-start network is built once per hidden width
+SOURCES = [("hyq_kernels.hip", "hyq_kernels.o", []), ("hyq_ensemble.hip", "hyq_ensemble.o", []),
+           ("hyq_ens_fp64.hip", "hyq_ens_fp64.o", []), ("hyq_ens_mixed.hip", "hyq_ens_mixed.o", []),
+           ("hyq_ens_newton.hip", "hyq_ens_newton.o", []),
+           ("hyq_mlp.hip", "hyq_mlp_h32.o", ["-DHYQ_MLP_HID=32", "-DHYQ_MLP_NS=mlp_h32"]),
+           ("hyq_mlp.hip", "hyq_mlp_h128.o", ["-DHYQ_MLP_HID=128", "-DHYQ_MLP_NS=mlp_h128"]),
+           ("hyq_mlp_api.hip", "hyq_mlp_api.o", [])]
 
 
 def _run(cmd, verbose):
@@ -36,8 +41,9 @@ def _run(cmd, verbose):
 def build(verbose=True):
     out = os.path.join(HERE, "build")
     os.makedirs(out, exist_ok=True)
-    objs = [os.path.join(out, f.replace(".hip", ".o")) for f in SOURCES]
-    cmds = [[HIPCC] + FLAGS + ["-c", "-o", o, os.path.join(HERE, "csrc", f)] for f, o in zip(SOURCES, objs)]
+    objs = [os.path.join(out, o) for _, o, _ in SOURCES]
+    cmds = [[HIPCC] + FLAGS + extra + ["-c", "-o", o, os.path.join(HERE, "csrc", f)]
+            for (f, _, extra), o in zip(SOURCES, objs)]
     with ThreadPoolExecutor(len(cmds)) as ex:
         list(ex.map(lambda c: _run(c, verbose), cmds))
     target = os.path.join(out, "libhyqmom_hip.so")

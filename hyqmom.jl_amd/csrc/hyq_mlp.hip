@@ -36,8 +36,14 @@ int mfail(int code, const char* what, hipError_t e = hipSuccess) {
     if (_e != hipSuccess) return mfail(HYQ_EHIP, msg, _e); \
   } while (0)
 
+// The hidden width is fixed per compilation: this file is built once per supported width,
+// each build with its own HYQ_MLP_HID and namespace HYQ_MLP_NS for its entry points, and
+// hyq_mlp_api.hip dispatches the C ABI to the width a network was created with.
 #ifndef HYQ_MLP_HID
 #define HYQ_MLP_HID 32
+#endif
+#ifndef HYQ_MLP_NS
+#define HYQ_MLP_NS mlp_h32
 #endif
 constexpr int IN = 16, HID = HYQ_MLP_HID, OUT = 16;
 static_assert(HID % 32 == 0 && HID >= 32 && HID <= 128, "hidden width: 32, 64, 96 or 128");
@@ -469,9 +475,7 @@ void gemm(hipStream_t s, const float* A, long sam, long sak, const float* B, lon
                      (int)K, bias, Z, H, ldc, Nst, act);
 }
 
-}  // namespace
-
-struct hyq_mlp {
+struct Net {
   float* blob;   // parameters + normalization (device)
   float* grad;   // NPARAM
   float* m1;     // Adam moments
@@ -489,7 +493,7 @@ struct hyq_mlp {
   long version, packed_version;  // blob changes / version the packed copy was made from
 };
 
-static int ensure_icap(hyq_mlp* m, long n) {
+static int ensure_icap(Net* m, long n) {
   if (n <= m->icap) return HYQ_OK;
   float** bufs[] = {&m->iX, &m->iA, &m->iB, &m->iY};
   const long width[] = {IN, HID, HID, OUT};
@@ -503,7 +507,7 @@ static int ensure_icap(hyq_mlp* m, long n) {
   return HYQ_OK;
 }
 
-static int ensure_cap(hyq_mlp* m, long n) {
+static int ensure_cap(Net* m, long n) {
   if (n <= m->cap) return HYQ_OK;
   float** bufs[] = {&m->Xn, &m->Z1, &m->H1, &m->Z2, &m->H2, &m->Z3, &m->H3, &m->Y, &m->dY, &m->dA, &m->dB};
   const long width[] = {IN, HID, HID, HID, HID, HID, HID, OUT, OUT, HID, HID};
@@ -518,7 +522,7 @@ static int ensure_cap(hyq_mlp* m, long n) {
 }
 
 // forward on normalized inputs m->Xn (n rows); fills Z*, H*, Y
-static void forward(hyq_mlp* m, long n) {
+static void forward(Net* m, long n) {
   const float* p = m->blob;
   gemm<NTH>(m->stream, m->Xn, IN, 1, p + OFF_W1, HID, 1, n, HID, IN, p + OFF_B1, m->Z1, m->H1, HID, HID, 1);
   gemm<NTH>(m->stream, m->H1, HID, 1, p + OFF_W2, HID, 1, n, HID, HID, p + OFF_B2, m->Z2, m->H2, HID, HID, 1);
@@ -527,7 +531,7 @@ static void forward(hyq_mlp* m, long n) {
 }
 
 // gradients of the loss into m->grad (forward must have run)
-static void backward(hyq_mlp* m, long n) {
+static void backward(Net* m, long n) {
   const float* p = m->blob;
   float* g = m->grad;
   hipStream_t s = m->stream;
@@ -551,15 +555,19 @@ static void backward(hyq_mlp* m, long n) {
   hipLaunchKernelGGL(k_colsum, dim3(HID), dim3(256), 0, s, m->dA, n, HID, g + OFF_B1);
 }
 
-extern "C" {
+}  // namespace
 
-const char* hyq_mlp_last_error(void) { return g_mlp_err.c_str(); }
-int64_t hyq_mlp_blob_size(void) { return (int64_t)NBLOB; }
-int64_t hyq_mlp_param_count(void) { return (int64_t)NPARAM; }
+// Entry points of this width's network (the C ABI in hyq_mlp_api.hip calls them); a network
+// is passed as the void* that create returned.
+namespace HYQ_MLP_NS {
 
-int hyq_mlp_create(const float* blob_host, hyq_mlp** out) {
+const char* last_error() { return g_mlp_err.c_str(); }
+int64_t blob_size() { return (int64_t)NBLOB; }
+int64_t param_count() { return (int64_t)NPARAM; }
+
+int create(const float* blob_host, void** out) {
   if (!out) return mfail(HYQ_EINVAL, "null argument");
-  hyq_mlp* m = new hyq_mlp();
+  Net* m = new Net();
   std::memset(m, 0, sizeof(*m));
   if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
     delete m;
@@ -591,7 +599,8 @@ int hyq_mlp_create(const float* blob_host, hyq_mlp** out) {
   return HYQ_OK;
 }
 
-int hyq_mlp_destroy(hyq_mlp* m) {
+int destroy(void* h) {
+  Net* m = static_cast<Net*>(h);
   if (!m) return HYQ_OK;
   (void)hipStreamSynchronize(m->stream);
   float* bufs[] = {m->blob, m->grad, m->m1, m->m2, m->loss, m->Xn, m->Z1, m->H1, m->Z2,
@@ -605,13 +614,15 @@ int hyq_mlp_destroy(hyq_mlp* m) {
   return HYQ_OK;
 }
 
-int hyq_mlp_get_blob(hyq_mlp* m, float* blob_host) {
+int get_blob(void* h, float* blob_host) {
+  Net* m = static_cast<Net*>(h);
   if (!m || !blob_host) return mfail(HYQ_EINVAL, "null argument");
   MCHK(hipStreamSynchronize(m->stream), "mlp sync");
   MCHK(hipMemcpy(blob_host, m->blob, sizeof(float) * NBLOB, hipMemcpyDeviceToHost), "mlp blob");
   return HYQ_OK;
 }
-int hyq_mlp_set_blob(hyq_mlp* m, const float* blob_host) {
+int set_blob(void* h, const float* blob_host) {
+  Net* m = static_cast<Net*>(h);
   if (!m || !blob_host) return mfail(HYQ_EINVAL, "null argument");
   MCHK(hipStreamSynchronize(m->stream), "mlp sync");
   MCHK(hipMemcpy(m->blob, blob_host, sizeof(float) * NBLOB, hipMemcpyHostToDevice), "mlp blob");
@@ -622,7 +633,7 @@ int hyq_mlp_set_blob(hyq_mlp* m, const float* blob_host) {
   return HYQ_OK;
 }
 
-static int infer_fused(hyq_mlp* m, const void* X, int64_t n, float* P, hipStream_t stream,
+int infer_fused(Net* m, const void* X, int64_t n, float* P, hipStream_t stream,
                        bool mask, bool bin = false) {
   if (!m || !X || !P || n < 0) return mfail(HYQ_EINVAL, "bad argument");
   if (n == 0) return HYQ_OK;
@@ -660,27 +671,33 @@ static int infer_fused(hyq_mlp* m, const void* X, int64_t n, float* P, hipStream
 // Predicted (lmin, lmax) for n feature rows X [n][16] (device) into P [n][2] (device),
 // enqueued on `stream` without synchronizing: the fused kernel (the packed weights are
 // refreshed first when the parameters changed).
-int hyq_mlp_infer_async(hyq_mlp* m, const float* X, int64_t n, float* P, hipStream_t stream) {
+int infer_async(void* h, const float* X, int64_t n, float* P, hipStream_t stream) {
+  Net* m = static_cast<Net*>(h);
   return infer_fused(m, X, n, P, stream, false);
 }
 // The step's inference: rows whose column 15 is 0 (near-Gaussian planes) need no
 // prediction; 16-row tiles made only of such rows are skipped and get NaN.
-int hyq_mlp_infer_masked_async(hyq_mlp* m, const float* X, int64_t n, float* P,
-                               hipStream_t stream) {
+int infer_masked_async(void* h, const float* X, int64_t n, float* P, hipStream_t stream) {
+  Net* m = static_cast<Net*>(h);
   return infer_fused(m, X, n, P, stream, true);
 }
 
 // The same from normalized bf16 rows Xb [n][16] (column 15 the need flag, 0 or 1), as the
 // step's projection kernel writes them with the network's input normalization
 // (hyq_mlp_input_norm); the predictions are the same bits as from the float rows.
-int hyq_mlp_infer_bf16_masked_async(hyq_mlp* m, const short* Xb, int64_t n, float* P,
-                                    hipStream_t stream) {
+int infer_bf16_masked_async(void* h, const short* Xb, int64_t n, float* P,
+                            hipStream_t stream) {
+  Net* m = static_cast<Net*>(h);
   return infer_fused(m, Xb, n, P, stream, true, true);
 }
 // Device pointer to the input normalization: 16 means, then 16 standard deviations.
-const float* hyq_mlp_input_norm(hyq_mlp* m) { return m ? m->blob + OFF_XMU : nullptr; }
+const float* input_norm(void* h) {
+  Net* m = static_cast<Net*>(h);
+  return m ? m->blob + OFF_XMU : nullptr;
+}
 
-int hyq_mlp_set_shape(hyq_mlp* m, int32_t shape) {
+int set_shape(void* h, int32_t shape) {
+  Net* m = static_cast<Net*>(h);
   if (!m || shape < -1 || shape >= kFusedShapes) return mfail(HYQ_EINVAL, "bad argument");
   m->shape = shape;
   return HYQ_OK;
@@ -688,7 +705,8 @@ int hyq_mlp_set_shape(hyq_mlp* m, int32_t shape) {
 
 // The same prediction through the layer-by-layer GEMM kernels of training (a cross-check
 // of the fused kernel).
-int hyq_mlp_infer_layers(hyq_mlp* m, const float* X, int64_t n, float* P) {
+int infer_layers(void* h, const float* X, int64_t n, float* P) {
+  Net* m = static_cast<Net*>(h);
   if (!m || !X || !P || n < 0) return mfail(HYQ_EINVAL, "bad argument");
   if (n == 0) return HYQ_OK;
   int rc = ensure_icap(m, n);
@@ -705,9 +723,10 @@ int hyq_mlp_infer_layers(hyq_mlp* m, const float* X, int64_t n, float* P) {
   MCHK(hipStreamSynchronize(stream), "mlp infer sync");
   return HYQ_OK;
 }
-int hyq_mlp_infer(hyq_mlp* m, const float* X, int64_t n, float* P) {
+int infer(void* h, const float* X, int64_t n, float* P) {
+  Net* m = static_cast<Net*>(h);
   if (!m) return mfail(HYQ_EINVAL, "null argument");
-  int rc = hyq_mlp_infer_async(m, X, n, P, m->stream);
+  int rc = infer_async(m, X, n, P, m->stream);
   if (rc) return rc;
   MCHK(hipStreamSynchronize(m->stream), "mlp infer sync");
   return HYQ_OK;
@@ -715,8 +734,9 @@ int hyq_mlp_infer(hyq_mlp* m, const float* X, int64_t n, float* P) {
 
 // Loss and parameter gradients on one batch (X [n][16], T [n][2], device); grad_host may
 // be null.  No update.
-int hyq_mlp_grad(hyq_mlp* m, const float* X, const float* T, int64_t n, float* loss_out,
-                 float* grad_host) {
+int grad(void* h, const float* X, const float* T, int64_t n, float* loss_out,
+         float* grad_host) {
+  Net* m = static_cast<Net*>(h);
   if (!m || !X || !T || n <= 0) return mfail(HYQ_EINVAL, "bad argument");
   int rc = ensure_cap(m, n);
   if (rc) return rc;
@@ -736,9 +756,10 @@ int hyq_mlp_grad(hyq_mlp* m, const float* X, const float* T, int64_t n, float* l
 }
 
 // One Adam step on a batch.
-int hyq_mlp_train_step(hyq_mlp* m, const float* X, const float* T, int64_t n, float lr,
-                       float* loss_out) {
-  int rc = hyq_mlp_grad(m, X, T, n, loss_out, nullptr);
+int train_step(void* h, const float* X, const float* T, int64_t n, float lr,
+               float* loss_out) {
+  Net* m = static_cast<Net*>(h);
+  int rc = grad(m, X, T, n, loss_out, nullptr);
   if (rc) return rc;
   m->step += 1;
   m->version += 1;
@@ -751,4 +772,4 @@ int hyq_mlp_train_step(hyq_mlp* m, const float* X, const float* T, int64_t n, fl
   return HYQ_OK;
 }
 
-}  // extern "C"
+}  // namespace HYQ_MLP_NS

@@ -74,6 +74,10 @@ SIGNATURES = {
     "hyq_mlp_blob_size": ([], ctypes.c_int64),
     "hyq_mlp_param_count": ([], ctypes.c_int64),
     "hyq_mlp_create": ([VP, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "hyq_mlp_create_hidden": ([ctypes.c_int32, VP, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "hyq_mlp_blob_size_hidden": ([ctypes.c_int32], ctypes.c_int64),
+    "hyq_mlp_param_count_hidden": ([ctypes.c_int32], ctypes.c_int64),
+    "hyq_mlp_hidden": ([VP], ctypes.c_int32),
     "hyq_mlp_destroy": ([VP], ctypes.c_int),
     "hyq_mlp_get_blob": ([VP, VP], ctypes.c_int),
     "hyq_mlp_set_blob": ([VP, VP], ctypes.c_int),

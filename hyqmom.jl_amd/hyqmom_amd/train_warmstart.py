@@ -122,6 +122,7 @@ def main(argv=None):
     ap.add_argument("--c2-steps", default="0,5,10,20,30")
     ap.add_argument("--holdout-step", type=int, default=40)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--hidden", type=int, default=32, choices=list(W.HIDDEN_WIDTHS))
     a = ap.parse_args(argv)
 
     def log(msg):
@@ -139,7 +140,7 @@ def main(argv=None):
     log("training samples: %d" % X.shape[0])
     t1 = time.time()
     model, hist = W.train(X, L, steps=a.train_steps, batch=a.batch, lr=a.lr, lr_final=a.lr * 0.05,
-                          seed=a.seed, log=log)
+                          seed=a.seed, log=log, hidden=a.hidden)
     torch.cuda.synchronize()
     train_s = time.time() - t1
     model.save(a.out)
@@ -150,7 +151,7 @@ def main(argv=None):
     P = model.infer(Xh[keep]).cpu().numpy()
     T = Lh[keep].cpu().numpy()
     err = np.abs(P - T)
-    rep = {"samples": int(X.shape[0]), "train_steps": a.train_steps, "batch": a.batch,
+    rep = {"hidden": a.hidden, "samples": int(X.shape[0]), "train_steps": a.train_steps, "batch": a.batch,
            "train_seconds": train_s, "loss_first": hist[0][1], "loss_last": hist[-1][1],
            "holdout_planes": int(keep.sum()),
            "holdout_abs_err": {"median": float(np.median(err)), "p95": float(np.percentile(err, 95)),

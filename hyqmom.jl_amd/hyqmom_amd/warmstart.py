@@ -1,5 +1,5 @@
-"""The learned warm start of the wave-speed solve: a 15 -> H -> H -> H -> 2 GELU MLP (H = 32
-as the library is built) on bf16 MFMA that predicts the standardized extreme eigenvalues of
+"""The learned warm start of the wave-speed solve: a 15 -> H -> H -> H -> 2 GELU MLP (H = 32,
+the shipped network, or 128) on bf16 MFMA that predicts the standardized extreme eigenvalues of
 each plane's flux Jacobian (no counterpart in HyQMOM.jl).  Everything runs in libhyqmom_hip.so; this module
 holds device buffers (torch tensors, plumbing only) and the training loop.
 
@@ -17,6 +17,8 @@ from . import _lib
 
 MODEL_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "models")
 DEFAULT_MODEL = os.path.join(MODEL_DIR, "warmstart_mlp.f32")
+# the warm-start configuration's network: three hidden layers of width 128
+MODEL_H128 = os.path.join(MODEL_DIR, "warmstart_mlp_h128.f32")
 
 
 def _torch():
@@ -36,20 +38,39 @@ def _sync():
     _torch().cuda.current_stream().synchronize()
 
 
+HIDDEN_WIDTHS = (32, 128)
+
+
+def width_of_blob(nblob):
+    """The hidden width of a parameter blob of nblob floats."""
+    L = _lib.lib()
+    for H in HIDDEN_WIDTHS:
+        if int(L.hyq_mlp_blob_size_hidden(H)) == nblob:
+            return H
+    raise ValueError("no supported hidden width has a %d-float blob" % nblob)
+
+
 class WarmStartMLP:
-    def __init__(self, blob=None):
+    def __init__(self, blob=None, hidden=None):
+        """A network from a parameter blob (its width follows from its size) or, with no blob,
+        zeros of the given hidden width (32 by default)."""
         L = _lib.lib()
-        self.nblob = int(L.hyq_mlp_blob_size())
-        self.nparam = int(L.hyq_mlp_param_count())
-        h = ctypes.c_void_p()
         if blob is not None:
             blob = np.ascontiguousarray(blob, dtype=np.float32)
-            if blob.size != self.nblob:
-                raise ValueError("blob has %d floats, expected %d" % (blob.size, self.nblob))
-            rc = L.hyq_mlp_create(blob.ctypes.data_as(ctypes.c_void_p), ctypes.byref(h))
+            H = width_of_blob(blob.size)
+            if hidden is not None and hidden != H:
+                raise ValueError("blob is for hidden width %d, not %d" % (H, hidden))
         else:
-            rc = L.hyq_mlp_create(None, ctypes.byref(h))
-        self._check(rc, "hyq_mlp_create")
+            H = 32 if hidden is None else int(hidden)
+            if H not in HIDDEN_WIDTHS:
+                raise ValueError("hidden width must be one of %s" % (HIDDEN_WIDTHS,))
+        self.hidden = H
+        self.nblob = int(L.hyq_mlp_blob_size_hidden(H))
+        self.nparam = int(L.hyq_mlp_param_count_hidden(H))
+        h = ctypes.c_void_p()
+        ptr = blob.ctypes.data_as(ctypes.c_void_p) if blob is not None else None
+        rc = L.hyq_mlp_create_hidden(H, ptr, ctypes.byref(h))
+        self._check(rc, "hyq_mlp_create_hidden")
         self.handle = h
 
     @staticmethod
@@ -188,13 +209,14 @@ def init_blob(nparam, norm, seed=0):
     return blob
 
 
-def train(X, L, steps=3000, batch=8192, lr=2e-3, lr_final=1e-4, seed=0, log=None):
-    """Fit a fresh network to (X, L) (device tensors of valid samples).  Adam with a cosine
-    learning-rate decay from lr to lr_final.  Returns (model, list of (step, loss))."""
+def train(X, L, steps=3000, batch=8192, lr=2e-3, lr_final=1e-4, seed=0, log=None, hidden=32):
+    """Fit a fresh network of the given hidden width to (X, L) (device tensors of valid
+    samples).  Adam with a cosine learning-rate decay from lr to lr_final.  Returns (model,
+    list of (step, loss))."""
     torch = _torch()
     L_ = _lib.lib()
     norm = fit_normalization(X, L)
-    m = WarmStartMLP(init_blob(int(L_.hyq_mlp_param_count()), norm, seed))
+    m = WarmStartMLP(init_blob(int(L_.hyq_mlp_param_count_hidden(hidden)), norm, seed))
     g = torch.Generator(device=X.device)
     g.manual_seed(seed)
     n = X.shape[0]

@@ -208,16 +208,21 @@ int hyq_hll_lines(const double* M, const double* F, const double* vmin, const do
 int hyq_collision_batch(const double* M, int64_t n, double dt, double Kn, int32_t positivity,
                         double* Mout);
 
-/* ---- learned warm start (a 15 -> H -> H -> H -> 2 GELU MLP on bf16 MFMA, H = 32) --------
+/* ---- learned warm start (a 15 -> H -> H -> H -> 2 GELU MLP on bf16 MFMA, H = 32 or 128) ---
  * No counterpart in the reference: the network predicts the extreme eigenvalues of the
  * standardized flux Jacobian of a plane from its 15 standardized moments; the predictions
  * only seed a Laguerre iteration whose result is exact to FP64 or falls back to the full solve.
- * Parameters travel as one FP32 blob of hyq_mlp_blob_size() floats (weights, biases, input
- * and label normalization); hyq_mlp_param_count() = 2 H^2 + 35 H + 16 gives H. */
+ * Parameters travel as one FP32 blob of hyq_mlp_blob_size_hidden(H) floats (weights, biases,
+ * input and label normalization); hyq_mlp_param_count_hidden(H) = 2 H^2 + 35 H + 16.  The
+ * hidden width is chosen when a network is created; the calls without it mean H = 32. */
 typedef struct hyq_mlp hyq_mlp;
 int64_t hyq_mlp_blob_size(void);
 int64_t hyq_mlp_param_count(void);
+int64_t hyq_mlp_blob_size_hidden(int32_t hidden);   /* -1 for an unsupported width */
+int64_t hyq_mlp_param_count_hidden(int32_t hidden); /* -1 for an unsupported width */
 int hyq_mlp_create(const float* blob_host /* NULL: zeros */, hyq_mlp** out);
+int hyq_mlp_create_hidden(int32_t hidden /* 32 or 128 */, const float* blob_host, hyq_mlp** out);
+int32_t hyq_mlp_hidden(hyq_mlp* m);
 int hyq_mlp_destroy(hyq_mlp* m);
 int hyq_mlp_get_blob(hyq_mlp* m, float* blob_host);
 int hyq_mlp_set_blob(hyq_mlp* m, const float* blob_host);

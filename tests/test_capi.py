@@ -10,7 +10,7 @@ ROOT = os.path.dirname(HERE)
 
 def declared_symbols():
     txt = open(os.path.join(ROOT, "include", "hyqmom.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(hyq_\w+)\s*\(", txt, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|int32_t|const char\*)\s+(hyq_\w+)\s*\(", txt, re.M)))
 
 
 def test_library_exports_header_symbols():

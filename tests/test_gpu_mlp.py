@@ -1,6 +1,7 @@
-"""The warm-start network on the GPU (bf16 MFMA kernels) against its CPU reference, the
-training dataset against the oracle's eigen solve, and the mixed_learned / mixed_newton precision
-modes against the plain mixed step."""
+"""The warm-start network on the GPU (bf16 MFMA kernels) against its CPU reference, at both
+hidden widths (32, the shipped network, and 128, the width of the warm-start configuration),
+the training dataset against the oracle's eigen solve, and the mixed_learned / mixed_newton
+precision modes against the plain mixed step."""
 import numpy as np
 import pytest
 
@@ -15,34 +16,40 @@ def _dev(a, dtype=np.float32):
     return to_dev(np.ascontiguousarray(a, dtype=dtype))
 
 
+WIDTHS = [32, 128]
+
+
+@pytest.mark.parametrize("hidden", WIDTHS)
 @pytest.mark.parametrize("n", [1, 37, 1000])
-def test_forward_matches_reference(n):
+def test_forward_matches_reference(n, hidden):
     """Random weights: any mistake in the MFMA operand/result lane layout or the strides
     gives O(1) errors; bf16 rounding of the operands is reproduced by the reference, so
     the remaining difference is FP32 accumulation order (and rare bf16 rounding flips)."""
     from hyqmom_amd.warmstart import WarmStartMLP
-    blob = R.random_blob(11)
+    blob = R.random_blob(11, hidden=hidden)
     X = np.random.default_rng(12).standard_normal((n, 16)).astype(np.float32)
     m = WarmStartMLP(blob)
+    assert m.hidden == hidden
     P = m.infer(_dev(X)).cpu().numpy()
     ref = R.predict(blob, X, use_bf16=True)
     ysd = R.unpack(blob)["ysd"]
     err = np.abs(P - ref) / ysd
-    print("n=%d forward max err %.3e mean %.3e" % (n, err.max(), err.mean()))
+    print("H=%d n=%d forward max err %.3e mean %.3e" % (hidden, n, err.max(), err.mean()))
     assert err.max() < 2e-2 and err.mean() < 2e-3
     # and the reference differs from the unrounded FP64 network by more than that
     ref64 = R.predict(blob, X, use_bf16=False)
     assert np.abs(P - ref64).max() / ysd.max() > 0 or n == 1
 
 
+@pytest.mark.parametrize("hidden", WIDTHS)
 @pytest.mark.parametrize("n", [1, 63, 4099, 70001])
-def test_fused_inference_equals_layers(n):
+def test_fused_inference_equals_layers(n, hidden):
     """The fused kernel (weights packed in LDS, activations per wave) rounds and accumulates
     exactly like the layer-by-layer kernels: bitwise equal predictions (n = 70001 also runs
     the launch-shape timing, which keeps the faster of two equal-valued shapes)."""
     import torch
     from hyqmom_amd.warmstart import WarmStartMLP
-    blob = R.random_blob(13)
+    blob = R.random_blob(13, hidden=hidden)
     X = np.random.default_rng(14).standard_normal((n, 16)).astype(np.float32)
     m = WarmStartMLP(blob)
     Xd = _dev(X)
@@ -50,15 +57,16 @@ def test_fused_inference_equals_layers(n):
     b = m.infer_layers(Xd).cpu().numpy()
     assert np.array_equal(a, b), np.abs(a - b).max()
     # a parameter change reaches the packed copy
-    blob2 = R.random_blob(15)
+    blob2 = R.random_blob(15, hidden=hidden)
     m.set_blob(blob2)
     c = m.infer(Xd).cpu().numpy()
     assert np.array_equal(c, m.infer_layers(Xd).cpu().numpy())
     assert not np.array_equal(a, c)
 
 
+@pytest.mark.parametrize("hidden", WIDTHS)
 @pytest.mark.parametrize("shape", [0, 1, 2])
-def test_every_fused_shape_equals_layers(shape):
+def test_every_fused_shape_equals_layers(shape, hidden):
     """Each launch shape of the fused inference gives the layer-by-layer predictions bit for
     bit, masked rows aside:
     with the mask a 16-row tile whose rows all have column 15 = 0 gets NaN."""
@@ -68,7 +76,7 @@ def test_every_fused_shape_equals_layers(shape):
     X = np.random.default_rng(21).standard_normal((n, 16)).astype(np.float32)
     X[:, 15] = 1.0
     X[1600:1632, 15] = 0.0  # two whole 16-row tiles to skip
-    m = WarmStartMLP(R.random_blob(22))
+    m = WarmStartMLP(R.random_blob(22, hidden=hidden))
     m.set_shape(shape)
     Xd = _dev(X)
     want = m.infer_layers(Xd).cpu().numpy()
@@ -91,9 +99,10 @@ def test_every_fused_shape_equals_layers(shape):
     assert np.array_equal(Pm[~skip], want[~skip])
 
 
-def test_gradient_matches_reference():
+@pytest.mark.parametrize("hidden", WIDTHS)
+def test_gradient_matches_reference(hidden):
     from hyqmom_amd.warmstart import WarmStartMLP
-    blob = R.random_blob(21)
+    blob = R.random_blob(21, hidden=hidden)
     rng = np.random.default_rng(22)
     n = 777
     X = rng.standard_normal((n, 16)).astype(np.float32)
@@ -103,7 +112,7 @@ def test_gradient_matches_reference():
     lref, gref = R.loss_and_grad(blob, X, T, use_bf16=True)
     assert abs(loss - lref) <= 1e-3 * lref, (loss, lref)
     o = 0
-    for name, shp in R.SHAPES:
+    for name, shp in R.shapes(hidden):
         k = int(np.prod(shp))
         a, b = g[o:o + k], gref[o:o + k]
         o += k
@@ -116,7 +125,8 @@ def test_gradient_matches_reference():
         assert rel < 1e-2, (name, rel)
 
 
-def test_training_reduces_loss():
+@pytest.mark.parametrize("hidden", WIDTHS)
+def test_training_reduces_loss(hidden):
     """Adam on a smooth synthetic regression: the loss falls by an order of magnitude."""
     import torch
     from hyqmom_amd import warmstart as W
@@ -127,8 +137,8 @@ def test_training_reduces_loss():
     T = np.stack([-1.5 - 0.3 * np.tanh(X[:, 0] * X[:, 3]) - 0.2 * X[:, 4] ** 2,
                   1.5 + 0.3 * np.sin(X[:, 6]) + 0.2 * X[:, 7] * X[:, 8]], axis=1).astype(np.float32)
     Xd, Td = _dev(X), _dev(T)
-    m, hist = W.train(Xd, Td, steps=400, batch=2048, lr=2e-3, lr_final=2e-4, seed=1)
-    print("loss history", hist[0], hist[-1])
+    m, hist = W.train(Xd, Td, steps=400, batch=2048, lr=2e-3, lr_final=2e-4, seed=1, hidden=hidden)
+    print("H=%d loss history" % hidden, hist[0], hist[-1])
     assert hist[-1][1] < 0.1 * hist[0][1]
     b1 = m.blob()
     m2 = W.WarmStartMLP(b1)
@@ -244,8 +254,9 @@ def test_trained_network_reduces_iterations():
     assert (np.abs(M_l - M_n) / sc).max() < 1e-6
 
 
+@pytest.mark.parametrize("hidden", WIDTHS)
 @pytest.mark.parametrize("shape", [0, 1, 2])
-def test_bf16_rows_give_the_same_predictions(shape):
+def test_bf16_rows_give_the_same_predictions(shape, hidden):
     """The step's projection kernel hands the network its input rows already normalized and
     rounded to bf16 (the same FP32 subtraction and division, then round to nearest even);
     the fused kernel fed those rows gives the float rows' predictions bit for bit, masked
@@ -258,11 +269,12 @@ def test_bf16_rows_give_the_same_predictions(shape):
     X = np.random.default_rng(23).standard_normal((n, 16)).astype(np.float32)
     X[:, 15] = 1.0
     X[1600:1632, 15] = 0.0
-    blob = R.random_blob(24)
+    blob = R.random_blob(24, hidden=hidden)
     m = WarmStartMLP(blob)
     m.set_shape(shape)
-    mu = blob[R.NPARAM:R.NPARAM + 16].astype(np.float32)
-    sd = blob[R.NPARAM + 16:R.NPARAM + 32].astype(np.float32)
+    npar = R.nparam(hidden)
+    mu = blob[npar:npar + 16].astype(np.float32)
+    sd = blob[npar + 16:npar + 32].astype(np.float32)
     Xn = X.copy()
     Xn[:, :15] = (X[:, :15] - mu[:15]) / sd[:15]
     bits = R.bf16(Xn).view(np.uint32) >> 16
@@ -283,3 +295,61 @@ def test_bf16_rows_give_the_same_predictions(shape):
     skip[1600:1632] = True
     assert np.isnan(b[skip]).all()
     assert np.array_equal(a[~skip], b[~skip]), np.abs(a - b)[~skip].max()
+
+
+def test_width128_network_in_the_production_step():
+    """The warm-start configuration: the 15 -> 128 -> 128 -> 128 -> 2 network
+    (hyqmom.jl_amd/models/warmstart_mlp_h128.f32, trained on the GPU by the library's own
+    kernels on 10^7 planes of crossing-jets runs) applied in the 120x120x60 step (Ma=1.5,
+    Kn=0.01) from the state of step 40 (the training's held-out state): fewer Laguerre iterations than the analytic start, the
+    same fallback cells, and the step within 1e-6 of each field of the plain FP64 step on
+    every cell whose FP64 step is not chaotic under one-ulp input changes."""
+    import hyqmom_amd as H
+    import parity_util as PU
+    from hyqmom_amd import _lib
+    from hyqmom_amd.warmstart import MODEL_H128, WarmStartMLP
+    N = (120, 120, 60)
+    p = dict(_params(N, 1.5, 0.01, "fp64"), tmax=1e9, nnmax=10 ** 9)
+    s = H.Solver(p)
+    s.set_state(H.initial_state(p))
+    s.run(1e9, 40)
+    M = np.asfortranarray(s.get_state())
+    s.close()
+
+    def one(X, prec, model=None):
+        q = dict(p, precision=prec)
+        if model is not None:
+            q["warmstart_model"] = model
+        s = H.Solver(q)
+        s.set_state(X)
+        s.step()
+        fl = s.flags()
+        out = (s.get_state().reshape(-1, 35, order="F"), fl)
+        s.close()
+        return out
+
+    net = WarmStartMLP.load(MODEL_H128)
+    assert net.hidden == 128
+    f64, _ = one(M, "fp64")
+    new, fl_n = one(M, "mixed_newton")
+    lrn, fl_l = one(M, "mixed_learned", net)
+    it_n, it_l = _lib.newton_iterations(fl_n).mean(), _lib.newton_iterations(fl_l).mean()
+    fb_n, fb_l = int(((fl_n >> 12) & 1).sum()), int(((fl_l >> 12) & 1).sum())
+    rng = np.random.default_rng(61)
+    sens = np.zeros(len(f64))
+    for _ in range(2):
+        sens = np.maximum(sens, PU.cell_err(one(PU.ulp_perturb(M, rng), "fp64")[0], f64, PU.field_scale(f64)))
+    keep = sens <= PU.CHAOTIC
+
+    def fields(X):
+        r = X[:, 0]
+        return np.stack([r, X[:, 1] / r, X[:, 5] / r, X[:, 15] / r, 0.5 * (X[:, 2] + X[:, 9] + X[:, 19])], axis=1)
+
+    fo, fg = fields(f64)[keep], fields(lrn)[keep]
+    err = (np.abs(fg - fo) / np.abs(fo).max(axis=0)).max(axis=0)
+    print("width 128: Laguerre iterations per cell %.2f (analytic start %.2f, %.0f%% fewer), fallback "
+          "cells %d / %d, per-field error vs FP64 %s over %d cells (%d chaotic)"
+          % (it_l, it_n, 100 * (1 - it_l / it_n), fb_l, fb_n, ["%.1e" % e for e in err], keep.sum(), (~keep).sum()))
+    assert it_l < 0.9 * it_n
+    assert fb_l == fb_n
+    assert np.all(err <= 1e-6), err
