@@ -15,18 +15,42 @@ accumulate).  g is the tanh form of GELU, z sigmoid(2 sqrt(2/pi) (z + 0.044715 z
 With bf16=False everything stays in FP64, which the finite-difference check of the backward
 pass uses.
 
-Parameter blob (FP32), hidden width H = HID (32, as the library is built): W1 [16][H], b1 [H],
-W2 [H][H], b2, W3 [H][H], b3, W4 [H][16], b4 [16], then xmu [16], xsd [16], ymu [2], ysd [2].
+Parameter blob (FP32), hidden width H (32 or 128; the blob's size gives it): W1 [16][H],
+b1 [H], W2 [H][H], b2, W3 [H][H], b3, W4 [H][16], b4 [16], then xmu [16], xsd [16], ymu [2],
+ysd [2].
 """
 import math
 
 import numpy as np
 
 IN, HID, OUT = 16, 32, 16
-SHAPES = [("W1", (IN, HID)), ("b1", (HID,)), ("W2", (HID, HID)), ("b2", (HID,)),
-          ("W3", (HID, HID)), ("b3", (HID,)), ("W4", (HID, OUT)), ("b4", (OUT,))]
-NPARAM = sum(int(np.prod(s)) for _, s in SHAPES)
-NBLOB = NPARAM + 16 + 16 + 2 + 2
+
+
+def shapes(H=HID):
+    return [("W1", (IN, H)), ("b1", (H,)), ("W2", (H, H)), ("b2", (H,)),
+            ("W3", (H, H)), ("b3", (H,)), ("W4", (H, OUT)), ("b4", (OUT,))]
+
+
+def nparam(H=HID):
+    return sum(int(np.prod(s)) for _, s in shapes(H))
+
+
+def nblob(H=HID):
+    return nparam(H) + 16 + 16 + 2 + 2
+
+
+SHAPES = shapes()
+NPARAM = nparam()
+NBLOB = nblob()
+
+
+def width(blob):
+    """The hidden width of a parameter blob."""
+    for H in (32, 64, 96, 128):
+        if nblob(H) == np.asarray(blob).size:
+            return H
+    raise ValueError("no hidden width has a %d-float blob" % np.asarray(blob).size)
+
 
 def bf16(a):
     """Round float32 values to bfloat16 (nearest even), returned as float32."""
@@ -39,7 +63,7 @@ def bf16(a):
 def unpack(blob):
     blob = np.asarray(blob)
     out, o = {}, 0
-    for name, shp in SHAPES:
+    for name, shp in shapes(width(blob)):
         n = int(np.prod(shp))
         out[name] = blob[o:o + n].reshape(shp)
         o += n
@@ -50,11 +74,11 @@ def unpack(blob):
     return out
 
 
-def random_blob(seed=0, scale=1.0):
+def random_blob(seed=0, scale=1.0, hidden=HID):
     """He-style random weights with unit normalization statistics."""
     rng = np.random.default_rng(seed)
     parts = []
-    for name, shp in SHAPES:
+    for name, shp in shapes(hidden):
         if name.startswith("W"):
             fan_in = shp[0]
             parts.append(rng.standard_normal(shp).ravel() * scale * math.sqrt(2.0 / fan_in))
@@ -147,5 +171,5 @@ def loss_and_grad(blob, X, T, use_bf16=True):
         if k > 1:
             dA = _mm(dA, p["W%d" % k].T, use_bf16) * _dgelu(np.asarray(act["Z%d" % (k - 1)], np.float64))
             dA = _f(dA, use_bf16)
-    grad = np.concatenate([np.ravel(g[name]) for name, _ in SHAPES])
+    grad = np.concatenate([np.ravel(g[name]) for name, _ in shapes(width(blob))])
     return loss, grad
