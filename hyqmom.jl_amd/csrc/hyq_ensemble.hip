@@ -164,6 +164,10 @@ int hyq_ensemble_create(const hyq_params* runs, int32_t nruns, hyq_ensemble** ou
     q.nnmax = (1ll << 62);
     e->runs.push_back(q);
     off += q.N;
+    if (off >= hyq::kMaxUpdateCells) {
+      delete e;
+      return efail(HYQ_EINVAL, "more than 2^29 - 1 cells in all runs");
+    }
   }
   e->P = off;
   const size_t fsz = prec == HYQ_PREC_FP64 ? sizeof(double) : sizeof(float);

@@ -245,8 +245,11 @@ __global__ __launch_bounds__(kBlock) void k_hll_lines(const double* __restrict__
   if (c >= N) return;
   const int s = (int)(c % np);
   const UpdDir<double> d = upd_dir(vmin + c, vmax + c, s, np, 1);
+  const long long l0 = c - s;  // the line's first cell: offsets within the line
+  const UpdOff o = upd_off<double, double>(d, (unsigned)s);
   for (int n = 0; n < 35; ++n)
-    out[(size_t)n * N + c] = upd_value(d, M + (size_t)n * N + c, F + (size_t)n * N + c, dtdx);
+    out[(size_t)n * N + c] =
+        upd_value(d, upd_load(o, M + (size_t)n * N + l0, F + (size_t)n * N + l0), dtdx);
 }
 
 // eigvals of jacobian6 for a 15-moment slice (jacobian6.jl argument order), sorted by real
@@ -444,7 +447,7 @@ struct FastLists {
   int* cells;          // [N] cells for the full path
   int* slot_cell;      // [N] cell of each slot
   unsigned* slot_mask; // [N] pending planes (bits 0..4)
-  unsigned* slot_bad;  // [N] nonzero: a pending plane has a complex pair or is not finite
+  unsigned* slot_bad;  // [N] kSlotBad: a pending plane has a complex pair or is not finite
   uint32_t* slot_fl;   // [N] the cell's flags
   double* slot_v;      // [16][N] pass 1: plane extremes (10), 1D speeds (6)
   double* slot_A;      // [35][N] the projected state the planes come from
@@ -454,7 +457,7 @@ struct FastLists {
   unsigned* full_known;  // [N] pass 2: the decisions of its x/y planes on full_S (known_of)
 };
 // A slot whose pending planes include a complex pair or a non-finite plane: slot_bad holds
-// this bit and the decisions the plane solves reached (known_of).
+// this bit; its low bits hold the decisions the plane solves reached (known_of).
 constexpr unsigned kSlotBad = 0x80000000u;
 // Appends cell c with its projected state A to the full path's list; returns its index.
 __device__ __forceinline__ unsigned new_hard(const FastLists& L, unsigned* count, long long c,
@@ -549,43 +552,51 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_fast(
 }
 
 // The pending planes of every slot: the fast solve with the 2x2 block test of a nearly double
-// pair, the QR where that does not decide.  VALS: pass 1 (extremes into the slot; P: the
-// network's seeds or null); pass 2 only needs the complex-pair decision.
+// pair, the QR where that does not decide.  One item per (slot, plane), items spread over
+// every wave of the grid (ListSpread), one plane per lane: the kernel's time is one plane
+// solve's latency, not that of a cell's pending planes in turn.  A plane's results go to
+// the slot: its extremes to slot_v (pass 1), its flags and iterations to slot_fl by atomic
+// OR and add (the iteration field, bits 16-27, holds at most 5 x 2 x 40 per pass and the
+// fast kernel's as many, far below its 4095 cap), its decision (known_of) to slot_bad, whose
+// kSlotBad bit marks a complex pair or a non-finite plane.  VALS: pass 1 (extremes; P: the
+// network's seeds or null); pass 2 only needs the decisions.
 template <bool VALS>
 __global__ __launch_bounds__(kBlock, kSolveWaves) void k_slot_solve(FastLists L, const unsigned* count,
                                                                     unsigned* nplanes,
                                                                     const float* __restrict__ P,
                                                                     DevScalars* sc, long long N) {
   if (sc->active == 0.0) return;
-  const unsigned n = *count;
-  for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-    atomicAdd(nplanes, (unsigned)__popc(L.slot_mask[s]));
-    double A[35], v[10];
+  constexpr unsigned NP = VALS ? 5u : 4u;  // planes per slot: pass 2 checks x and y only
+  const ListSpread ls(*count * NP);
+  if (ls.idle()) return;
+  for (unsigned i = ls.first(); i < ls.n; i += ls.stride) {
+    if (!ls.mine(i)) continue;
+    const unsigned s = i / NP, q = i % NP;
+    const unsigned pend = L.slot_mask[s];
+    if (q == 0) atomicAdd(nplanes, (unsigned)__popc(pend));
+    if (!(pend & (1u << q))) continue;
+    double A[35];
 #pragma unroll
     for (int t = 0; t < 35; ++t) A[t] = L.slot_A[(size_t)t * N + s];
-    Warm w = no_warm();
-    if (VALS) {
-#pragma unroll
-      for (int t = 0; t < 10; ++t) v[t] = L.slot_v[(size_t)t * N + s];
-      if (P != nullptr) {
-        const long long c = L.slot_cell[s];
-#pragma unroll
-        for (int q = 0; q < 5; ++q) {
-          w.w[2 * q] = P[((size_t)q * N + c) * 2];
-          w.w[2 * q + 1] = P[((size_t)q * N + c) * 2 + 1];
-        }
-      }
+    float wmin = NAN, wmax = NAN;
+    if (VALS && P != nullptr) {
+      const long long c = L.slot_cell[s];
+      wmin = P[((size_t)q * N + c) * 2];
+      wmax = P[((size_t)q * N + c) * 2 + 1];
     }
-    uint32_t fl = VALS ? L.slot_fl[s] : 0u;
-    unsigned known = 0u;
-    if (!solve_pending_planes(A, L.slot_mask[s], VALS, w, v, fl, &known)) {
-      L.slot_bad[s] = kSlotBad | known;
+    uint32_t fl = 0;
+    double v0 = 0.0, v1 = 0.0;
+    const unsigned d = solve_plane(A, (int)q, VALS, wmin, wmax, v0, v1, fl);
+    if (d != kKnownReal) {
+      atomicOr(&L.slot_bad[s], kSlotBad | (d << (2 * q)));
       continue;
     }
+    if (NP == 4) atomicOr(&L.slot_bad[s], d << (2 * q));  // pass 2: the decision for the full path
     if (VALS) {
-#pragma unroll
-      for (int t = 0; t < 10; ++t) L.slot_v[(size_t)t * N + s] = v[t];
-      L.slot_fl[s] = fl;
+      L.slot_v[(size_t)(2 * q) * N + s] = v0;
+      L.slot_v[(size_t)(2 * q + 1) * N + s] = v1;
+      atomicOr(&L.slot_fl[s], fl & 0xFFFFu);
+      atomicAdd(&L.slot_fl[s], fl & 0xFFFF0000u);
     }
   }
 }
@@ -600,7 +611,7 @@ __global__ __launch_bounds__(kBlock) void k_pass1_fixup(FastLists L, FT* __restr
   unsigned long long vb = 0;
   for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
     const long long c = L.slot_cell[s];
-    if (L.slot_bad[s]) {
+    if (L.slot_bad[s] & kSlotBad) {
       atomicAdd(&sc->ncplx1, 1u);
       slot_to_hard(L, &sc->nhard1, s, c, flags[c], N);  // flags[c]: those of the projection
       continue;
@@ -784,7 +795,7 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_fixup(double* __re
   for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
     const long long c = L.slot_cell[s];
     const unsigned bad = L.slot_bad[s];
-    if (bad) {
+    if (bad & kSlotBad) {
       atomicAdd(&sc->ncplx2, 1u);
       const unsigned i = slot_to_hard(L, &sc->nhard2, s, c, 0u, N);
       // the planes that were not pending were proven hyperbolic by the check
@@ -1045,8 +1056,8 @@ int hyq_create(const hyq_params* params, hyq_ctx** out) {
   const hyq_params& p = *params;
   if (p.Nmom != 35) return fail(HYQ_EINVAL, "Nmom must be 35");
   if (p.Nx < 1 || p.Ny < 1 || p.Nz < 1) return fail(HYQ_EINVAL, "grid sizes must be >= 1");
-  if ((long long)p.Nx * p.Ny * p.Nz > 0x7FFFFFFFLL)
-    return fail(HYQ_EINVAL, "more than 2^31 - 1 cells");
+  if ((long long)p.Nx * p.Ny * p.Nz >= kMaxUpdateCells)
+    return fail(HYQ_EINVAL, "more than 2^29 - 1 cells");
   if (p.precision < HYQ_PREC_FP64 || p.precision > HYQ_PREC_MIXED_NEWTON)
     return fail(HYQ_EINVAL, "unknown precision mode");
   int ndev = 0;
@@ -1648,6 +1659,7 @@ int hyq_hll_lines(const double* M, const double* F, const double* vmin, const do
                   int64_t nlines, int32_t np, double dt, double dx, double* Mout) {
   if (!M || !F || !vmin || !vmax || !Mout || nlines < 0 || np < 1 || !(dx > 0))
     return fail(HYQ_EINVAL, "bad argument");
+  if (np >= kMaxUpdateCells) return fail(HYQ_EINVAL, "line longer than 2^29 - 1 cells");
   const long long n = (long long)nlines * np;
   if (n == 0) return HYQ_OK;
   hipLaunchKernelGGL(k_hll_lines, dim3(grid_for(n)), dim3(kBlock), 0, 0, M, F, vmin, vmax, n,

@@ -17,6 +17,13 @@
 #else  // host build of the cell numerics (CPU test harness only)
 #define HD inline
 #endif
+// Keeps the instruction scheduler from moving instructions across this point (device code;
+// nothing on the host): loads placed before it are all issued before any use after it.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define HYQ_SCHED_BARRIER() __builtin_amdgcn_sched_barrier(0)
+#else
+#define HYQ_SCHED_BARRIER() ((void)0)
+#endif
 
 namespace hyq {
 

@@ -462,38 +462,29 @@ HD int pass2_check_planes(const double* M, int flag2D, double Ma, double* A, uns
   double v[10];
   return fast_cell_planes<PAIR, false>(A, false, no_warm(), v, pend, fl, known);
 }
-// The full decision of the planes in pend (fast solve with the pair test, else QR) of a
-// cell whose other planes the small fast kernel already proved: v[2q], v[2q+1] set for
-// each such plane; returns false when one of them has a complex pair (or is not finite),
-// i.e. the cell needs the eigenvalue correction's full path.
-HD bool solve_pending_planes(const double* A, unsigned pend, bool need_vals, const Warm& w,
-                             double v[10], uint32_t& fl, unsigned* known = nullptr) {
+// The full decision of plane q of projected state A (the fast solve with the pair test, else
+// the QR): kKnownReal, with (v0, v1) its extremes when need_vals; kKnownComplex; or kKnownNone
+// when the plane is not finite.  Either of the last two sends the cell to the full path.
+HD unsigned solve_plane(const double* A, int q, bool need_vals, float wmin, float wmax,
+                        double& v0, double& v1, uint32_t& fl) {
   double c[35], M000, um, vm, wm;
   raw_to_central4(A, c, M000, um, vm, wm);
   const float nan = NAN;
-#pragma unroll
-  for (int q = 0; q < 5; ++q) {
-    if (!(pend & (1u << q))) continue;
-    const Plane p = cell_plane(c, um, vm, wm, q);
-    const int st = plane_fast_status<true>(p, need_vals ? w.w[2 * q] : nan,
-                                           need_vals ? w.w[2 * q + 1] : nan, need_vals, fl,
-                                           v[2 * q], v[2 * q + 1]);
-    bool cplx = st == kPlaneComplex;
-    if (st == kPlaneFull) return false;
-    if (st == kPlaneSolve) {
-      fl |= F_FALLBACK;
-      const EigRes r = plane_eigs(p, fl);
-      if (!r.ok) return false;
-      cplx = r.cplx;
-      v[2 * q] = r.vmin;
-      v[2 * q + 1] = r.vmax;
-    }
-    if (known != nullptr) *known |= (cplx ? kKnownComplex : kKnownReal) << (2 * q);
-    if (cplx) return false;
+  const Plane p = cell_plane(c, um, vm, wm, q);
+  const int st = plane_fast_status<true>(p, need_vals ? wmin : nan, need_vals ? wmax : nan,
+                                         need_vals, fl, v0, v1);
+  if (st == kPlaneFull) return kKnownNone;
+  if (st == kPlaneComplex) return kKnownComplex;
+  if (st == kPlaneSolve) {
+    fl |= F_FALLBACK;
+    const EigRes r = plane_eigs(p, fl);
+    if (!r.ok) return kKnownNone;
+    v0 = r.vmin;
+    v1 = r.vmax;
+    return r.cplx ? kKnownComplex : kKnownReal;
   }
-  return true;
+  return kKnownReal;
 }
-
 // Pass 2's x or y check of a full-path cell whose two planes (UV, UW or VU, VW) were already
 // decided on this state (known, as fast_cell_planes returns it): the correction of the planes
 // with a complex pair, without solving the planes again, or a copy when there is none; when

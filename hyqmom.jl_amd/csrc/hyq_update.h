@@ -45,29 +45,28 @@ HD FaceHLL<float> face_hll(float vl0, float vl1, float vh0, float vh1) {
 // 1: two cells (both take cell 1); 2: general.
 template <typename T>
 struct UpdDir {
-  long long o0, o1, o2;
+  int o0, o1, o2;  // a neighbour is at most one z plane away: |offset| <= Nx * Ny < 2^31
   int mode;
   bool first, last;
   FaceHLL<T> A, B;
 };
 // vl/vh: the line's lower/upper wave-speed planes at this cell's position.
 template <typename T>
-HD UpdDir<T> upd_dir(const T* vl, const T* vh, int s, int n, long long st) {
+HD UpdDir<T> upd_dir(const T* vl, const T* vh, int s, int n, int st) {
   UpdDir<T> d;
   d.mode = n == 1 ? 0 : (n == 2 ? 1 : 2);
   d.first = d.last = false;
-  if (d.mode == 0) {
-    d.o0 = d.o1 = d.o2 = 0;
-    return d;
-  }
-  if (d.mode == 1) {
-    d.o0 = d.o1 = d.o2 = (long long)(1 - s) * st;
+  if (d.mode != 2) {
+    // the cell keeps its own value (one cell) or takes cell 1's (two cells); the faces are
+    // not used, but upd_value evaluates them, so they hold zeros
+    d.o0 = d.o1 = d.o2 = d.mode == 0 ? 0 : (1 - s) * st;
+    d.A = d.B = FaceHLL<T>{T(0), T(0), T(0), T(0), T(0), false};
     return d;
   }
   const int j = s < 1 ? 1 : (s > n - 2 ? n - 2 : s);
-  d.o0 = (long long)(j - 1 - s) * st;
-  d.o1 = (long long)(j - s) * st;
-  d.o2 = (long long)(j + 1 - s) * st;
+  d.o0 = (j - 1 - s) * st;
+  d.o1 = (j - s) * st;
+  d.o2 = (j + 1 - s) * st;
   d.first = j == 1;
   d.last = j == n - 2;
   const T l0 = vl[d.o0], l1 = vl[d.o1], l2 = vl[d.o2];
@@ -86,15 +85,13 @@ HD double upd_core(const UpdDir<double>& d, double m0, double m1, double m2, FT 
   const double f1 = (double)f1_, f2 = (double)f2_;
   const double wsB = d.B.sep ? (d.B.ll * m1 - d.B.lr * m2) * d.B.inv - (f1 - f2) * d.B.inv : 0.0;
   const double fB = 0.5 * (f1 + (d.last ? f1 : f2)) - 0.5 * (d.B.all * (wsB - m1) - d.B.alr * (wsB - m2));
-  double fA;
-  if (d.first) {
-    // face 0: zero wave speeds, W* of face 1, F[1] := F[2]
-    fA = 0.5 * (f1 + f1) - 0.5 * (0.0 * (wsB - m0) - 0.0 * (wsB - m1));
-  } else {
-    const double f0 = (double)f0_;
-    const double wsA = d.A.sep ? (d.A.ll * m0 - d.A.lr * m1) * d.A.inv - (f0 - f1) * d.A.inv : 0.0;
-    fA = 0.5 * (f0 + f1) - 0.5 * (d.A.all * (wsA - m0) - d.A.alr * (wsA - m1));
-  }
+  // face 0 (d.first): zero wave speeds, W* of face 1, F[1] := F[2]; both forms are evaluated
+  // and selected, so that no load waits behind a branch
+  const double fA0 = 0.5 * (f1 + f1) - 0.5 * (0.0 * (wsB - m0) - 0.0 * (wsB - m1));
+  const double f0 = (double)f0_;
+  const double wsA = d.A.sep ? (d.A.ll * m0 - d.A.lr * m1) * d.A.inv - (f0 - f1) * d.A.inv : 0.0;
+  const double fAg = 0.5 * (f0 + f1) - 0.5 * (d.A.all * (wsA - m0) - d.A.alr * (wsA - m1));
+  const double fA = d.first ? fA0 : fAg;
   return m1 - dtdx * (fB - fA);
 }
 
@@ -107,47 +104,91 @@ HD double upd_core(const UpdDir<float>& d, double m0, double m1, double m2, floa
   const float w0B = d.B.sep ? (d.B.lr * dmB - (f1 - f2)) * d.B.inv : -(float)m1;
   const float w1B = w0B + dmB;
   const float fB = 0.5f * (f1 + (d.last ? f1 : f2)) - 0.5f * (d.B.all * w0B - d.B.alr * w1B);
-  float fA;
-  if (d.first) {
-    fA = f1;  // zero wave speeds on face 0: the flux of cell 1
-  } else {
-    const float dmA = (float)(m0 - m1);
-    const float w0A = d.A.sep ? (d.A.lr * dmA - (f0 - f1)) * d.A.inv : -(float)m0;
-    const float w1A = w0A + dmA;
-    fA = 0.5f * (f0 + f1) - 0.5f * (d.A.all * w0A - d.A.alr * w1A);
-  }
+  const float dmA = (float)(m0 - m1);
+  const float w0A = d.A.sep ? (d.A.lr * dmA - (f0 - f1)) * d.A.inv : -(float)m0;
+  const float w1A = w0A + dmA;
+  const float fAg = 0.5f * (f0 + f1) - 0.5f * (d.A.all * w0A - d.A.alr * w1A);
+  const float fA = d.first ? f1 : fAg;  // zero wave speeds on face 0: the flux of cell 1
   const float inc = -(float)dtdx * (fB - fA);
   return m1 + (double)inc;
 }
 
-// Updated value of one moment along one direction.  m, f: the moment's state and flux
-// planes at this cell's position.
+// Byte offsets, within a state plane (FP64) and a flux plane (FT), of one direction's cells
+// j-1, j, j+1 (f0: cell j for face 0, whose flux is F[1] := F[2]).  32 bits: a plane holds at
+// most kMaxUpdateCells cells.
+constexpr long long kMaxUpdateCells = 1ll << 29;
+struct UpdOff {
+  unsigned m0, m1, m2, f0, f1, f2;
+};
 template <typename T, typename FT>
-HD double upd_value(const UpdDir<T>& d, const double* m, const FT* f, double dtdx) {
-  if (d.mode == 0) return m[0];
-  if (d.mode == 1) return m[d.o1];
-  return upd_core(d, m[d.o0], m[d.o1], m[d.o2], d.first ? f[d.o1] : f[d.o0], f[d.o1], f[d.o2], dtdx);
+HD UpdOff upd_off(const UpdDir<T>& d, unsigned c) {
+  UpdOff r;
+  r.m0 = (c + (unsigned)d.o0) * 8u;
+  r.m1 = (c + (unsigned)d.o1) * 8u;
+  r.m2 = (c + (unsigned)d.o2) * 8u;
+  r.f0 = (c + (unsigned)(d.first ? d.o1 : d.o0)) * (unsigned)sizeof(FT);
+  r.f1 = (c + (unsigned)d.o1) * (unsigned)sizeof(FT);
+  r.f2 = (c + (unsigned)d.o2) * (unsigned)sizeof(FT);
+  return r;
+}
+template <typename T>
+HD T at_byte(const T* plane, unsigned off) {
+  return *(const T*)((const char*)plane + off);
+}
+
+// One moment's values along one direction: state and flux at cells j-1, j, j+1.
+template <typename FT>
+struct UpdVals {
+  double m0, m1, m2;
+  FT f0, f1, f2;
+};
+// The six loads are made in every mode, from offsets that stay inside the line (the mode
+// selects the result in upd_value): no load sits behind a branch.
+template <typename FT>
+HD UpdVals<FT> upd_load(const UpdOff& o, const double* m, const FT* f) {
+  UpdVals<FT> v;
+  v.m0 = at_byte(m, o.m0);
+  v.m1 = at_byte(m, o.m1);
+  v.m2 = at_byte(m, o.m2);
+  v.f0 = at_byte(f, o.f0);
+  v.f1 = at_byte(f, o.f1);
+  v.f2 = at_byte(f, o.f2);
+  return v;
+}
+// Updated value of one moment along one direction.
+template <typename T, typename FT>
+HD double upd_value(const UpdDir<T>& d, const UpdVals<FT>& v, double dtdx) {
+  const double r = upd_core(d, v.m0, v.m1, v.m2, v.f0, v.f1, v.f2, dtdx);
+  return d.mode == 2 ? r : v.m1;
 }
 
 // Combined x/y/z update of cell c (i, j, k) for all 35 moments: Mx + My + Mz - 2M
-// (simulation_runner.jl:516-520); the z sweep is skipped when Nz == 1
-// (apply_flux_update_3d.jl:183).  Mr: state planes [35][N]; F: flux planes [105][N]
-// (x, y, z blocks); vp: wave-speed planes [6][N]; out: [35][N]; N is the plane stride
-// (a grid's cells may be a contiguous part of longer planes), c the cell's plane index.
+// (simulation_runner.jl:516-520); when Nz == 1 the z sweep leaves the value unchanged
+// (apply_flux_update_3d.jl:183), as a line of one cell does.  Mr: state planes [35][N]; F:
+// flux planes [105][N] (x, y, z blocks); vp: wave-speed planes [6][N]; out: [35][N]; N is
+// the plane stride (a grid's cells may be a contiguous part of longer planes; at most
+// kMaxUpdateCells), c the cell's plane index.  Each moment's 19 loads are issued together,
+// before any of them is used, so the moment waits for memory once.
 template <typename FT>
 HD void update_cell(const double* Mr, const FT* F, const FT* vp, double* out, long long c, int i,
                     int j, int k, int Nx, int Ny, int Nz, long long N, double dtdx, double dtdy,
                     double dtdz) {
-  const long long sy = Nx, sz = (long long)Nx * Ny;
+  const int sy = Nx, sz = Nx * Ny;
   const UpdDir<FT> dx = upd_dir(vp + c, vp + N + c, i, Nx, 1);
   const UpdDir<FT> dy = upd_dir(vp + 2 * N + c, vp + 3 * N + c, j, Ny, sy);
   const UpdDir<FT> dz = upd_dir(vp + 4 * N + c, vp + 5 * N + c, k, Nz, sz);
+  const unsigned c32 = (unsigned)c;
+  const UpdOff ox = upd_off<FT, FT>(dx, c32), oy = upd_off<FT, FT>(dy, c32), oz = upd_off<FT, FT>(dz, c32);
   for (int n = 0; n < 35; ++n) {
-    const double* m = Mr + (size_t)n * N + c;
-    const double m0 = m[0];
-    const double mx = upd_value(dx, m, F + (size_t)n * N + c, dtdx);
-    const double my = upd_value(dy, m, F + (size_t)(35 + n) * N + c, dtdy);
-    const double mz = (Nz == 1) ? m0 : upd_value(dz, m, F + (size_t)(70 + n) * N + c, dtdz);
+    const double* m = Mr + (size_t)n * N;
+    const double m0 = at_byte(m, c32 * 8u);
+    const UpdVals<FT> vx = upd_load(ox, m, F + (size_t)n * N);
+    const UpdVals<FT> vy = upd_load(oy, m, F + (size_t)(35 + n) * N);
+    const UpdVals<FT> vz = upd_load(oz, m, F + (size_t)(70 + n) * N);
+    HYQ_SCHED_BARRIER();
+    const double mx = upd_value(dx, vx, dtdx);
+    const double my = upd_value(dy, vy, dtdy);
+    const double mz = upd_value(dz, vz, dtdz);  // m0 when Nz == 1
     out[(size_t)n * N + c] = mx + my + mz - 2.0 * m0;
   }
 }

@@ -88,7 +88,7 @@ typedef struct hyq_params {
 
 typedef struct hyq_ctx hyq_ctx;
 
-/* Context: device SoA planes, stream and scratch for one run. */
+/* Context: device SoA planes, stream and scratch for one run (at most 2^29 - 1 cells). */
 int hyq_create(const hyq_params* params, hyq_ctx** out);
 int hyq_destroy(hyq_ctx* ctx);
 
@@ -200,7 +200,8 @@ int hyq_closure1d_batch(const double* m5, int64_t n, double* vmin, double* vmax)
 
 /* pas_HLL (src/numerics/pas_HLL.jl:24-84 with flux_HLL.jl:20-32) on nlines independent lines
  * of np cells: M, F, Mout [35][nlines*np] (cell s of line l at l*np + s), wave-speed bounds
- * vmin, vmax [nlines*np], Mout = the line's updated state with the reference's BC copies. */
+ * vmin, vmax [nlines*np], Mout = the line's updated state with the reference's BC copies;
+ * np at most 2^29 - 1. */
 int hyq_hll_lines(const double* M, const double* F, const double* vmin, const double* vmax,
                   int64_t nlines, int32_t np, double dt, double dx, double* Mout);
 
@@ -256,7 +257,7 @@ int hyq_plane_dataset(const double* M, int64_t n, float* X, float* L, uint8_t* v
  * CFL dt and loop condition t < tmax.  No counterpart in the reference, which runs one
  * simulation per process (simulation_runner.jl:63). */
 typedef struct hyq_ensemble hyq_ensemble;
-int hyq_ensemble_create(const hyq_params* runs, int32_t nruns, hyq_ensemble** out);
+int hyq_ensemble_create(const hyq_params* runs, int32_t nruns, hyq_ensemble** out);  // at most 2^29 - 1 cells in all
 int hyq_ensemble_destroy(hyq_ensemble* e);
 /* state of one run, host Julia layout (Nx,Ny,Nz,35); set_state resets its t and nn */
 int hyq_ensemble_set_state(hyq_ensemble* e, int32_t run, const double* M);
