@@ -211,12 +211,17 @@ HD int pair_block_decision(const double* J, double z, double sig) {
       }
     piv[k] = pk;
     if (!(best > 0.0)) return -1;
-    if (pk != k) {
+    // exchange rows k and pk by selects over the candidate rows: every index stays a
+    // compile-time constant, so A stays in registers (an A[pk * 6 + j] with a run-time pk
+    // would put the matrix in scratch memory)
+#pragma unroll
+    for (int i = k + 1; i < 6; ++i) {
+      const bool sw = i == pk;
 #pragma unroll
       for (int j = 0; j < 6; ++j) {
-        const double t = A[k * 6 + j];
-        A[k * 6 + j] = A[pk * 6 + j];
-        A[pk * 6 + j] = t;
+        const double a = A[k * 6 + j], b = A[i * 6 + j];
+        A[k * 6 + j] = sw ? b : a;
+        A[i * 6 + j] = sw ? a : b;
       }
     }
     const double inv = 1.0 / A[k * 6 + k];
