@@ -409,19 +409,21 @@ __global__ __launch_bounds__(kBlock, kPassWaves) void k_pass1a(const double* __r
 // `per` consecutive entries (per = ceil(n / waves), at most 64) on its first `per` lanes, and
 // the grid strides by waves * per.  Few cells per wave keep the union of their branch paths,
 // which a wave executes, short; the deferred cells are the rare, branch-heavy ones.
+// LANES lanes per entry (1, or 2: a lane pair that shares one cell's work, see PairSolve).
+template <unsigned LANES = 1>
 struct ListSpread {
   unsigned n, per, wave, lane, stride;
   __device__ explicit ListSpread(unsigned n_) : n(n_) {
     const unsigned waves = gridDim.x * (blockDim.x / 64);
     per = (n + waves - 1) / waves;
-    per = per < 1 ? 1 : (per > 64 ? 64 : per);
+    per = per < 1 ? 1 : (per > 64 / LANES ? 64 / LANES : per);
     wave = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-    lane = threadIdx.x & 63;
+    lane = (threadIdx.x & 63) / LANES;
     stride = waves * per;
   }
   __device__ bool idle() const { return wave * per >= n; }  // the same for every lane of a wave
   __device__ unsigned first() const { return wave * per + lane; }
-  __device__ bool mine(unsigned i) const { return lane < per && i < n; }
+  __device__ bool mine(unsigned i) const { return lane < per && i < n; }  // same on a pair
 };
 // Waves per SIMD of the list kernels (the full path's stages, the slot solver): their lists
 // are short, so each kernel's time is one wave's latency through its stage, which is
@@ -430,6 +432,9 @@ struct ListSpread {
 // every wave with work runs at once, in 128-thread blocks.
 constexpr int kFullWaves = 1, kSolveWaves = 1;
 constexpr unsigned kListBlocks = 256 * 4 * kFullWaves / 2;
+// The fix-up kernels finish the slots (up to a quarter of the cells in the middle of a run)
+// with short, converged code: two waves per SIMD, a grid-stride loop over the slots.
+constexpr unsigned kFixupBlocks = 256 * 4 * kFastWaves / 2;
 
 // ---- the fast step (mixed_newton / mixed_learned modes) ------------------------------------
 // A cell whose planes the fast solve proves hyperbolic finishes in the fast kernel.  A plane
@@ -567,7 +572,7 @@ __global__ __launch_bounds__(kBlock, kSolveWaves) void k_slot_solve(FastLists L,
                                                                     DevScalars* sc, long long N) {
   if (sc->active == 0.0) return;
   constexpr unsigned NP = VALS ? 5u : 4u;  // planes per slot: pass 2 checks x and y only
-  const ListSpread ls(*count * NP);
+  const ListSpread<> ls(*count * NP);
   if (ls.idle()) return;
   for (unsigned i = ls.first(); i < ls.n; i += ls.stride) {
     if (!ls.mine(i)) continue;
@@ -653,6 +658,124 @@ __device__ __forceinline__ Warm list_warm(const float* P, long long c, long long
   }
   return w;
 }
+// ---- lane pairs: the two plane solves of an axis side by side -------------------------------
+// A full-path cell's stage is a chain of latencies (one wave per SIMD, a few cells per
+// wave), and each axis solves its two planes one after the other.  In pass 1's stage
+// kernels a lane pair shares a cell: both lanes hold the same state and run the same code, except
+// that lane `half` = 0 solves the axis's first plane and lane 1 its second, and the two
+// exchange their results (and the flags their solves set) by a lane swap.  The values, the
+// decisions and the flags are those of eig6_axis / eig6_z: the same plane_solve on the same
+// planes, the flags OR-ed, the root-polishing iterations added (the field cannot reach its
+// 4095 cap: at most 5 planes x 2 solves x 80 per pass, and as many in the fast kernel).
+__device__ __forceinline__ EigRes swap_res(const EigRes& r) {
+  EigRes o;
+  o.vmin = __shfl_xor(r.vmin, 1);
+  o.vmax = __shfl_xor(r.vmax, 1);
+  o.cplx = __shfl_xor((int)r.cplx, 1) != 0;
+  o.ok = __shfl_xor((int)r.ok, 1) != 0;
+  return o;
+}
+HD uint32_t merge_flags(uint32_t fl, uint32_t fa, uint32_t fb) {
+  const uint32_t it = (fl >> 16) + (fa >> 16) + (fb >> 16);
+  return ((fl | fa | fb) & 0xFFFFu) | ((it < 0xFFFu ? it : 0xFFFu) << 16);
+}
+// Planes a and b solved on the two lanes of a pair; A, B: both results on both lanes.
+template <bool CORRECTS>
+__device__ __forceinline__ void pair_solve(const Plane& pa, const Plane& pb, float wa0, float wa1,
+                                           float wb0, float wb1, bool need_vals, uint32_t& fl,
+                                           int half, EigRes& A, EigRes& B) {
+  uint32_t f = 0;
+  const EigRes r = plane_solve<1, CORRECTS>(half ? pb : pa, half ? wb0 : wa0, half ? wb1 : wa1,
+                                            need_vals, f);
+  const EigRes o = swap_res(r);
+  const uint32_t fo = (uint32_t)__shfl_xor((int)f, 1);
+  A = half ? o : r;
+  B = half ? r : o;
+  fl = merge_flags(fl, half ? fo : f, half ? f : fo);
+}
+// eig6_axis<AXIS, 1> on a lane pair.
+template <int AXIS>
+__device__ bool eig6_axis_pair(const double* M, int flag2D, double Ma, double& vmin, double& vmax,
+                               double* Mr, uint32_t& fl, const Warm& wm_, bool need_vals, int half) {
+  double c[35];
+  double M000, um, vm, wm;
+  raw_to_central4(M, c, M000, um, vm, wm);
+  const int ia = AXIS == 0 ? 0 : 2, ib = AXIS == 0 ? 1 : 3;  // Warm slots: UV UW VU VW WU
+  EigRes A, B;
+  pair_solve<true>(AXIS == 0 ? plane<0, 1>(c, um, vm, wm) : plane<1, 0>(c, um, vm, wm),
+                   AXIS == 0 ? plane<0, 2>(c, um, vm, wm) : plane<1, 2>(c, um, vm, wm),
+                   wm_.w[2 * ia], wm_.w[2 * ia + 1], wm_.w[2 * ib], wm_.w[2 * ib + 1], need_vals, fl,
+                   half, A, B);
+  combine_guarded(A, B, vmin, vmax);
+  if (A.cplx || B.cplx) {
+    axis_correct<AXIS>(c, M000, um, vm, wm, A.cplx, B.cplx, flag2D, Ma, Mr, fl);
+    if (!need_vals) return true;
+    raw_to_central4(Mr, c, M000, um, vm, wm);
+    pair_solve<false>(AXIS == 0 ? plane<0, 1>(c, um, vm, wm) : plane<1, 0>(c, um, vm, wm),
+                      AXIS == 0 ? plane<0, 2>(c, um, vm, wm) : plane<1, 2>(c, um, vm, wm),
+                      wm_.w[2 * ia], wm_.w[2 * ia + 1], wm_.w[2 * ib], wm_.w[2 * ib + 1], need_vals,
+                      fl, half, A, B);
+    combine_guarded(A, B, vmin, vmax);
+    return true;
+  }
+#pragma unroll
+  for (int n = 0; n < 35; ++n) Mr[n] = M[n];
+  return false;
+}
+// eig6_z<1> on a lane pair: the density clamp and the re-projection on both lanes, the WU
+// and VW solves side by side.
+__device__ void eig6_z_pair(const double* M, int flag2D, double Ma, bool positivity, double& vmin,
+                            double& vmax, double* Mr, uint32_t& fl, const Warm& wm_, int half) {
+#pragma unroll
+  for (int n = 0; n < 35; ++n) Mr[n] = M[n];
+  if (positivity && M[0] <= 0.0) {
+    Mr[0] = jmax(M[0], 1e-14);
+    fl |= F_RHO;
+  }
+  double W[35];
+#pragma unroll
+  for (int n = 0; n < 35; ++n) W[n] = M[n];
+  W[0] = Mr[0];
+  if (positivity && W[0] > 1e-15) {
+    const double m0 = W[0];
+    const double vx = M[2] / m0 - (M[1] / m0) * (M[1] / m0);
+    const double vy = M[9] / m0 - (M[5] / m0) * (M[5] / m0);
+    const double vz = M[19] / m0 - (M[15] / m0) * (M[15] / m0);
+    if (vx < 0.0 || vy < 0.0 || vz < 0.0) {
+      fl |= F_ZNEG;
+      flux_closure35<false, double>(M, flag2D, Ma, Mr, nullptr, nullptr, nullptr, 0, fl);
+#pragma unroll
+      for (int n = 0; n < 35; ++n) W[n] = Mr[n];
+    }
+  }
+  double c[35];
+  double M000, um, vm, wm;
+  raw_to_central4(W, c, M000, um, vm, wm);
+  EigRes A, B;
+  pair_solve<true>(plane<2, 0>(c, um, vm, wm), plane<1, 2>(c, um, vm, wm), wm_.w[8], wm_.w[9],
+                   wm_.w[6], wm_.w[7], true, fl, half, A, B);
+  if (!A.ok || !B.ok) fl |= F_NONFIN;  // the reference's eigvals throws here
+  vmin = jmin(A.vmin, B.vmin);
+  vmax = jmax(A.vmax, B.vmax);
+  if (A.cplx || B.cplx) {
+    fl |= F_ECZ;
+    raw_to_central4(M, c, M000, um, vm, wm);  // corrections start from the input M
+    double cn[35];
+#pragma unroll
+    for (int t = 0; t < 35; ++t) cn[t] = c[t];
+    if (A.cplx) correct_plane<0, 2>(c, cn);
+    if (B.cplx) correct_plane<1, 2>(c, cn);
+    central_to_raw<4>(cn, M000, um, vm, wm);
+    double Mh[35];
+    pack35(cn, Mh);
+    flux_closure35<false, double>(Mh, flag2D, Ma, Mr, nullptr, nullptr, nullptr, 0, fl);
+    raw_to_central4(Mr, c, M000, um, vm, wm);
+    pair_solve<false>(plane<2, 0>(c, um, vm, wm), plane<1, 2>(c, um, vm, wm), wm_.w[8], wm_.w[9],
+                      wm_.w[6], wm_.w[7], true, fl, half, A, B);
+    vmin = jmin(A.vmin, B.vmin);
+    vmax = jmax(A.vmax, B.vmax);
+  }
+}
 // The list holds each cell's projected state (pass1a_cell, computed once by k_pass1a) and
 // the flags of that projection.  STAGE 1: x; 2: y; 3: z.
 template <int STAGE>
@@ -660,8 +783,9 @@ __global__ __launch_bounds__(kBlock, kFullWaves) void k_full1_stage(
     const float* __restrict__ P, FastLists L, DevScalars* sc, long long N, int flag2D, double Ma,
     int positivity) {
   if (sc->active == 0.0) return;
-  const ListSpread ls(sc->nhard1);
+  const ListSpread<2> ls(sc->nhard1);  // a lane pair per cell
   if (ls.idle()) return;  // wave-uniform
+  const int half = threadIdx.x & 1;
   for (unsigned i = ls.first(); i < ls.n; i += ls.stride) {
     if (!ls.mine(i)) continue;
     const long long c = L.cells[i];
@@ -669,15 +793,14 @@ __global__ __launch_bounds__(kBlock, kFullWaves) void k_full1_stage(
 #pragma unroll
     for (int t = 0; t < 35; ++t) a[t] = L.full_S[(size_t)t * N + i];
     uint32_t fl = L.full_fl[i];
-    {
-      const Warm w = list_warm(P, c, N);
-      double mn, mx;
-      if (STAGE == 1) eig6_axis<0, 1>(a, flag2D, Ma, mn, mx, b, fl, w);
-      if (STAGE == 2) eig6_axis<1, 1>(a, flag2D, Ma, mn, mx, b, fl, w);
-      if (STAGE == 3) eig6_z<1>(a, flag2D, Ma, positivity != 0, mn, mx, b, fl, w);
-      L.full_v[(size_t)(2 * (STAGE - 1)) * N + i] = mn;
-      L.full_v[(size_t)(2 * (STAGE - 1) + 1) * N + i] = mx;
-    }
+    const Warm w = list_warm(P, c, N);
+    double mn, mx;
+    if (STAGE == 1) eig6_axis_pair<0>(a, flag2D, Ma, mn, mx, b, fl, w, true, half);
+    if (STAGE == 2) eig6_axis_pair<1>(a, flag2D, Ma, mn, mx, b, fl, w, true, half);
+    if (STAGE == 3) eig6_z_pair(a, flag2D, Ma, positivity != 0, mn, mx, b, fl, w, half);
+    if (half) continue;  // both lanes hold the same results
+    L.full_v[(size_t)(2 * (STAGE - 1)) * N + i] = mn;
+    L.full_v[(size_t)(2 * (STAGE - 1) + 1) * N + i] = mx;
 #pragma unroll
     for (int t = 0; t < 35; ++t) L.full_S[(size_t)t * N + i] = b[t];
     L.full_fl[i] = fl;
@@ -689,7 +812,7 @@ __global__ __launch_bounds__(kBlock, kFullWaves) void k_full1_finish(
     double* __restrict__ AMr, FT* __restrict__ F, FT* __restrict__ vp, uint32_t* __restrict__ flags,
     FastLists L, DevScalars* sc, long long N, int flag2D, double Ma) {
   if (sc->active == 0.0) return;
-  const ListSpread ls(sc->nhard1);
+  const ListSpread<> ls(sc->nhard1);
   if (ls.idle()) return;
   unsigned long long vb = 0;
   for (unsigned i = ls.first(); i < ls.n; i += ls.stride) {
@@ -825,7 +948,7 @@ __global__ __launch_bounds__(kBlock, kFullWaves) void k_full2_stage(FastLists L,
                                                                    long long N, int flag2D,
                                                                    double Ma) {
   if (sc->active == 0.0) return;
-  const ListSpread ls(sc->nhard2);
+  const ListSpread<> ls(sc->nhard2);
   if (ls.idle()) return;
   for (unsigned i = ls.first(); i < ls.n; i += ls.stride) {
     if (!ls.mine(i)) continue;
@@ -834,13 +957,11 @@ __global__ __launch_bounds__(kBlock, kFullWaves) void k_full2_stage(FastLists L,
 #pragma unroll
     for (int t = 0; t < 35; ++t) a[t] = L.full_S[(size_t)t * N + i];
     const unsigned known = L.full_known[i];
-    // the wave speeds of pass 2 are not used, only its corrections
-    if (STAGE == 1) {
-      // after an x correction the y decisions no longer hold
-      if (eig6_axis_decided<0>(a, flag2D, Ma, b, fl, known)) L.full_known[i] = known & 0x0Fu;
-    } else {
-      eig6_axis_decided<1>(a, flag2D, Ma, b, fl, known);
-    }
+    // the wave speeds of pass 2 are not used, only its corrections; after an x correction
+    // the y decisions no longer hold
+    const bool corrected = STAGE == 1 ? eig6_axis_decided<0>(a, flag2D, Ma, b, fl, known)
+                                      : eig6_axis_decided<1>(a, flag2D, Ma, b, fl, known);
+    if (STAGE == 1 && corrected) L.full_known[i] = known & 0x0Fu;
 #pragma unroll
     for (int t = 0; t < 35; ++t) L.full_S[(size_t)t * N + i] = b[t];
   }
@@ -851,7 +972,7 @@ __global__ __launch_bounds__(kBlock, kFullWaves) void k_full2_finish(double* __r
                                                                     double Ma, int positivity,
                                                                     double Kn) {
   if (sc->active == 0.0) return;
-  const ListSpread ls(sc->nhard2);
+  const ListSpread<> ls(sc->nhard2);
   if (ls.idle()) return;
   for (unsigned i = ls.first(); i < ls.n; i += ls.stride) {
     if (!ls.mine(i)) continue;
@@ -1195,7 +1316,7 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
                        (float*)c->vp, c->flags, P, c->L, c->sc, c->N, p.flag2D, p.Ma, p.positivity);
     hipLaunchKernelGGL(k_slot_solve<true>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
                        (const unsigned*)&c->sc->nslot1, &c->sc->nitem1, P, c->sc, c->N);
-    hipLaunchKernelGGL(k_pass1_fixup<float>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
+    hipLaunchKernelGGL(k_pass1_fixup<float>, dim3(kFixupBlocks), dim3(kBlock), 0, c->stream, c->L,
                        (float*)c->vp, c->flags, c->sc, c->N);
     enqueue_full1<float>(c->stream, c->Mr, (float*)c->F, (float*)c->vp, c->flags, P, c->L,
                          c->sc, c->N, p.flag2D, p.Ma, p.positivity);
@@ -1227,7 +1348,7 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
     hipLaunchKernelGGL(k_slot_solve<false>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
                        (const unsigned*)&c->sc->nslot2, &c->sc->nitem2, (const float*)nullptr, c->sc,
                        c->N);
-    hipLaunchKernelGGL(k_pass2_fixup, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->M, c->L,
+    hipLaunchKernelGGL(k_pass2_fixup, dim3(kFixupBlocks), dim3(kBlock), 0, c->stream, c->M, c->L,
                        c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
     hipLaunchKernelGGL(k_full2_stage<1>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
                        c->sc, c->N, p.flag2D, p.Ma);
@@ -1542,7 +1663,7 @@ static void enqueue_fast_batch(const double* M, long long n, double Ma, int flag
                      (const float*)nullptr, L, sc, n, flag2D, Ma, positivity);
   hipLaunchKernelGGL(k_slot_solve<true>, dim3(kListBlocks), dim3(kBlock), 0, 0, L,
                      (const unsigned*)&sc->nslot1, &sc->nitem1, (const float*)nullptr, sc, n);
-  hipLaunchKernelGGL(k_pass1_fixup<double>, dim3(kListBlocks), dim3(kBlock), 0, 0, L, vp, flags, sc, n);
+  hipLaunchKernelGGL(k_pass1_fixup<double>, dim3(kFixupBlocks), dim3(kBlock), 0, 0, L, vp, flags, sc, n);
   enqueue_full1<double>(0, Mr, F, vp, flags, (const float*)nullptr, L, sc, n, flag2D, Ma,
                         positivity);
 }
