@@ -93,4 +93,7 @@ def test_config5_ensemble_against_sequential():
     stepped one after another."""
     from hyqmom_amd.ensemble import compare_sequential, ensemble_runs
     out = compare_sequential(ensemble_runs(64, precision="mixed"), steps=10)
-    assert out["speedup"] > 1.0
+    print("config 5: ensemble %.3g cell-steps/s, sequential %.3g, speedup %.2f"
+          % (out["ensemble_cell_steps_per_s"], out["sequential_cell_steps_per_s"], out["speedup"]))
+    # measured 4.4x; the ensemble's point is to fill the chip that single small runs leave idle
+    assert out["speedup"] > 2.0
