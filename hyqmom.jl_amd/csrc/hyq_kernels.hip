@@ -31,6 +31,9 @@ extern "C" int hyq_mlp_infer_masked_async(hyq_mlp* m, const float* X, int64_t n,
                                           hipStream_t stream);
 extern "C" int hyq_mlp_infer_bf16_masked_async(hyq_mlp* m, const short* Xb, int64_t n, float* P,
                                                hipStream_t stream);
+extern "C" int hyq_mlp_infer_bf16_segments_async(hyq_mlp* m, const short* Xb, const uint32_t* idx,
+                                                 const uint32_t* seg, int64_t nseg, int32_t seg_rows,
+                                                 float* P, hipStream_t stream);
 extern "C" const float* hyq_mlp_input_norm(hyq_mlp* m);
 
 namespace {
@@ -364,20 +367,28 @@ __global__ __launch_bounds__(kBlock) void k_plane_dataset(const double* __restri
     }
   }
 }
-// Pass 1, first half: projection (state written to A) and, with FEAT, the plane features.
-// FEAT: also the network's input rows of the cell's five planes, normalized with `norm`
-// (16 means, 16 standard deviations) and rounded to bf16 as the network's first layer takes
-// them (the same FP32 operations and rounding as its own normalization), column 15 the need
-// flag; row q N + c of Xb, 32 bytes.
+// Pass 1, first half: projection (state written to A) and, with FEAT, the network's work.
+// FEAT: a plane whose standardized moments are within kNetTol of a Gaussian's takes the
+// Gaussian's extreme eigenvalues, -sqrt(3), +sqrt(3), as its seeds (written to P directly;
+// NaN for a plane that is not finite); the others are handed to the network as compacted
+// rows, normalized with `norm` (16 means, 16 standard deviations) and rounded to bf16 as the
+// network's first layer takes them (the same FP32 operations and rounding as its own
+// normalization; column 15 is 1).  Each wave owns a segment of kSegRows rows of Xb, one row
+// per lane and plane: its rows, plane by plane in lane order, go to the front of its
+// segment with their plane indices q N + c in idx, and the wave writes its row count to
+// seg[wave].  No atomics: the network's kernel reads each segment's count.
+constexpr unsigned kSegRows = 320;  // 64 lanes x 5 planes
 __device__ __forceinline__ short to_bf16(float f) { return __builtin_bit_cast(short, (__bf16)f); }
 template <bool FEAT>
 __global__ __launch_bounds__(kBlock, kPassWaves) void k_pass1a(const double* __restrict__ M, double* __restrict__ A,
-                                                   short* __restrict__ Xb, uint32_t* __restrict__ flags,
-                                                   DevScalars* sc, long long N, int flag2D, double Ma,
+                                                   short* __restrict__ Xb, unsigned* __restrict__ idx,
+                                                   unsigned* __restrict__ seg, float* __restrict__ P,
+                                                   uint32_t* __restrict__ flags, DevScalars* sc,
+                                                   long long N, int flag2D, double Ma,
                                                    const float* __restrict__ norm) {
   if (sc->active == 0.0) return;
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= N) return;
+  if (c >= N) return;  // whole waves past N never run; the last wave's lane 0 is inside
   double m[35], a[35];
 #pragma unroll
   for (int t = 0; t < 35; ++t) m[t] = M[(size_t)t * N + c];
@@ -389,20 +400,36 @@ __global__ __launch_bounds__(kBlock, kPassWaves) void k_pass1a(const double* __r
   if (!FEAT) return;
   Plane pl[5];
   cell_planes(a, pl);
+  const unsigned lane = threadIdx.x & 63;
+  const long long wave = c / 64;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  unsigned used = 0u;  // rows of the earlier planes (the same on every lane)
   for (int q = 0; q < 5; ++q) {
-    float x[16];
-    plane_feature(pl[q], x);
-    typedef short s8v __attribute__((ext_vector_type(8)));
-    s8v h[2];
+    const bool need = pl[q].ok && !(plane_gauss_dev(pl[q]) < kNetTol);
+    const unsigned long long want = __ballot(need);  // the wave's active lanes
+    if (need) {
+      const size_t row = (size_t)wave * kSegRows + used + (unsigned)__popcll(want & below);
+      float x[16];
+      plane_feature(pl[q], x);
+      typedef short s8v __attribute__((ext_vector_type(8)));
+      s8v h[2];
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-      const float v = t < 15 ? (x[t] - norm[t]) / norm[16 + t] : x[15];
-      h[t / 8][t % 8] = to_bf16(v);
+      for (int t = 0; t < 16; ++t) {
+        const float v = t < 15 ? (x[t] - norm[t]) / norm[16 + t] : 1.0f;
+        h[t / 8][t % 8] = to_bf16(v);
+      }
+      s8v* dst = reinterpret_cast<s8v*>(Xb + row * 16);
+      dst[0] = h[0];
+      dst[1] = h[1];
+      idx[row] = (unsigned)((long long)q * N + c);
+    } else {
+      const float g = pl[q].ok ? 1.7320508075688772f : NAN;
+      P[((size_t)q * N + c) * 2] = -g;
+      P[((size_t)q * N + c) * 2 + 1] = g;
     }
-    s8v* dst = reinterpret_cast<s8v*>(Xb + ((size_t)q * N + c) * 16);
-    dst[0] = h[0];
-    dst[1] = h[1];
+    used += (unsigned)__popcll(want);
   }
+  if (lane == 0) seg[wave] = used;
 }
 
 // Spread of a deferred-cell list of n entries over every wave of the grid: each wave takes
@@ -1161,8 +1188,11 @@ struct hyq_ctx {
   double ms_pass1, ms_update, ms_pass2;
   long long timed_steps;
   hyq_mlp* mlp;  // learned mode: warm-start network (not owned)
-  float* X;      // [5N][16] plane features
-  float* P;      // [5N][2] predictions
+  float* X;        // learned mode: the network's input rows, a segment of kSegRows rows of
+                   // 16 bf16 per wave of 64 cells
+  unsigned* Xidx;  // [segments x kSegRows] plane index q N + c of each row
+  unsigned* Xseg;  // [ceil(N / 64)] rows in each wave's segment
+  float* P;        // [5N][2] seeds of each plane: the network's predictions or the Gaussian's
   FastLists L;             // fast modes: full-path list, slots and plane items
   double* diag;            // [8] results of the diagnostics
 };
@@ -1204,8 +1234,14 @@ int hyq_create(const hyq_params* params, hyq_ctx** out) {
             hipMalloc(&c->sc, sizeof(DevScalars)) == hipSuccess &&
             hipMalloc(&c->diag, sizeof(double) * 8) == hipSuccess;
   if (ok && p.precision == HYQ_PREC_MIXED_LEARNED)
-    ok = hipMalloc(&c->X, sizeof(float) * 16 * 5 * c->N) == hipSuccess &&
-         hipMalloc(&c->P, sizeof(float) * 2 * 5 * c->N) == hipSuccess;
+    ok = hipMalloc(&c->X, sizeof(short) * 16 * kSegRows * ((c->N + 63) / 64)) == hipSuccess &&
+         hipMalloc(&c->Xidx, sizeof(unsigned) * kSegRows * ((c->N + 63) / 64)) == hipSuccess &&
+         hipMalloc(&c->Xseg, sizeof(unsigned) * ((c->N + 63) / 64)) == hipSuccess &&
+         hipMalloc(&c->P, sizeof(float) * 2 * 5 * c->N) == hipSuccess &&
+         // no rows until a step's projection has listed some: the network's kernel then
+         // reads counts of 0 (it runs, as every kernel of the step, also in a step that has
+         // nothing left to do)
+         hipMemset(c->Xseg, 0, sizeof(unsigned) * ((c->N + 63) / 64)) == hipSuccess;
   if (ok && (params->precision == HYQ_PREC_MIXED_NEWTON || params->precision == HYQ_PREC_MIXED_LEARNED)) {
     FastLists& L = c->L;
     ok = hipMalloc(&L.cells, sizeof(int) * c->N) == hipSuccess &&
@@ -1245,6 +1281,8 @@ int hyq_destroy(hyq_ctx* c) {
   hipFree(c->sc);
   hipFree(c->diag);
   hipFree(c->X);
+  hipFree(c->Xidx);
+  hipFree(c->Xseg);
   hipFree(c->P);
   hipFree(c->L.cells);
   hipFree(c->L.slot_cell);
@@ -1302,14 +1340,16 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
     const bool learned = p.precision == HYQ_PREC_MIXED_LEARNED && c->mlp != nullptr;
     if (learned) {
       hipLaunchKernelGGL(k_pass1a<true>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
-                         reinterpret_cast<short*>(c->X), c->flags, c->sc, c->N, p.flag2D, p.Ma,
-                         hyq_mlp_input_norm(c->mlp));
-      const int rc = hyq_mlp_infer_bf16_masked_async(c->mlp, reinterpret_cast<const short*>(c->X),
-                                                     5 * c->N, c->P, c->stream);
+                         reinterpret_cast<short*>(c->X), c->Xidx, c->Xseg, c->P, c->flags, c->sc,
+                         c->N, p.flag2D, p.Ma, hyq_mlp_input_norm(c->mlp));
+      const int rc = hyq_mlp_infer_bf16_segments_async(
+          c->mlp, reinterpret_cast<const short*>(c->X), c->Xidx, c->Xseg, (c->N + 63) / 64,
+          (int32_t)kSegRows, c->P, c->stream);
       if (rc) return rc;
     } else {
       hipLaunchKernelGGL(k_pass1a<false>, dim3(g), dim3(kBlock), 0, c->stream, c->M, c->Mr,
-                         (short*)nullptr, c->flags, c->sc, c->N, p.flag2D, p.Ma, (const float*)nullptr);
+                         (short*)nullptr, (unsigned*)nullptr, (unsigned*)nullptr, (float*)nullptr,
+                         c->flags, c->sc, c->N, p.flag2D, p.Ma, (const float*)nullptr);
     }
     const float* P = learned ? (const float*)c->P : (const float*)nullptr;
     hipLaunchKernelGGL(k_pass1_fast<float>, dim3(g), dim3(kBlock), 0, c->stream, c->Mr, (float*)c->F,
@@ -1657,8 +1697,9 @@ static void enqueue_fast_batch(const double* M, long long n, double Ma, int flag
   fast_batch_carve(scr, n, &sc, &L);
   const unsigned g = grid_for(n);
   hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(1), 0, 0, sc);
-  hipLaunchKernelGGL(k_pass1a<false>, dim3(g), dim3(kBlock), 0, 0, M, Mr, (short*)nullptr, flags,
-                     sc, n, flag2D, Ma, (const float*)nullptr);
+  hipLaunchKernelGGL(k_pass1a<false>, dim3(g), dim3(kBlock), 0, 0, M, Mr, (short*)nullptr,
+                     (unsigned*)nullptr, (unsigned*)nullptr, (float*)nullptr, flags, sc, n, flag2D,
+                     Ma, (const float*)nullptr);
   hipLaunchKernelGGL(k_pass1_fast<double>, dim3(g), dim3(kBlock), 0, 0, Mr, F, vp, flags,
                      (const float*)nullptr, L, sc, n, flag2D, Ma, positivity);
   hipLaunchKernelGGL(k_slot_solve<true>, dim3(kListBlocks), dim3(kBlock), 0, 0, L,

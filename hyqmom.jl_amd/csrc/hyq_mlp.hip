@@ -261,11 +261,20 @@ __device__ __forceinline__ void wave_lds_fence() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
-template <int WAVES, int RT, bool MASK, bool BIN>
+template <int WAVES, int RT, bool MASK, bool BIN, bool SEG = false>
 __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const void* __restrict__ Xv, long n,
                                                           const float* __restrict__ p,
                                                           const short* __restrict__ pk,
-                                                          float* __restrict__ P) {
+                                                          float* __restrict__ P,
+                                                          const unsigned* __restrict__ seg,
+                                                          const unsigned* __restrict__ idx,
+                                                          int seg_rows) {
+  // SEG: n rows in segments of seg_rows rows (a multiple of 16 RT), of which the first
+  // seg[s] of segment s are rows to predict; row i's prediction to P[idx[i]].  Otherwise
+  // row i's prediction to P[i].
+  auto valid = [&](long row) {
+    return SEG ? (row % seg_rows) < (long)seg[row / seg_rows] : row < n;
+  };
   const float* X = reinterpret_cast<const float*>(Xv);
   const s8* Xb = reinterpret_cast<const s8*>(Xv);  // BIN: row b is Xb[2 b], Xb[2 b + 1]
   __shared__ s8 wts[PK_N / 8];
@@ -308,7 +317,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const void* __restrict
     for (int u = 0; u < RT; ++u) {
       const long row = g * 16 * RT + 16 * u + r;
       if (BIN) {
-        bn[u] = (q < 2 && row < n) ? Xb[row * 2 + q] : s8{0, 0, 0, 0, 0, 0, 0, 0};
+        bn[u] = (q < 2 && row < n && valid(row)) ? Xb[row * 2 + q] : s8{0, 0, 0, 0, 0, 0, 0, 0};
       } else {
 #pragma unroll
         for (int e = 0; e < 8; ++e) xn[u][e] = (q < 2 && row < n) ? X[row * 16 + 8 * q + e] : 0.f;
@@ -332,6 +341,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const void* __restrict
       }
     }
     if (grp + gstride < ngroups) load_group(grp + gstride);
+    if (SEG && !valid(row0)) continue;  // the segment's rows end before this group
     if (MASK) {
       // column 15 of row 16u + r sits in lane (r, q = 1), element 7
       bool need = false;
@@ -427,7 +437,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const void* __restrict
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const long row = row0 + 16 * u + 4 * q + e;
-          if (row < n) P[row * 2 + r] = (o[e] + b) * sd + mu;
+          if (row < n && valid(row)) P[(SEG ? (long)idx[row] : row) * 2 + r] = (o[e] + b) * sd + mu;
         }
       }
     }
@@ -440,9 +450,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const void* __restrict
 // CU).  They compute the same values in the same order; the first large inference times them
 // and keeps the fastest.
 constexpr int kFusedShapes = 3;
-template <bool MASK, bool BIN>
+template <bool MASK, bool BIN, bool SEG = false>
 void launch_fused_t(int shape, const void* X, long n, const float* p, const short* pk, float* P,
-                    hipStream_t s) {
+                    hipStream_t s, const unsigned* seg = nullptr, const unsigned* idx = nullptr,
+                    int seg_rows = 0) {
   const int waves = shape == 0 ? 16 : 8;
   const int rt = shape == 1 ? 2 : 1;
   const long groups = (n + 16 * rt - 1) / (16 * rt);
@@ -450,11 +461,14 @@ void launch_fused_t(int shape, const void* X, long n, const float* p, const shor
   if (blocks > 2048) blocks = 2048;
   const dim3 g((unsigned)blocks);
   if (shape == 0)
-    hipLaunchKernelGGL((k_mlp_fused<16, 1, MASK, BIN>), g, dim3(64 * 16), 0, s, X, n, p, pk, P);
+    hipLaunchKernelGGL((k_mlp_fused<16, 1, MASK, BIN, SEG>), g, dim3(64 * 16), 0, s, X, n, p, pk, P,
+                       seg, idx, seg_rows);
   else if (shape == 1)
-    hipLaunchKernelGGL((k_mlp_fused<8, 2, MASK, BIN>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
+    hipLaunchKernelGGL((k_mlp_fused<8, 2, MASK, BIN, SEG>), g, dim3(64 * 8), 0, s, X, n, p, pk, P,
+                       seg, idx, seg_rows);
   else
-    hipLaunchKernelGGL((k_mlp_fused<8, 1, MASK, BIN>), g, dim3(64 * 8), 0, s, X, n, p, pk, P);
+    hipLaunchKernelGGL((k_mlp_fused<8, 1, MASK, BIN, SEG>), g, dim3(64 * 8), 0, s, X, n, p, pk, P,
+                       seg, idx, seg_rows);
 }
 // X: float rows (bin = false) or normalized bf16 rows (bin = true), 16 values each.
 void launch_fused(int shape, bool mask, bool bin, const void* X, long n, const float* p,
@@ -689,6 +703,29 @@ int infer_bf16_masked_async(void* h, const short* Xb, int64_t n, float* P,
                             hipStream_t stream) {
   Net* m = static_cast<Net*>(h);
   return infer_fused(m, Xb, n, P, stream, true, true);
+}
+// The step's inference on segmented rows: nseg segments of seg_rows rows of Xb (normalized
+// bf16 rows, as the step's projection kernel writes them), of which the first seg[s] of
+// segment s (read on the device) are rows to predict; row i's prediction to P[idx[i]].
+// seg_rows must be a multiple of 32 (whole row groups of every launch shape).  Every
+// listed row is computed, with the launch shape the timing trial chose (8 waves x 1 tile
+// until one was chosen): the same predictions, bit for bit, as the unsegmented forms give
+// for the same rows.
+int infer_bf16_segments_async(void* h, const short* Xb, const uint32_t* idx, const uint32_t* seg,
+                              int64_t nseg, int32_t seg_rows, float* P, hipStream_t stream) {
+  Net* m = static_cast<Net*>(h);
+  if (!m || !Xb || !idx || !seg || !P || nseg < 0 || seg_rows <= 0 || seg_rows % 32 != 0)
+    return mfail(HYQ_EINVAL, "bad argument");
+  if (nseg == 0) return HYQ_OK;
+  if (m->packed_version != m->version) {
+    MCHK(hipStreamSynchronize(m->stream), "mlp sync");
+    hipLaunchKernelGGL(k_pack, dim3(g1(PK_N)), dim3(256), 0, stream, m->blob, m->packed);
+    m->packed_version = m->version;
+  }
+  launch_fused_t<false, true, true>(m->shape < 0 ? 2 : m->shape, Xb, (long)nseg * seg_rows, m->blob,
+                                    m->packed, P, stream, seg, idx, seg_rows);
+  MCHK(hipGetLastError(), "mlp infer launch");
+  return HYQ_OK;
 }
 // Device pointer to the input normalization: 16 means, then 16 standard deviations.
 const float* input_norm(void* h) {

@@ -20,6 +20,9 @@
   int infer_masked_async(void* h, const float* X, int64_t n, float* P, hipStream_t stream);    \
   int infer_bf16_masked_async(void* h, const short* Xb, int64_t n, float* P,                   \
                               hipStream_t stream);                                             \
+  int infer_bf16_segments_async(void* h, const short* Xb, const uint32_t* idx,                 \
+                                const uint32_t* seg, int64_t nseg, int32_t seg_rows, float* P,   \
+                                hipStream_t stream);                                             \
   const float* input_norm(void* h);                                                            \
   int set_shape(void* h, int32_t shape);                                                       \
   int infer_layers(void* h, const float* X, int64_t n, float* P);                              \
@@ -102,6 +105,11 @@ int hyq_mlp_infer_masked_async(hyq_mlp* m, const float* X, int64_t n, float* P, 
 int hyq_mlp_infer_bf16_masked_async(hyq_mlp* m, const short* Xb, int64_t n, float* P,
                                     hipStream_t stream) {
   return HYQ_MLP_CALL(m, infer_bf16_masked_async, Xb, n, P, stream);
+}
+int hyq_mlp_infer_bf16_segments_async(hyq_mlp* m, const short* Xb, const uint32_t* idx,
+                                      const uint32_t* seg, int64_t nseg, int32_t seg_rows,
+                                      float* P, hipStream_t stream) {
+  return HYQ_MLP_CALL(m, infer_bf16_segments_async, Xb, idx, seg, nseg, seg_rows, P, stream);
 }
 const float* hyq_mlp_input_norm(hyq_mlp* m) {
   if (!m) return nullptr;

@@ -38,8 +38,13 @@ HD void pack35(const double* c, double* M) {
 // Largest deviation of a plane's standardized moments from a Gaussian's (S30 = S03 = S21 =
 // S12 = 0, S40 = S04 = 3, S31 = S13 = 3 S11, S22 = 1 + 2 S11^2).  Below kGaussTol the
 // Gaussian's extreme eigenvalues seed the fast solve (the seed is still verified by the
-// Sturm count); in the learned mode such planes skip the network.
+// Sturm count).
 constexpr double kGaussTol = 1e-3;
+// The learned mode hands a plane to the network only when its standardized moments are
+// at least this far from a Gaussian's; nearer ones take the Gaussian's seeds (verified by
+// the Sturm count like any seed): a seed from the network saves at most about one Laguerre
+// iteration per end there, less than the network's own cost.
+constexpr double kNetTol = 0.1;
 HD double plane_gauss_dev(const Plane& p) {
   double d = fmax(fmax(fabs(p.S30), fabs(p.S03)), fmax(fabs(p.S21), fabs(p.S12)));
   d = fmax(d, fmax(fabs(p.S40 - 3.0), fabs(p.S04 - 3.0)));

@@ -297,6 +297,59 @@ def test_bf16_rows_give_the_same_predictions(shape, hidden):
     assert np.array_equal(a[~skip], b[~skip]), np.abs(a - b)[~skip].max()
 
 
+@pytest.mark.parametrize("hidden", WIDTHS)
+def test_segmented_rows_give_the_same_predictions(hidden):
+    """The step hands the network only the planes that need it: each wave of the projection
+    kernel fills the front of its own segment of 320 rows and writes its count
+    (hyq_mlp_infer_bf16_segments_async reads the counts on the device).  Each listed row's
+    prediction lands at its plane index, bit for bit the prediction of the unsegmented
+    kernel for the same row; rows past a segment's count and unlisted planes are not
+    written.  Counts 0, partial and full are covered."""
+    import ctypes
+    import torch
+    from hyqmom_amd import _lib
+    from hyqmom_amd.warmstart import WarmStartMLP
+    nseg, srows, nplanes = 17, 320, 20000
+    n = nseg * srows
+    rng = np.random.default_rng(29)
+    X = rng.standard_normal((n, 16)).astype(np.float32)
+    X[:, 15] = 1.0
+    counts = rng.integers(0, srows + 1, nseg).astype(np.uint32)
+    counts[0], counts[1], counts[2] = 0, srows, 1
+    blob = R.random_blob(25, hidden=hidden)
+    m = WarmStartMLP(blob)
+    npar = R.nparam(hidden)
+    mu = blob[npar:npar + 16].astype(np.float32)
+    sd = blob[npar + 16:npar + 32].astype(np.float32)
+    Xn = X.copy()
+    Xn[:, :15] = (X[:, :15] - mu[:15]) / sd[:15]
+    bits = (R.bf16(Xn).view(np.uint32) >> 16).astype(np.uint16).view(np.int16)
+    Xb = torch.from_numpy(bits.copy()).cuda()
+    idx = rng.choice(nplanes, n, replace=False).astype(np.uint32)
+    Id = torch.from_numpy(idx.view(np.int32).copy()).cuda()
+    Sd = torch.from_numpy(counts.view(np.int32).copy()).cuda()
+    Pref = torch.empty((n, 2), dtype=torch.float32, device="cuda")
+    Pc = torch.full((nplanes, 2), 7.0, dtype=torch.float32, device="cuda")
+    L = _lib.lib()
+    fm = L.hyq_mlp_infer_bf16_masked_async
+    fm.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+    fm.restype = ctypes.c_int
+    fs = L.hyq_mlp_infer_bf16_segments_async
+    fs.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                   ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]
+    fs.restype = ctypes.c_int
+    assert fm(m.handle, Xb.data_ptr(), n, Pref.data_ptr(), None) == 0
+    assert fs(m.handle, Xb.data_ptr(), Id.data_ptr(), Sd.data_ptr(), nseg, srows, Pc.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    a, b = Pref.cpu().numpy(), Pc.cpu().numpy()
+    listed = (np.arange(n) % srows) < counts[np.arange(n) // srows]
+    assert listed.sum() == counts.sum()
+    assert np.array_equal(b[idx[listed]], a[listed]), np.abs(b[idx[listed]] - a[listed]).max()
+    untouched = np.ones(nplanes, bool)
+    untouched[idx[listed]] = False
+    assert (b[untouched] == 7.0).all()
+
+
 def test_width128_network_in_the_production_step():
     """The warm-start configuration: the 15 -> 128 -> 128 -> 128 -> 2 network
     (hyqmom.jl_amd/models/warmstart_mlp_h128.f32, trained on the GPU by the library's own
