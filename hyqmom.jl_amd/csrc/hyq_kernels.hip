@@ -459,8 +459,10 @@ struct ListSpread {
 // every wave with work runs at once, in 128-thread blocks.
 constexpr int kFullWaves = 1, kSolveWaves = 1;
 constexpr unsigned kListBlocks = 256 * 4 * kFullWaves / 2;
-// The fix-up kernels finish the slots (up to a quarter of the cells in the middle of a run)
-// with short, converged code: two waves per SIMD, a grid-stride loop over the slots.
+// Pass 2's fix-up kernel finishes the slots (up to a quarter of the cells in the middle of
+// a run) with the projection and BGK: two waves per SIMD, a grid-stride loop over the
+// slots.  (Pass 1's, which only combines speeds, was slower on this grid: more waves,
+// each with its atomic wave maximum.)
 constexpr unsigned kFixupBlocks = 256 * 4 * kFastWaves / 2;
 
 // ---- the fast step (mixed_newton / mixed_learned modes) ------------------------------------
@@ -1356,7 +1358,7 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
                        (float*)c->vp, c->flags, P, c->L, c->sc, c->N, p.flag2D, p.Ma, p.positivity);
     hipLaunchKernelGGL(k_slot_solve<true>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
                        (const unsigned*)&c->sc->nslot1, &c->sc->nitem1, P, c->sc, c->N);
-    hipLaunchKernelGGL(k_pass1_fixup<float>, dim3(kFixupBlocks), dim3(kBlock), 0, c->stream, c->L,
+    hipLaunchKernelGGL(k_pass1_fixup<float>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
                        (float*)c->vp, c->flags, c->sc, c->N);
     enqueue_full1<float>(c->stream, c->Mr, (float*)c->F, (float*)c->vp, c->flags, P, c->L,
                          c->sc, c->N, p.flag2D, p.Ma, p.positivity);
@@ -1704,7 +1706,7 @@ static void enqueue_fast_batch(const double* M, long long n, double Ma, int flag
                      (const float*)nullptr, L, sc, n, flag2D, Ma, positivity);
   hipLaunchKernelGGL(k_slot_solve<true>, dim3(kListBlocks), dim3(kBlock), 0, 0, L,
                      (const unsigned*)&sc->nslot1, &sc->nitem1, (const float*)nullptr, sc, n);
-  hipLaunchKernelGGL(k_pass1_fixup<double>, dim3(kFixupBlocks), dim3(kBlock), 0, 0, L, vp, flags, sc, n);
+  hipLaunchKernelGGL(k_pass1_fixup<double>, dim3(kListBlocks), dim3(kBlock), 0, 0, L, vp, flags, sc, n);
   enqueue_full1<double>(0, Mr, F, vp, flags, (const float*)nullptr, L, sc, n, flag2D, Ma,
                         positivity);
 }

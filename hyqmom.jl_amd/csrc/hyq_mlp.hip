@@ -325,9 +325,23 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const void* __restrict
     }
   };
   short* A = act[w];
-  long grp = (long)blockIdx.x * WAVES + w;
+  // Group order.  Plain: group W, W + gstride, ... for wave W.  SEG: the groups that hold
+  // rows of segments W, W + gstride, ... (a segment's seg[s] rows fill its first
+  // ceil(seg[s] / (16 RT)) groups), so that a wave never visits a group past a count.
+  const long wid = (long)blockIdx.x * WAVES + w;
+  const long gps = SEG ? seg_rows / (16 * RT) : 1;  // groups per segment
+  const long nseg = SEG ? n / seg_rows : 0;
+  auto seg_groups = [&](long t) { return ((long)seg[t] + 16 * RT - 1) / (16 * RT); };
+  auto first_in = [&](long t, long& gc) {  // first group of the first segment t, t + gstride, ... with rows
+    gc = 0;
+    while (t < nseg && (gc = seg_groups(t)) == 0) t += gstride;
+    return t < nseg ? t * gps : ngroups;
+  };
+  long gc = 0;  // SEG: groups of the current segment
+  long grp = SEG ? first_in(wid, gc) : wid;
+  long nxt_g = ngroups, nxt_c = 0;
   if (grp < ngroups) load_group(grp);
-  for (; grp < ngroups; grp += gstride) {
+  for (; grp < ngroups; grp = nxt_g, gc = nxt_c) {
     const long row0 = grp * 16 * RT;
     float xv[RT][8];
     s8 bv8[RT];
@@ -340,8 +354,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const void* __restrict
         for (int e = 0; e < 8; ++e) xv[u][e] = xn[u][e];
       }
     }
-    if (grp + gstride < ngroups) load_group(grp + gstride);
-    if (SEG && !valid(row0)) continue;  // the segment's rows end before this group
+    if (SEG) {
+      nxt_c = gc;
+      nxt_g = (grp % gps) + 1 < gc ? grp + 1 : first_in(grp / gps + gstride, nxt_c);
+    } else {
+      nxt_g = grp + gstride;
+    }
+    if (nxt_g < ngroups) load_group(nxt_g);
     if (MASK) {
       // column 15 of row 16u + r sits in lane (r, q = 1), element 7
       bool need = false;
