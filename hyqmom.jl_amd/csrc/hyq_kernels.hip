@@ -59,6 +59,11 @@ constexpr int kBlock = 128;
 // hide more latency at the price of spills to scratch.  Measured on the 120x120x60 workload,
 // 2 beats 1 for every pass kernel, and the fast kernels are slower at 3 or 4.
 constexpr int kPassWaves = 2, kFastWaves = 2;
+// The fast modes' projection kernels (k_pass1a, k_pass2_relax) need about 310 registers:
+// one wave per SIMD holds them without scratch, where two waves spill 180-190 B per lane;
+// measured 5-15 us faster each at 120x120x60 (k_pass2_check, 196 B of spill at two waves,
+// was 35 us slower at one).
+constexpr int kProjWaves = 1;
 
 constexpr int kNev = 5;  // timing events per step
 constexpr const char* kNonfiniteMsg =
@@ -380,7 +385,7 @@ __global__ __launch_bounds__(kBlock) void k_plane_dataset(const double* __restri
 constexpr unsigned kSegRows = 320;  // 64 lanes x 5 planes
 __device__ __forceinline__ short to_bf16(float f) { return __builtin_bit_cast(short, (__bf16)f); }
 template <bool FEAT>
-__global__ __launch_bounds__(kBlock, kPassWaves) void k_pass1a(const double* __restrict__ M, double* __restrict__ A,
+__global__ __launch_bounds__(kBlock, kProjWaves) void k_pass1a(const double* __restrict__ M, double* __restrict__ A,
                                                    short* __restrict__ Xb, unsigned* __restrict__ idx,
                                                    unsigned* __restrict__ seg, float* __restrict__ P,
                                                    uint32_t* __restrict__ flags, DevScalars* sc,
@@ -916,7 +921,7 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_check(const double
     new_slot(L, &sc->nslot2, c, pend, a, N);
 }
 
-__global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_relax(double* __restrict__ M,
+__global__ __launch_bounds__(kBlock, kProjWaves) void k_pass2_relax(double* __restrict__ M,
                                                                    const double* __restrict__ A,
                                                                    DevScalars* sc, long long N,
                                                                    int flag2D, double Ma,
