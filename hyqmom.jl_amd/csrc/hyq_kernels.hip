@@ -467,7 +467,8 @@ constexpr unsigned kListBlocks = 256 * 4 * kFullWaves / 2;
 // Pass 2's fix-up kernel finishes the slots (up to a quarter of the cells in the middle of
 // a run) with the projection and BGK: two waves per SIMD, a grid-stride loop over the
 // slots.  (Pass 1's, which only combines speeds, was slower on this grid: more waves,
-// each with its atomic wave maximum.)
+// each with its atomic wave maximum.)  Pass 2's slot solver, which fits two waves per SIMD
+// (it needs no extremes), runs on this grid too.
 constexpr unsigned kFixupBlocks = 256 * 4 * kFastWaves / 2;
 
 // ---- the fast step (mixed_newton / mixed_learned modes) ------------------------------------
@@ -1392,7 +1393,7 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
                        c->Mr, c->L, c->sc, c->N, p.flag2D, p.Ma);
     hipLaunchKernelGGL(k_pass2_relax, dim3(g), dim3(kBlock), 0, c->stream, c->M,
                        (const double*)c->Mr, c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
-    hipLaunchKernelGGL(k_slot_solve<false>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
+    hipLaunchKernelGGL(k_slot_solve<false>, dim3(kFixupBlocks), dim3(kBlock), 0, c->stream, c->L,
                        (const unsigned*)&c->sc->nslot2, &c->sc->nitem2, (const float*)nullptr, c->sc,
                        c->N);
     hipLaunchKernelGGL(k_pass2_fixup, dim3(kFixupBlocks), dim3(kBlock), 0, c->stream, c->M, c->L,
