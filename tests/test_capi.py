@@ -39,6 +39,25 @@ def test_invalid_params_rejected():
     assert L.hyq_create(ctypes.byref(p), ctypes.byref(h)) == -1
 
 
+def test_cell_limit_rejected_before_any_allocation():
+    """The update addresses a plane with 32-bit byte offsets, so a context, an ensemble (all
+    its runs together) and an HLL line hold at most 2^29 - 1 cells; larger requests are
+    rejected before anything touches the GPU."""
+    from hyqmom_amd import _lib
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    p = _lib.HyqParams(Nx=1024, Ny=1024, Nz=512, Nmom=35)  # 2^29 cells
+    assert L.hyq_create(ctypes.byref(p), ctypes.byref(h)) == -1
+    assert b"2^29" in L.hyq_last_error()
+    runs = (_lib.HyqParams * 2)(_lib.HyqParams(Nx=512, Ny=512, Nz=1024, Nmom=35),
+                                _lib.HyqParams(Nx=512, Ny=512, Nz=1024, Nmom=35))
+    assert L.hyq_ensemble_create(runs, 2, ctypes.byref(h)) == -1
+    assert b"2^29" in L.hyq_ensemble_last_error()
+    d = ctypes.c_void_p(8)  # never dereferenced: the length is checked first
+    assert L.hyq_hll_lines(d, d, d, d, 1, 1 << 29, 1e-3, 0.01, d) == -1
+    assert b"2^29" in L.hyq_last_error()
+
+
 def test_network_width_matches_cpu_reference():
     """The library's parameter blob has the layout of the CPU reference network (same hidden
     width), so blobs move between them (a host-only query, no GPU)."""
