@@ -464,11 +464,11 @@ struct ListSpread {
 // every wave with work runs at once, in 128-thread blocks.
 constexpr int kFullWaves = 1, kSolveWaves = 1;
 constexpr unsigned kListBlocks = 256 * 4 * kFullWaves / 2;
-// Pass 2's fix-up kernel finishes the slots (up to a quarter of the cells in the middle of
-// a run) with the projection and BGK: two waves per SIMD, a grid-stride loop over the
-// slots.  (Pass 1's, which only combines speeds, was slower on this grid: more waves,
-// each with its atomic wave maximum.)  Pass 2's slot solver, which fits two waves per SIMD
-// (it needs no extremes), runs on this grid too.
+// Pass 2's slot solver (which fits two waves per SIMD: it needs no extremes) and fix-up
+// kernel, which sorts out the slots (up to a quarter of the cells in the middle of a run),
+// run on a grid of two waves per SIMD, with grid-stride loops over the slots.  (Pass 1's
+// fix-up, which only combines speeds, was slower on this grid: more waves, each with its
+// atomic wave maximum.)
 constexpr unsigned kFixupBlocks = 256 * 4 * kFastWaves / 2;
 
 // ---- the fast step (mixed_newton / mixed_learned modes) ------------------------------------
@@ -894,8 +894,9 @@ static void enqueue_full1(hipStream_t stream, double* AMr, FT* F, FT* vp, uint32
 }
 
 // The fast modes' pass 2: projection and the x/y hyperbolicity checks (projected state to
-// A; A[0] = NaN marks a cell that k_pass2_relax leaves alone), BGK of the proven cells, the
-// plane solves, the fix-up of the cells with plane solves, and the full pass 2 of the rest.
+// A; A[0] = NaN marks a cell that k_pass2_relax leaves alone), the plane solves, the fix-up
+// of the cells with plane solves (those proven hyperbolic back into A, the others to the
+// full path), BGK of every proven cell, and the full pass 2 of the rest.
 __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_check(const double* __restrict__ M,
                                                                    double* __restrict__ A,
                                                                    FastLists L, DevScalars* sc,
@@ -942,12 +943,11 @@ __global__ __launch_bounds__(kBlock, kProjWaves) void k_pass2_relax(double* __re
   for (int n = 0; n < 35; ++n) M[(size_t)n * N + c] = o[n];
 }
 
-// Pass 2 of the cells with plane solves: BGK, or the full path after a complex pair.
-__global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_fixup(double* __restrict__ M,
+// Pass 2 of the cells with plane solves: back into A for k_pass2_relax, or the full path
+// after a complex pair.
+__global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_fixup(double* __restrict__ A,
                                                                    FastLists L, DevScalars* sc,
-                                                                   long long N, int flag2D,
-                                                                   double Ma, int positivity,
-                                                                   double Kn) {
+                                                                   long long N) {
   if (sc->active == 0.0) return;
   const unsigned n = sc->nslot2;
   for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
@@ -965,13 +965,10 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_fixup(double* __re
       L.full_known[i] = known;
       continue;
     }
-    double a[35], o[35];
+    // proven hyperbolic: back into the relaxation kernel's input A, which then relaxes it
+    // with the cells the check proved
 #pragma unroll
-    for (int t = 0; t < 35; ++t) a[t] = L.slot_A[(size_t)t * N + s];
-    uint32_t fl = 0;
-    pass2_relax<true>(a, flag2D, Ma, positivity != 0, sc->dt, Kn, o, fl);
-#pragma unroll
-    for (int t = 0; t < 35; ++t) M[(size_t)t * N + c] = o[t];
+    for (int t = 0; t < 35; ++t) A[(size_t)t * N + c] = L.slot_A[(size_t)t * N + s];
   }
 }
 
@@ -1391,13 +1388,13 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
   } else {
     hipLaunchKernelGGL(k_pass2_check, dim3(g), dim3(kBlock), 0, c->stream, (const double*)c->M,
                        c->Mr, c->L, c->sc, c->N, p.flag2D, p.Ma);
-    hipLaunchKernelGGL(k_pass2_relax, dim3(g), dim3(kBlock), 0, c->stream, c->M,
-                       (const double*)c->Mr, c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
     hipLaunchKernelGGL(k_slot_solve<false>, dim3(kFixupBlocks), dim3(kBlock), 0, c->stream, c->L,
                        (const unsigned*)&c->sc->nslot2, &c->sc->nitem2, (const float*)nullptr, c->sc,
                        c->N);
-    hipLaunchKernelGGL(k_pass2_fixup, dim3(kFixupBlocks), dim3(kBlock), 0, c->stream, c->M, c->L,
-                       c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
+    hipLaunchKernelGGL(k_pass2_fixup, dim3(kFixupBlocks), dim3(kBlock), 0, c->stream, c->Mr, c->L,
+                       c->sc, c->N);
+    hipLaunchKernelGGL(k_pass2_relax, dim3(g), dim3(kBlock), 0, c->stream, c->M,
+                       (const double*)c->Mr, c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
     hipLaunchKernelGGL(k_full2_stage<1>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
                        c->sc, c->N, p.flag2D, p.Ma);
     hipLaunchKernelGGL(k_full2_stage<2>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
