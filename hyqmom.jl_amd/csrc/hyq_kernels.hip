@@ -974,11 +974,16 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_fixup(double* __re
 
 // The full pass 2 of the listed cells, in stages like the full pass 1: the x and y
 // corrections of the projected state the check left in the list (with the plane decisions the
-// check and the plane solves reached on it), then the closure and BGK.  STAGE 1: x; 2: y.
+// check and the plane solves reached on it).  Stage 2 hands the corrected state to
+// k_pass2_relax through A, which relaxes it with every other cell (the same BGK on the same
+// state); a state whose density is not a number, which would read as A's "not proven"
+// mark, is relaxed here instead.  STAGE 1: x; 2: y.
 template <int STAGE>
 __global__ __launch_bounds__(kBlock, kFullWaves) void k_full2_stage(FastLists L, DevScalars* sc,
                                                                    long long N, int flag2D,
-                                                                   double Ma) {
+                                                                   double Ma, double* __restrict__ A,
+                                                                   double* __restrict__ M,
+                                                                   int positivity, double Kn) {
   if (sc->active == 0.0) return;
   const ListSpread<> ls(sc->nhard2);
   if (ls.idle()) return;
@@ -993,29 +998,22 @@ __global__ __launch_bounds__(kBlock, kFullWaves) void k_full2_stage(FastLists L,
     // the y decisions no longer hold
     const bool corrected = STAGE == 1 ? eig6_axis_decided<0>(a, flag2D, Ma, b, fl, known)
                                       : eig6_axis_decided<1>(a, flag2D, Ma, b, fl, known);
-    if (STAGE == 1 && corrected) L.full_known[i] = known & 0x0Fu;
+    if (STAGE == 1) {
+      if (corrected) L.full_known[i] = known & 0x0Fu;
 #pragma unroll
-    for (int t = 0; t < 35; ++t) L.full_S[(size_t)t * N + i] = b[t];
-  }
-}
-__global__ __launch_bounds__(kBlock, kFullWaves) void k_full2_finish(double* __restrict__ M,
-                                                                    FastLists L, DevScalars* sc,
-                                                                    long long N, int flag2D,
-                                                                    double Ma, int positivity,
-                                                                    double Kn) {
-  if (sc->active == 0.0) return;
-  const ListSpread<> ls(sc->nhard2);
-  if (ls.idle()) return;
-  for (unsigned i = ls.first(); i < ls.n; i += ls.stride) {
-    if (!ls.mine(i)) continue;
+      for (int t = 0; t < 35; ++t) L.full_S[(size_t)t * N + i] = b[t];
+      continue;
+    }
     const long long c = L.cells[i];
-    double a[35], o[35];
+    if (b[0] == b[0]) {
 #pragma unroll
-    for (int t = 0; t < 35; ++t) a[t] = L.full_S[(size_t)t * N + i];
-    uint32_t fl = 0;
-    pass2_relax<true>(a, flag2D, Ma, positivity != 0, sc->dt, Kn, o, fl);
+      for (int t = 0; t < 35; ++t) A[(size_t)t * N + c] = b[t];
+    } else {
+      double o[35];
+      pass2_relax<true>(b, flag2D, Ma, positivity != 0, sc->dt, Kn, o, fl);
 #pragma unroll
-    for (int t = 0; t < 35; ++t) M[(size_t)t * N + c] = o[t];
+      for (int t = 0; t < 35; ++t) M[(size_t)t * N + c] = o[t];
+    }
   }
 }
 
@@ -1393,14 +1391,12 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
                        c->N);
     hipLaunchKernelGGL(k_pass2_fixup, dim3(kFixupBlocks), dim3(kBlock), 0, c->stream, c->Mr, c->L,
                        c->sc, c->N);
+    hipLaunchKernelGGL(k_full2_stage<1>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
+                       c->sc, c->N, p.flag2D, p.Ma, c->Mr, c->M, p.positivity, p.Kn);
+    hipLaunchKernelGGL(k_full2_stage<2>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
+                       c->sc, c->N, p.flag2D, p.Ma, c->Mr, c->M, p.positivity, p.Kn);
     hipLaunchKernelGGL(k_pass2_relax, dim3(g), dim3(kBlock), 0, c->stream, c->M,
                        (const double*)c->Mr, c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
-    hipLaunchKernelGGL(k_full2_stage<1>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
-                       c->sc, c->N, p.flag2D, p.Ma);
-    hipLaunchKernelGGL(k_full2_stage<2>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
-                       c->sc, c->N, p.flag2D, p.Ma);
-    hipLaunchKernelGGL(k_full2_finish, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->M, c->L,
-                       c->sc, c->N, p.flag2D, p.Ma, p.positivity, p.Kn);
   }
   if (ev) hipEventRecord(ev[4], c->stream);
   const hipError_t e = hipGetLastError();
