@@ -532,60 +532,87 @@ __device__ __forceinline__ unsigned new_slot(const FastLists& L, unsigned* count
 
 // Pass 1 from the projected state that k_pass1a left in AMr (and, in the learned mode, the
 // network's seeds in P).
+// The common cells' wave speeds: the five fast plane solves of the projected state A; a
+// proven cell's six plane extremes (min and max over its two planes per direction) go to
+// vp, for k_pass1_close to combine with the 1D speeds; a cell with pending planes gets a
+// slot, one for the full path a list entry.  The closure and the fluxes of every cell are
+// k_pass1_close's, after the deferred cells: this kernel holds the plane solves' registers
+// only (no scratch at two waves per SIMD).
 template <typename FT>
 __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_fast(
-    double* __restrict__ AMr, FT* __restrict__ F, FT* __restrict__ vp,
-    uint32_t* __restrict__ flags, const float* __restrict__ P, FastLists L, DevScalars* sc,
-    long long N, int flag2D, double Ma, int positivity) {
+    const double* __restrict__ A, FT* __restrict__ vp, uint32_t* __restrict__ flags,
+    const float* __restrict__ P, FastLists L, DevScalars* sc, long long N, int positivity) {
+  if (sc->active == 0.0) return;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  double a[35];
+#pragma unroll
+  for (int t = 0; t < 35; ++t) a[t] = A[(size_t)t * N + c];
+  const uint32_t fl0 = flags[c];
+  uint32_t fl = fl0;
+  Warm w = no_warm();
+  if (P != nullptr) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      w.w[2 * q] = P[((size_t)q * N + c) * 2];
+      w.w[2 * q + 1] = P[((size_t)q * N + c) * 2 + 1];
+    }
+  }
+  double v[10];
+  unsigned pend;
+  const int st = fast_cell_planes<false, true>(a, positivity != 0, w, v, pend, fl);
+  if (st == kPlaneFull) {
+    new_hard(L, &sc->nhard1, c, a, fl0, N);  // the full path starts from the projection
+    return;
+  }
+  if (st == kPlaneSolve) {
+    const unsigned slot = new_slot(L, &sc->nslot1, c, pend, a, N);
+    L.slot_fl[slot] = fl;
+#pragma unroll
+    for (int t = 0; t < 10; ++t) L.slot_v[(size_t)t * N + slot] = v[t];
+    return;
+  }
+  double v6[6];
+  combine_planes(v, v6);
+#pragma unroll
+  for (int d = 0; d < 6; ++d) vp[(size_t)d * N + c] = (FT)v6[d];
+  flags[c] = fl;
+}
+
+// Pass 1's closure and fluxes of every cell, after the deferred cells: A (the projected
+// state, or the full path's final state) to its closure Mr in place, the fluxes, the 1D
+// speeds of Mr combined with the plane extremes the speed kernels left in vp (min and max
+// commute with rounding to FT, so combining the rounded extremes gives the same FT speeds),
+// the flags, the wave maximum |vp|.  One wave per SIMD: the closure with fluxes needs about
+// 400 registers.
+template <typename FT>
+__global__ __launch_bounds__(kBlock, kProjWaves) void k_pass1_close(
+    double* __restrict__ AMr, FT* __restrict__ F, FT* __restrict__ vp, uint32_t* __restrict__ flags,
+    DevScalars* sc, long long N, int flag2D, double Ma) {
   if (sc->active == 0.0) return;
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long vb = 0;
   if (c < N) {
-    double a[35];
+    double a[35], mr[35], s1[6], v6[6], vd[6];
 #pragma unroll
     for (int t = 0; t < 35; ++t) a[t] = AMr[(size_t)t * N + c];
-    const uint32_t fl0 = flags[c];
-    uint32_t fl = fl0;
-    Warm w = no_warm();
-    if (P != nullptr) {
+    uint32_t fl = flags[c];
+    flux_closure35<true, FT>(a, flag2D, Ma, mr, F + c, F + (size_t)35 * N + c,
+                             F + (size_t)70 * N + c, (size_t)N, fl);
+    closure1d_all(mr, s1);
 #pragma unroll
-      for (int q = 0; q < 5; ++q) {
-        w.w[2 * q] = P[((size_t)q * N + c) * 2];
-        w.w[2 * q + 1] = P[((size_t)q * N + c) * 2 + 1];
-      }
+    for (int d = 0; d < 6; ++d) v6[d] = (double)vp[(size_t)d * N + c];
+    combine_vp(s1, v6, vd);
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+      vp[(size_t)d * N + c] = (FT)vd[d];
+      const unsigned long long bb = __double_as_longlong(fabs(vd[d]));
+      vb = bb > vb ? bb : vb;
     }
-    double v[10];
-    unsigned pend;
-    const int st = fast_cell_planes<false, true>(a, positivity != 0, w, v, pend, fl);
-    if (st == kPlaneFull) {
-      new_hard(L, &sc->nhard1, c, a, fl0, N);  // the full path starts from the projection
-    } else {
-      const unsigned slot = st == kPlaneSolve ? new_slot(L, &sc->nslot1, c, pend, a, N) : 0u;
-      double mr[35], s1[6];
-      flux_closure35<true, FT>(a, flag2D, Ma, mr, F + c, F + (size_t)35 * N + c,
-                               F + (size_t)70 * N + c, (size_t)N, fl);
-      closure1d_all(mr, s1);
-      if (st == kPlaneProven) {
-        double v6[6], vd[6];
-        combine_planes(v, v6);
-        combine_vp(s1, v6, vd);
 #pragma unroll
-        for (int d = 0; d < 6; ++d) {
-          vp[(size_t)d * N + c] = (FT)vd[d];
-          const unsigned long long b = __double_as_longlong(fabs(vd[d]));
-          vb = b > vb ? b : vb;
-        }
-        flags[c] = fl;
-      } else {
-        L.slot_fl[slot] = fl;
-#pragma unroll
-        for (int t = 0; t < 10; ++t) L.slot_v[(size_t)t * N + slot] = v[t];
-#pragma unroll
-        for (int t = 0; t < 6; ++t) L.slot_v[(size_t)(10 + t) * N + slot] = s1[t];
-      }
-#pragma unroll
-      for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = mr[t];
-    }
+    for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = mr[t];
+    flags[c] = fl;
+    if (fl & F_NONFIN) atomicOr(&sc->err, 1ull);
   }
   vb = wave_max_u64(vb);
   if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
@@ -648,7 +675,6 @@ __global__ __launch_bounds__(kBlock) void k_pass1_fixup(FastLists L, FT* __restr
                                                         long long N) {
   if (sc->active == 0.0) return;
   const unsigned n = sc->nslot1;
-  unsigned long long vb = 0;
   for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
     const long long c = L.slot_cell[s];
     if (L.slot_bad[s] & kSlotBad) {
@@ -656,32 +682,23 @@ __global__ __launch_bounds__(kBlock) void k_pass1_fixup(FastLists L, FT* __restr
       slot_to_hard(L, &sc->nhard1, s, c, flags[c], N);  // flags[c]: those of the projection
       continue;
     }
-    double v[10], s1[6], v6[6], vd[6];
+    double v[10], v6[6];
 #pragma unroll
     for (int t = 0; t < 10; ++t) v[t] = L.slot_v[(size_t)t * N + s];
-#pragma unroll
-    for (int t = 0; t < 6; ++t) s1[t] = L.slot_v[(size_t)(10 + t) * N + s];
     combine_planes(v, v6);
-    combine_vp(s1, v6, vd);
 #pragma unroll
-    for (int d = 0; d < 6; ++d) {
-      vp[(size_t)d * N + c] = (FT)vd[d];
-      const unsigned long long b = __double_as_longlong(fabs(vd[d]));
-      vb = b > vb ? b : vb;
-    }
+    for (int d = 0; d < 6; ++d) vp[(size_t)d * N + c] = (FT)v6[d];
     flags[c] = L.slot_fl[s];
   }
-  vb = wave_max_u64(vb);
-  if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
 }
 
-// The full pass 1 of the listed cells, in stages: projection, x, y and z wave speeds (with
-// their eigenvalue corrections, the z density clamp and re-projection), and the closure with
-// the fluxes.  Each stage is one kernel over the list, the cell's state kept in full_S
-// between stages: the same functions, on the same values, as pass1_cell, so the results are
-// the same bits, but each kernel holds one stage's registers instead of the whole path's,
-// which keeps its latency (the kernel's time on a short list) down.  P: the network's seeds
-// (learned mode) or null.
+// The full pass 1 of the listed cells, in stages: x, y and z wave speeds (with their
+// eigenvalue corrections, the z density clamp and re-projection) of the projected state; the
+// closure with the fluxes is k_pass1_close's, with every other cell.  Each stage is one
+// kernel over the list, the cell's state kept in full_S between stages: the same functions,
+// on the same values, as pass1_cell, so the results are the same bits, but each kernel
+// holds one stage's registers instead of the whole path's, which keeps its latency (the
+// kernel's time on a short list) down.  P: the network's seeds (learned mode) or null.
 __device__ __forceinline__ Warm list_warm(const float* P, long long c, long long N) {
   Warm w = no_warm();
   if (P != nullptr) {
@@ -813,10 +830,10 @@ __device__ void eig6_z_pair(const double* M, int flag2D, double Ma, bool positiv
 }
 // The list holds each cell's projected state (pass1a_cell, computed once by k_pass1a) and
 // the flags of that projection.  STAGE 1: x; 2: y; 3: z.
-template <int STAGE>
+template <int STAGE, typename FT>
 __global__ __launch_bounds__(kBlock, kFullWaves) void k_full1_stage(
     const float* __restrict__ P, FastLists L, DevScalars* sc, long long N, int flag2D, double Ma,
-    int positivity) {
+    int positivity, double* __restrict__ AMr, FT* __restrict__ vp, uint32_t* __restrict__ flags) {
   if (sc->active == 0.0) return;
   const ListSpread<2> ls(sc->nhard1);  // a lane pair per cell
   if (ls.idle()) return;  // wave-uniform
@@ -834,63 +851,37 @@ __global__ __launch_bounds__(kBlock, kFullWaves) void k_full1_stage(
     if (STAGE == 2) eig6_axis_pair<1>(a, flag2D, Ma, mn, mx, b, fl, w, true, half);
     if (STAGE == 3) eig6_z_pair(a, flag2D, Ma, positivity != 0, mn, mx, b, fl, w, half);
     if (half) continue;  // both lanes hold the same results
-    L.full_v[(size_t)(2 * (STAGE - 1)) * N + i] = mn;
-    L.full_v[(size_t)(2 * (STAGE - 1) + 1) * N + i] = mx;
+    if (STAGE < 3) {
+      L.full_v[(size_t)(2 * (STAGE - 1)) * N + i] = mn;
+      L.full_v[(size_t)(2 * (STAGE - 1) + 1) * N + i] = mx;
 #pragma unroll
-    for (int t = 0; t < 35; ++t) L.full_S[(size_t)t * N + i] = b[t];
-    L.full_fl[i] = fl;
-  }
-}
-// Last stage: closure and fluxes, 1D speeds, outputs.
-template <typename FT>
-__global__ __launch_bounds__(kBlock, kFullWaves) void k_full1_finish(
-    double* __restrict__ AMr, FT* __restrict__ F, FT* __restrict__ vp, uint32_t* __restrict__ flags,
-    FastLists L, DevScalars* sc, long long N, int flag2D, double Ma) {
-  if (sc->active == 0.0) return;
-  const ListSpread<> ls(sc->nhard1);
-  if (ls.idle()) return;
-  unsigned long long vb = 0;
-  for (unsigned i = ls.first(); i < ls.n; i += ls.stride) {
-    if (!ls.mine(i)) continue;
-    const long long c = L.cells[i];
-    double a[35], mr[35], s1[6], v6[6], v[6];
-#pragma unroll
-    for (int t = 0; t < 35; ++t) a[t] = L.full_S[(size_t)t * N + i];
-#pragma unroll
-    for (int t = 0; t < 6; ++t) v6[t] = L.full_v[(size_t)t * N + i];
-    uint32_t fl = L.full_fl[i];
-    flux_closure35<true, FT>(a, flag2D, Ma, mr, F + c, F + (size_t)35 * N + c,
-                             F + (size_t)70 * N + c, (size_t)N, fl);
-    closure1d_all(mr, s1);
-    combine_vp(s1, v6, v);
-#pragma unroll
-    for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = mr[t];
-#pragma unroll
-    for (int d = 0; d < 6; ++d) {
-      vp[(size_t)d * N + c] = (FT)v[d];
-      const unsigned long long bb = __double_as_longlong(fabs(v[d]));
-      vb = bb > vb ? bb : vb;
+      for (int t = 0; t < 35; ++t) L.full_S[(size_t)t * N + i] = b[t];
+      L.full_fl[i] = fl;
+      continue;
     }
+    // the cell's final state, speeds and flags, where k_pass1_close takes every cell's
+#pragma unroll
+    for (int d = 0; d < 4; ++d) vp[(size_t)d * N + c] = (FT)L.full_v[(size_t)d * N + i];
+    vp[(size_t)4 * N + c] = (FT)mn;
+    vp[(size_t)5 * N + c] = (FT)mx;
+#pragma unroll
+    for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = b[t];
     flags[c] = fl;
-    if (fl & F_NONFIN) atomicOr(&sc->err, 1ull);
   }
-  vb = wave_max_u64(vb);
-  if ((threadIdx.x & 63) == 0) atomicMax(&sc->vmax_bits, vb);
 }
-
-// The full pass 1's stages on `stream`.
+// The full pass 1's stages on `stream`, then the closure and fluxes of every cell.
 template <typename FT>
 static void enqueue_full1(hipStream_t stream, double* AMr, FT* F, FT* vp, uint32_t* flags,
                           const float* P, const FastLists& L, DevScalars* sc, long long N,
                           int flag2D, double Ma, int positivity) {
-  hipLaunchKernelGGL(k_full1_stage<1>, dim3(kListBlocks), dim3(kBlock), 0, stream, P, L, sc, N,
-                     flag2D, Ma, positivity);
-  hipLaunchKernelGGL(k_full1_stage<2>, dim3(kListBlocks), dim3(kBlock), 0, stream, P, L, sc, N,
-                     flag2D, Ma, positivity);
-  hipLaunchKernelGGL(k_full1_stage<3>, dim3(kListBlocks), dim3(kBlock), 0, stream, P, L, sc, N,
-                     flag2D, Ma, positivity);
-  hipLaunchKernelGGL(k_full1_finish<FT>, dim3(kListBlocks), dim3(kBlock), 0, stream, AMr, F, vp,
-                     flags, L, sc, N, flag2D, Ma);
+  hipLaunchKernelGGL((k_full1_stage<1, FT>), dim3(kListBlocks), dim3(kBlock), 0, stream, P, L, sc, N,
+                     flag2D, Ma, positivity, AMr, vp, flags);
+  hipLaunchKernelGGL((k_full1_stage<2, FT>), dim3(kListBlocks), dim3(kBlock), 0, stream, P, L, sc, N,
+                     flag2D, Ma, positivity, AMr, vp, flags);
+  hipLaunchKernelGGL((k_full1_stage<3, FT>), dim3(kListBlocks), dim3(kBlock), 0, stream, P, L, sc, N,
+                     flag2D, Ma, positivity, AMr, vp, flags);
+  hipLaunchKernelGGL(k_pass1_close<FT>, dim3((unsigned)((N + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                     stream, AMr, F, vp, flags, sc, N, flag2D, Ma);
 }
 
 // The fast modes' pass 2: projection and the x/y hyperbolicity checks (projected state to
@@ -1355,8 +1346,8 @@ static int enqueue_step(hyq_ctx* c, hipEvent_t* ev) {
                          c->flags, c->sc, c->N, p.flag2D, p.Ma, (const float*)nullptr);
     }
     const float* P = learned ? (const float*)c->P : (const float*)nullptr;
-    hipLaunchKernelGGL(k_pass1_fast<float>, dim3(g), dim3(kBlock), 0, c->stream, c->Mr, (float*)c->F,
-                       (float*)c->vp, c->flags, P, c->L, c->sc, c->N, p.flag2D, p.Ma, p.positivity);
+    hipLaunchKernelGGL(k_pass1_fast<float>, dim3(g), dim3(kBlock), 0, c->stream, (const double*)c->Mr,
+                       (float*)c->vp, c->flags, P, c->L, c->sc, c->N, p.positivity);
     hipLaunchKernelGGL(k_slot_solve<true>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
                        (const unsigned*)&c->sc->nslot1, &c->sc->nitem1, P, c->sc, c->N);
     hipLaunchKernelGGL(k_pass1_fixup<float>, dim3(kListBlocks), dim3(kBlock), 0, c->stream, c->L,
@@ -1701,8 +1692,8 @@ static void enqueue_fast_batch(const double* M, long long n, double Ma, int flag
   hipLaunchKernelGGL(k_pass1a<false>, dim3(g), dim3(kBlock), 0, 0, M, Mr, (short*)nullptr,
                      (unsigned*)nullptr, (unsigned*)nullptr, (float*)nullptr, flags, sc, n, flag2D,
                      Ma, (const float*)nullptr);
-  hipLaunchKernelGGL(k_pass1_fast<double>, dim3(g), dim3(kBlock), 0, 0, Mr, F, vp, flags,
-                     (const float*)nullptr, L, sc, n, flag2D, Ma, positivity);
+  hipLaunchKernelGGL(k_pass1_fast<double>, dim3(g), dim3(kBlock), 0, 0, (const double*)Mr, vp, flags,
+                     (const float*)nullptr, L, sc, n, positivity);
   hipLaunchKernelGGL(k_slot_solve<true>, dim3(kListBlocks), dim3(kBlock), 0, 0, L,
                      (const unsigned*)&sc->nslot1, &sc->nitem1, (const float*)nullptr, sc, n);
   hipLaunchKernelGGL(k_pass1_fixup<double>, dim3(kListBlocks), dim3(kBlock), 0, 0, L, vp, flags, sc, n);
