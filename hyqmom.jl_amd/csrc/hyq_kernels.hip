@@ -537,7 +537,7 @@ __device__ __forceinline__ unsigned new_slot(const FastLists& L, unsigned* count
 // vp, for k_pass1_close to combine with the 1D speeds; a cell with pending planes gets a
 // slot, one for the full path a list entry.  The closure and the fluxes of every cell are
 // k_pass1_close's, after the deferred cells: this kernel holds the plane solves' registers
-// only (no scratch at two waves per SIMD).
+// only (96 B of scratch per lane at two waves per SIMD, against 568 B with the closure).
 template <typename FT>
 __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass1_fast(
     const double* __restrict__ A, FT* __restrict__ vp, uint32_t* __restrict__ flags,

@@ -12,6 +12,7 @@
 // features and labels.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -261,6 +262,9 @@ __device__ __forceinline__ void wave_lds_fence() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
+// Waves that share one segment of the step's rows (a divisor of every launch shape's waves).
+constexpr int kSegWaves = 4;
+
 template <int WAVES, int RT, bool MASK, bool BIN, bool SEG = false>
 __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const void* __restrict__ Xv, long n,
                                                           const float* __restrict__ p,
@@ -271,10 +275,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const void* __restrict
                                                           int seg_rows) {
   // SEG: n rows in segments of seg_rows rows (a multiple of 16 RT), of which the first
   // seg[s] of segment s are rows to predict; row i's prediction to P[idx[i]].  Otherwise
-  // row i's prediction to P[i].
-  auto valid = [&](long row) {
-    return SEG ? (row % seg_rows) < (long)seg[row / seg_rows] : row < n;
-  };
+  // row i's prediction to P[i].  cnt: the row's segment's count (SEG).
+  auto valid = [&](long row, long cnt) { return SEG ? (row % seg_rows) < cnt : row < n; };
   const float* X = reinterpret_cast<const float*>(Xv);
   const s8* Xb = reinterpret_cast<const s8*>(Xv);  // BIN: row b is Xb[2 b], Xb[2 b + 1]
   __shared__ s8 wts[PK_N / 8];
@@ -309,42 +311,63 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const void* __restrict
     xsd[e] = k < 15 ? p[OFF_XSD + k] : 1.f;
   }
   // raw feature rows of the next group, loaded while the current one is computed: lane
-  // (r, q < 2) holds columns 8q..8q+7 of row 16u + r of the group, column 15 included
+  // (r, q < 2) holds columns 8q..8q+7 of row 16u + r of the group, column 15 included;
+  // SEG: with the destinations of the rows 16u + 4q + e whose outputs the lane writes
   float xn[RT][8];
   s8 bn[RT];
-  auto load_group = [&](long g) {
+  unsigned dn[RT][4];
+  auto load_group = [&](long g, long cnt) {
 #pragma unroll
     for (int u = 0; u < RT; ++u) {
       const long row = g * 16 * RT + 16 * u + r;
       if (BIN) {
-        bn[u] = (q < 2 && row < n && valid(row)) ? Xb[row * 2 + q] : s8{0, 0, 0, 0, 0, 0, 0, 0};
+        bn[u] = (q < 2 && row < n && valid(row, cnt)) ? Xb[row * 2 + q] : s8{0, 0, 0, 0, 0, 0, 0, 0};
       } else {
 #pragma unroll
         for (int e = 0; e < 8; ++e) xn[u][e] = (q < 2 && row < n) ? X[row * 16 + 8 * q + e] : 0.f;
       }
+      if (SEG) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const long ro = g * 16 * RT + 16 * u + 4 * q + e;
+          dn[u][e] = (r < 2 && ro < n && valid(ro, cnt)) ? idx[ro] : 0u;
+        }
+      }
     }
   };
   short* A = act[w];
-  // Group order.  Plain: group W, W + gstride, ... for wave W.  SEG: the groups that hold
-  // rows of segments W, W + gstride, ... (a segment's seg[s] rows fill its first
-  // ceil(seg[s] / (16 RT)) groups), so that a wave never visits a group past a count.
+  // Group order.  Plain: group W, W + gstride, ... for wave W.  SEG: kSegWaves waves share
+  // a segment: wave W takes groups j, j + kSegWaves, ... (j = W mod kSegWaves) of the groups
+  // that hold rows of segments W / kSegWaves, + sstride, ... (a segment's seg[s] rows fill
+  // its first ceil(seg[s] / (16 RT)) groups), so that a wave never visits a group past a
+  // count, and a full segment's groups are computed by several waves side by side.
+  constexpr long J = SEG ? kSegWaves : 1;
   const long wid = (long)blockIdx.x * WAVES + w;
+  const long sub = wid % J, sstride = gstride / J;
   const long gps = SEG ? seg_rows / (16 * RT) : 1;  // groups per segment
   const long nseg = SEG ? n / seg_rows : 0;
-  auto seg_groups = [&](long t) { return ((long)seg[t] + 16 * RT - 1) / (16 * RT); };
-  auto first_in = [&](long t, long& gc) {  // first group of the first segment t, t + gstride, ... with rows
+  // first group of this wave's in the first of segments t, t + sstride, ... that has one;
+  // gc, rc: that segment's groups and rows
+  auto first_in = [&](long t, long& gc, long& rc) {
     gc = 0;
-    while (t < nseg && (gc = seg_groups(t)) == 0) t += gstride;
-    return t < nseg ? t * gps : ngroups;
+    rc = 0;
+    while (t < nseg) {
+      rc = (long)seg[t];
+      gc = (rc + 16 * RT - 1) / (16 * RT);
+      if (sub < gc) break;
+      t += sstride;
+    }
+    return t < nseg ? t * gps + sub : ngroups;
   };
-  long gc = 0;  // SEG: groups of the current segment
-  long grp = SEG ? first_in(wid, gc) : wid;
-  long nxt_g = ngroups, nxt_c = 0;
-  if (grp < ngroups) load_group(grp);
-  for (; grp < ngroups; grp = nxt_g, gc = nxt_c) {
+  long gc = 0, rc = 0;  // SEG: groups and rows of the current group's segment
+  long grp = SEG ? first_in(wid / J, gc, rc) : wid;
+  long nxt_g = ngroups, nxt_c = 0, nxt_r = 0;
+  if (grp < ngroups) load_group(grp, rc);
+  for (; grp < ngroups; grp = nxt_g, gc = nxt_c, rc = nxt_r) {
     const long row0 = grp * 16 * RT;
     float xv[RT][8];
     s8 bv8[RT];
+    unsigned dv[RT][4];
 #pragma unroll
     for (int u = 0; u < RT; ++u) {
       if (BIN) {
@@ -353,14 +376,19 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const void* __restrict
 #pragma unroll
         for (int e = 0; e < 8; ++e) xv[u][e] = xn[u][e];
       }
+      if (SEG) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) dv[u][e] = dn[u][e];
+      }
     }
     if (SEG) {
       nxt_c = gc;
-      nxt_g = (grp % gps) + 1 < gc ? grp + 1 : first_in(grp / gps + gstride, nxt_c);
+      nxt_r = rc;
+      nxt_g = (grp % gps) + J < gc ? grp + J : first_in(grp / gps + sstride, nxt_c, nxt_r);
     } else {
       nxt_g = grp + gstride;
     }
-    if (nxt_g < ngroups) load_group(nxt_g);
+    if (nxt_g < ngroups) load_group(nxt_g, nxt_r);
     if (MASK) {
       // column 15 of row 16u + r sits in lane (r, q = 1), element 7
       bool need = false;
@@ -456,7 +484,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_fused(const void* __restrict
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const long row = row0 + 16 * u + 4 * q + e;
-          if (row < n && valid(row)) P[(SEG ? (long)idx[row] : row) * 2 + r] = (o[e] + b) * sd + mu;
+          if (row < n && valid(row, rc)) P[(SEG ? (long)dv[u][e] : row) * 2 + r] = (o[e] + b) * sd + mu;
         }
       }
     }
@@ -478,6 +506,8 @@ void launch_fused_t(int shape, const void* X, long n, const float* p, const shor
   const long groups = (n + 16 * rt - 1) / (16 * rt);
   long blocks = (groups + waves - 1) / waves;
   if (blocks > 2048) blocks = 2048;
+  // segments: kSegWaves waves for each (the grid's waves are a multiple of kSegWaves)
+  if (SEG) blocks = std::min<long>((n / seg_rows * kSegWaves + waves - 1) / waves, 1l << 16);
   const dim3 g((unsigned)blocks);
   if (shape == 0)
     hipLaunchKernelGGL((k_mlp_fused<16, 1, MASK, BIN, SEG>), g, dim3(64 * 16), 0, s, X, n, p, pk, P,
