@@ -182,9 +182,9 @@ HD void closure_emit(const ClosureState& q, double* Mr, FT* Fx, FT* Fy, FT* Fz, 
       constexpr E35T E = e35();
       const int i = E.e[n][0], j = E.e[n][1], k = E.e[n][2];
       Mr[n] = c[tpos(i, j, k)];
-      Fx[n * st] = (FT)c[tpos(i + 1, j, k)];
-      Fy[n * st] = (FT)c[tpos(i, j + 1, k)];
-      Fz[n * st] = (FT)c[tpos(i, j, k + 1)];
+      HYQ_STORE_STREAM(&Fx[n * st], (FT)c[tpos(i + 1, j, k)]);
+      HYQ_STORE_STREAM(&Fy[n * st], (FT)c[tpos(i, j + 1, k)]);
+      HYQ_STORE_STREAM(&Fz[n * st], (FT)c[tpos(i, j, k + 1)]);
     }
   } else {
     central_to_raw<4>(c, M000, um, vm, wm);

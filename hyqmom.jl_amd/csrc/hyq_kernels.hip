@@ -396,11 +396,11 @@ __global__ __launch_bounds__(kBlock, kProjWaves) void k_pass1a(const double* __r
   if (c >= N) return;  // whole waves past N never run; the last wave's lane 0 is inside
   double m[35], a[35];
 #pragma unroll
-  for (int t = 0; t < 35; ++t) m[t] = M[(size_t)t * N + c];
+  for (int t = 0; t < 35; ++t) m[t] = HYQ_LOAD_STREAM(&M[(size_t)t * N + c]);
   uint32_t fl = 0;
   pass1a_cell(m, flag2D, Ma, a, fl);
 #pragma unroll
-  for (int t = 0; t < 35; ++t) A[(size_t)t * N + c] = a[t];
+  for (int t = 0; t < 35; ++t) HYQ_STORE_STREAM(&A[(size_t)t * N + c], a[t]);
   flags[c] = fl;
   if (!FEAT) return;
   Plane pl[5];
@@ -595,7 +595,7 @@ __global__ __launch_bounds__(kBlock, kProjWaves) void k_pass1_close(
   if (c < N) {
     double a[35], mr[35], s1[6], v6[6], vd[6];
 #pragma unroll
-    for (int t = 0; t < 35; ++t) a[t] = AMr[(size_t)t * N + c];
+    for (int t = 0; t < 35; ++t) a[t] = HYQ_LOAD_STREAM(&AMr[(size_t)t * N + c]);
     uint32_t fl = flags[c];
     flux_closure35<true, FT>(a, flag2D, Ma, mr, F + c, F + (size_t)35 * N + c,
                              F + (size_t)70 * N + c, (size_t)N, fl);
@@ -610,7 +610,7 @@ __global__ __launch_bounds__(kBlock, kProjWaves) void k_pass1_close(
       vb = bb > vb ? bb : vb;
     }
 #pragma unroll
-    for (int t = 0; t < 35; ++t) AMr[(size_t)t * N + c] = mr[t];
+    for (int t = 0; t < 35; ++t) HYQ_STORE_STREAM(&AMr[(size_t)t * N + c], mr[t]);
     flags[c] = fl;
     if (fl & F_NONFIN) atomicOr(&sc->err, 1ull);
   }
@@ -898,13 +898,13 @@ __global__ __launch_bounds__(kBlock, kFastWaves) void k_pass2_check(const double
   if (c >= N) return;
   double m[35], a[35];
 #pragma unroll
-  for (int n = 0; n < 35; ++n) m[n] = M[(size_t)n * N + c];
+  for (int n = 0; n < 35; ++n) m[n] = HYQ_LOAD_STREAM(&M[(size_t)n * N + c]);
   uint32_t fl = 0;
   unsigned pend, known;
   const int st = pass2_check_planes<false>(m, flag2D, Ma, a, pend, fl, &known);
   if (st == kPlaneProven) {
 #pragma unroll
-    for (int n = 0; n < 35; ++n) A[(size_t)n * N + c] = a[n];
+    for (int n = 0; n < 35; ++n) HYQ_STORE_STREAM(&A[(size_t)n * N + c], a[n]);
     return;
   }
   A[c] = NAN;

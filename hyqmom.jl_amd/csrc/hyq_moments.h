@@ -24,6 +24,21 @@
 #else
 #define HYQ_SCHED_BARRIER() ((void)0)
 #endif
+// A store that the writing kernel never reads back, marked non-temporal on the device so
+// that it does not displace the lines the kernel still reads from the caches (a plain
+// store on the host).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define HYQ_STORE_STREAM(ptr, v) __builtin_nontemporal_store((v), (ptr))
+#else
+#define HYQ_STORE_STREAM(ptr, v) (*(ptr) = (v))
+#endif
+// A load of a value that the reading kernel reads once and no other lane reads again,
+// likewise marked non-temporal on the device.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define HYQ_LOAD_STREAM(ptr) __builtin_nontemporal_load(ptr)
+#else
+#define HYQ_LOAD_STREAM(ptr) (*(ptr))
+#endif
 
 namespace hyq {
 
