@@ -144,7 +144,7 @@ __global__ void k_dt(DevScalars* sc, double CFL, double hmin, double dtmax, doub
 
 // ---- HLL update (the line logic is in hyq_update.h) ----------------------------------
 // FT = double: FP64 fluxes and arithmetic; FT = float: the mixed-precision difference form.
-constexpr int kUpdBlock = 256;  // 128: 2-3% slower; 64: 5-9% slower (120x120x60)
+constexpr int kUpdBlock = 256;  // 128: 2-3% slower; 64: 5-9%; 512: 3% (120x120x60)
 HD inline unsigned update_grid(long long n) { return (unsigned)((n + kUpdBlock - 1) / kUpdBlock); }
 template <typename FT>
 __global__ __launch_bounds__(kUpdBlock) void k_update(const double* __restrict__ Mr,
